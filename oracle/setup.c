@@ -1,0 +1,460 @@
+/*
+ * setup.c -- oracle restatement of the host-side set-up math that feeds the
+ * 3D time loop (medium -> kernel coefficients, absorbers, sources, stations).
+ *
+ * TEST INFRASTRUCTURE ONLY (see swpc_oracle.h).  Always evaluated with the
+ * reference's single-precision semantics: real(SP) = float, and double where
+ * the reference promotes through PI (real(DP), src/shared/m_std.f90:11).
+ *
+ * Parity status:
+ *   orc_visco_set_relaxtime, orc_visco_constq_zeta, orc_momentrate (all six
+ *   STFs), orc_sdr2moment, orc_x2i/orc_i2x, orc_fdm_stable_dt
+ *       - pinned bit-for-bit against src/shared/m_fdtool.f90 compiled from the
+ *         reference sources (oracle/ref_fdtool) -> tests/golden/fdtool_*.
+ *   orc_kernel_coef (c1,c2,d1,d2)  - closed form of the tau-method recursion
+ *         (trapezoidal discretisation of the Zener ODE), tested in double.
+ *   orc_damping_profile            - closed form: the ADE recursion equals the
+ *         Crank-Nicolson step of the CFS-PML ODE; R0 formula by hand.
+ *   orc_relaxed_medium             - phase velocity at fq_ref recomputed in
+ *         complex128 equals the input velocity; complex rounding of the
+ *         Fortran runtime itself is parity unpinned (1-ulp level).
+ *   orc_surface_detection, orc_kbeg_a, orc_homogenize_absorber,
+ *   orc_vmodel_lhm, orc_vmodel_lgm, orc_cerjan_profile
+ *       - index/branch logic pinned by hand-built cases in tests/test_oracle_setup.py;
+ *         the Cerjan taper values by the closed form exp(-0.09 (1-d/L)^2).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#define PI_D 3.141592653589793238462643383279502884 /* atan(1.0_DP)*4 */
+#define HALO 3 /* cells on each side */
+#define OFF 2  /* storage offset of index 0: indices run -2..n+3 */
+
+static inline int64_t i3(int nx, int nz, int k, int i, int j)
+{
+    return ((int64_t)(j + OFF) * (nx + 2 * HALO) + (i + OFF)) * (nz + 2 * HALO) + (k + OFF);
+}
+static inline int64_t i2(int nx, int i, int j) { return (int64_t)(j + OFF) * (nx + 2 * HALO) + (i + OFF); }
+
+/* ---- index <-> coordinate maps, src/shared/m_fdtool.f90:600-649 ---- */
+int orc_x2i(float x, float xbeg, float dx) { return (int)ceilf((x - xbeg) / dx); }
+float orc_i2x(int i, float xbeg, float dx) { return xbeg + ((float)i - 0.5f) * dx; }
+
+/* ---- fdm_stable_dt, src/shared/m_fdtool.f90:81-96 ---- */
+float orc_fdm_stable_dt(float dx, float dy, float dz, float vmax)
+{
+    float hh = 1.0f / sqrtf(1 / (dx * dx) + 1 / (dy * dy) + 1 / (dz * dz));
+    float cc = 6.0f / 7.0f;
+    return cc * hh / vmax;
+}
+
+/* ---- visco_set_relaxtime, src/shared/m_fdtool.f90:691-728 ---- */
+void orc_visco_set_relaxtime(int nm, float fmin, float fmax, float *ts)
+{
+    float omega_a = (float)(2 * PI_D * (double)fmin);
+    float omega_b = (float)(2 * PI_D * (double)fmax);
+    if (nm == 0) return;
+    if (nm == 1) {
+        float omega = sqrtf(omega_a * omega_b);
+        ts[0] = 1.0f / omega;
+        return;
+    }
+    for (int im = 1; im <= nm; im++) {
+        double e = (double)(im - 1) / (double)(nm - 1);
+        float omega = (float)((double)omega_a * pow((double)(omega_b / omega_a), e));
+        ts[im - 1] = 1.0f / omega;
+    }
+}
+
+/* ---- visco_constq_zeta, src/shared/m_fdtool.f90:756-812 ---- */
+float orc_visco_constq_zeta(int nm, float fmin, float fmax, const float *ts)
+{
+    if (nm == 0) return 0.0f;
+    float om_a = (float)(2 * PI_D * (double)fmin);
+    float om_b = (float)(2 * PI_D * (double)fmax);
+    float i0[64], i1[64], i2m[64][64];
+    for (int im = 0; im < nm; im++) {
+        float t = ts[im];
+        i0[im] = (logf(1.0f + om_b * om_b * (t * t)) - logf(1.0f + om_a * om_a * (t * t))) / (2 * t);
+        i1[im] = ((atanf(om_b * t) - om_b * t / (1 + om_b * om_b * (t * t)))
+                  - (atanf(om_a * t) - om_a * t / (1 + om_a * om_a * (t * t)))) / (2 * t);
+    }
+    for (int im = 0; im < nm - 1; im++)
+        for (int km = im + 1; km < nm; km++) {
+            float wk1 = atanf(om_b * ts[im]) / ts[im] - atanf(om_b * ts[km]) / ts[km];
+            float wk2 = atanf(om_a * ts[im]) / ts[im] - atanf(om_a * ts[km]) / ts[km];
+            i2m[km][im] = ts[im] * ts[km] / (ts[km] * ts[km] - ts[im] * ts[im]) * (wk1 - wk2);
+        }
+    float i0sum = 0, i1sum = 0, i2sum = 0;
+    for (int im = 0; im < nm; im++) i0sum += i0[im];
+    for (int im = 0; im < nm; im++) i1sum += i1[im];
+    for (int im = 0; im < nm - 1; im++)
+        for (int km = im + 1; km < nm; km++) i2sum = i2sum + i2m[km][im];
+    return i0sum / (i1sum + 2 * i2sum);
+}
+
+/* ---- kernel__setup coefficients, src/swpc_3d/m_kernel.f90:58-67 ---- */
+void orc_kernel_coef(int nm, const float *ts, float dt, float *c1, float *c2, float *d1, float *d2)
+{
+    if (nm <= 0) { *d2 = 0.0f; return; }
+    for (int m = 0; m < nm; m++) {
+        c1[m] = (2 * ts[m] - dt) / (2 * ts[m] + dt);
+        c2[m] = 2.0f / (2 * ts[m] + dt) / (float)nm;
+    }
+    float acc = 0;
+    for (int m = 0; m < nm; m++) acc += dt / (2 * ts[m] - dt);
+    *d2 = acc / (float)nm;
+    for (int m = 0; m < nm; m++) d1[m] = 2 * ts[m] / (2 * ts[m] - dt);
+}
+
+/* ---- source time functions, src/shared/m_fdtool.f90:338-497 ---- */
+static float kupper(float t, float ts, float tr)
+{
+    if (ts <= t && t <= ts + tr) {
+        double s = sin(PI_D * (double)(t - ts) / (double)tr);
+        return (float)(3 * PI_D * (s * s * s) / (double)(4 * tr));
+    }
+    return 0.0f;
+}
+static float texp_stf(float t, float ts, float tr)
+{
+    if (ts <= t) {
+        float tt = t - ts;
+        return (float)((2 * PI_D) * (2 * PI_D) * (double)tt / (double)(tr * tr) * exp(-(2 * PI_D * (double)tt / (double)tr)));
+    }
+    return 0.0f;
+}
+static float cosine_stf(float t, float ts, float tr)
+{
+    if (ts <= t && t <= ts + tr) return (float)((1 - cos(2 * PI_D * (double)(t - ts) / (double)tr)) / (double)tr);
+    return 0.0f;
+}
+static float boxcar(float t, float ts, float tr)
+{
+    if (ts <= t && t <= ts + tr) return 1.0f / tr;
+    return 0.0f;
+}
+static float triangle(float t, float ts, float tr)
+{
+    if (ts <= t && t <= ts + tr / 2) return 4 * (t - ts) / (tr * tr);
+    if (ts + tr / 2 < t && t <= ts + tr) return -(4 * (t - ts - tr) / (tr * tr));
+    return 0.0f;
+}
+static float herrmann(float t, float ts, float tr)
+{
+    float t1 = ts + tr / 4, t2 = ts + 3 * tr / 4;
+    float tr3 = tr * tr * tr;
+    if (ts <= t && t < t1) return 16 * ((t - ts) * (t - ts)) / tr3;
+    if (t1 <= t && t < t2) return -(2 * (8 * (t * t + tr * ts + ts * ts - t * tr - 2 * t * ts) + tr * tr) / tr3);
+    if (t2 <= t && t <= ts + tr) return 16 * ((ts + tr - t) * (ts + tr - t)) / tr3;
+    return 0.0f;
+}
+/* stftype: 0 boxcar, 1 triangle, 2 herrmann, 3 kupper, 4 cosine, 5 texp */
+float orc_momentrate(float t, int stftype, float tbeg, float trise)
+{
+    switch (stftype) {
+    case 0: return boxcar(t, tbeg, trise);
+    case 1: return triangle(t, tbeg, trise);
+    case 2: return herrmann(t, tbeg, trise);
+    case 4: return cosine_stf(t, tbeg, trise);
+    case 5: return texp_stf(t, tbeg, trise);
+    default: return kupper(t, tbeg, trise);
+    }
+}
+
+/* STF table: stf[it-1][n] at t = tbeg + (it - shift) * dt (shift 0.5 for the
+   stress glut, m_source.f90:800; 0 for the body force, m_source.f90:873) */
+void orc_stf_table(int nt, int nsrc, int stftype, const float *t0, const float *tr, float tbeg, float dt,
+                   float shift, float *out)
+{
+    for (int it = 1; it <= nt; it++) {
+        float t = tbeg + ((float)it - shift) * dt;
+        for (int n = 0; n < nsrc; n++) out[(int64_t)(it - 1) * nsrc + n] = orc_momentrate(t, stftype, t0[n], tr[n]);
+    }
+}
+
+/* ---- sdr2moment, src/shared/m_fdtool.f90:307-336 ---- */
+static float d2r(float deg) { return (float)(PI_D / 180.0 * (double)deg); }
+void orc_sdr2moment(float strike, float dip, float rake, float *m6 /* xx yy zz yz xz xy */)
+{
+    float sind = sinf(d2r(dip)), cosd = cosf(d2r(dip));
+    float sin2d = sinf(d2r(2 * dip)), cos2d = cosf(d2r(2 * dip));
+    float sinl = sinf(d2r(rake)), cosl = cosf(d2r(rake));
+    float sinf_ = sinf(d2r(strike)), cosf_ = cosf(d2r(strike));
+    float sin2f = sinf(d2r(2 * strike)), cos2f = cosf(d2r(2 * strike));
+    m6[0] = -(sind * cosl * sin2f + sin2d * sinl * sinf_ * sinf_);
+    m6[5] = (sind * cosl * cos2f + sin2d * sinl * sin2f / 2);
+    m6[4] = -(cosd * cosl * cosf_ + cos2d * sinl * sinf_);
+    m6[1] = (sind * cosl * sin2f - sin2d * sinl * cosf_ * cosf_);
+    m6[3] = -(cosd * cosl * sinf_ - cos2d * sinl * cosf_);
+    m6[2] = (sin2d * sinl);
+}
+
+/* seismic_moment, src/shared/m_fdtool.f90:296-304 */
+float orc_seismic_moment(float mw) { return powf(10.0f, 1.5f * mw + 9.05f); }
+
+/* ---- vmodel_lhm (air column only), src/swpc_3d/m_vmodel_lhm.f90:20-158 ----
+   layers: nl rows of (depth, rho, vp, vs, qp, qs).  Fills every halo cell. */
+int orc_vmodel_lhm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int nl, const float *layers,
+                   float *rho, float *lam, float *mu, float *qp, float *qs)
+{
+    if (nl < 1 || nl > 64) return -1;
+    float dep[64], r0[64], vp0[64], vs0[64], q0p[64], q0s[64];
+    for (int l = 0; l < nl; l++) {
+        dep[l] = layers[6 * l + 0]; r0[l] = layers[6 * l + 1]; vp0[l] = layers[6 * l + 2];
+        vs0[l] = layers[6 * l + 3]; q0p[l] = layers[6 * l + 4]; q0s[l] = layers[6 * l + 5];
+    }
+    for (int l = nl - 2; l >= 0; l--) {
+        if ((vp0[l] < vcut || vs0[l] < vcut) && (vp0[l] > 0 && vs0[l] > 0)) {
+            vp0[l] = vp0[l + 1]; vs0[l] = vs0[l + 1]; r0[l] = r0[l + 1]; q0p[l] = q0p[l + 1]; q0s[l] = q0s[l + 1];
+        }
+    }
+    for (int k = -2; k <= nz + 3; k++) {
+        float zc = orc_i2x(k, zbeg, dz);
+        float vr, vl, vm, vqp, vqs;
+        if (zc < dep[0]) {
+            /* air (z < 0); the seawater branch (0 <= z < depth(1)) is not restated */
+            if (zc >= 0.0f) return -2;
+            vr = 0.001f; vm = 0.0f; vl = 0.0f; vqp = 10.0f; vqs = 10.0f;
+        } else {
+            float rho1 = 0, vp1 = 0, vs1 = 0, qp1 = 0, qs1 = 0;
+            for (int l = 0; l < nl; l++)
+                if (zc >= dep[l]) { rho1 = r0[l]; vp1 = vp0[l]; vs1 = vs0[l]; qp1 = q0p[l]; qs1 = q0s[l]; }
+            vr = rho1; vm = rho1 * vs1 * vs1; vl = rho1 * (vp1 * vp1 - 2 * vs1 * vs1); vqp = qp1; vqs = qs1;
+        }
+        for (int j = -2; j <= ny + 3; j++)
+            for (int i = -2; i <= nx + 3; i++) {
+                int64_t p = i3(nx, nz, k, i, j);
+                rho[p] = vr; lam[p] = vl; mu[p] = vm; qp[p] = vqp; qs[p] = vqs;
+            }
+    }
+    return 0;
+}
+
+/* ---- vmodel_lgm (linear-gradient layers, air column only),
+   src/swpc_3d/m_vmodel_lgm.f90:20-174 ---- */
+int orc_vmodel_lgm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int nl, const float *layers,
+                   float *rho, float *lam, float *mu, float *qp, float *qs)
+{
+    if (nl < 1 || nl > 64) return -1;
+    float dep[64], r0[64], vp0[64], vs0[64], q0p[64], q0s[64];
+    for (int l = 0; l < nl; l++) {
+        dep[l] = layers[6 * l + 0]; r0[l] = layers[6 * l + 1]; vp0[l] = layers[6 * l + 2];
+        vs0[l] = layers[6 * l + 3]; q0p[l] = layers[6 * l + 4]; q0s[l] = layers[6 * l + 5];
+    }
+    for (int l = nl - 2; l >= 0; l--) {
+        if ((vp0[l] < vcut || vs0[l] < vcut) && (vp0[l] > 0 && vs0[l] > 0)) {
+            vp0[l] = vp0[l + 1]; vs0[l] = vs0[l + 1]; r0[l] = r0[l + 1]; q0p[l] = q0p[l + 1]; q0s[l] = q0s[l + 1];
+        }
+    }
+    for (int k = -2; k <= nz + 3; k++) {
+        float zs = orc_i2x(k, zbeg, dz);
+        float rho1, vp1, vs1, qp1, qs1;
+        if (zs < dep[0]) {
+            if (zs >= 0.0f) return -2; /* seawater branch not restated */
+            rho1 = 0.001f; vp1 = 0.0f; vs1 = 0.0f; qp1 = 10.0f; qs1 = 10.0f;
+        } else {
+            rho1 = r0[nl - 1]; vp1 = vp0[nl - 1]; vs1 = vs0[nl - 1]; qp1 = q0p[nl - 1]; qs1 = q0s[nl - 1];
+            for (int l = 0; l < nl - 1; l++) {
+                if (dep[l] <= zs && zs < dep[l + 1]) {
+                    float h = dep[l + 1] - dep[l], d = zs - dep[l];
+                    rho1 = r0[l] + (r0[l + 1] - r0[l]) / h * d;
+                    vp1 = 1.0f * (vp0[l] + (vp0[l + 1] - vp0[l]) / h * d);
+                    vs1 = 1.0f * (vs0[l] + (vs0[l + 1] - vs0[l]) / h * d);
+                    qp1 = q0p[l] + (q0p[l + 1] - q0p[l]) / h * d;
+                    qs1 = q0s[l] + (q0s[l + 1] - q0s[l]) / h * d;
+                    break;
+                }
+            }
+        }
+        float vm = rho1 * vs1 * vs1, vl = rho1 * (vp1 * vp1 - 2 * vs1 * vs1);
+        for (int j = -2; j <= ny + 3; j++)
+            for (int i = -2; i <= nx + 3; i++) {
+                int64_t p = i3(nx, nz, k, i, j);
+                rho[p] = rho1; lam[p] = vl; mu[p] = vm; qp[p] = qp1; qs[p] = qs1;
+            }
+    }
+    return 0;
+}
+
+/* ---- absorber homogenisation, src/swpc_3d/m_medium.f90:124-190 ---- */
+void orc_homogenize_absorber(int nx, int ny, int nz, int na, float *rho, float *lam, float *mu, float *taup,
+                             float *taus)
+{
+    float *a[5] = {rho, lam, mu, taup, taus};
+    for (int f = 0; f < 5; f++) {
+        float *v = a[f];
+        for (int i = -2; i <= na; i++)
+            for (int j = -2; j <= ny + 3; j++)
+                for (int k = -2; k <= nz + 3; k++) v[i3(nx, nz, k, i, j)] = v[i3(nx, nz, k, na + 1, j)];
+        for (int i = nx - na + 1; i <= nx + 3; i++)
+            for (int j = -2; j <= ny + 3; j++)
+                for (int k = -2; k <= nz + 3; k++) v[i3(nx, nz, k, i, j)] = v[i3(nx, nz, k, nx - na, j)];
+        for (int j = -2; j <= na; j++)
+            for (int i = -2; i <= nx + 3; i++)
+                for (int k = -2; k <= nz + 3; k++) v[i3(nx, nz, k, i, j)] = v[i3(nx, nz, k, i, na + 1)];
+        for (int j = ny - na + 1; j <= ny + 3; j++)
+            for (int i = -2; i <= nx + 3; i++)
+                for (int k = -2; k <= nz + 3; k++) v[i3(nx, nz, k, i, j)] = v[i3(nx, nz, k, i, ny - na)];
+        for (int j = -2; j <= ny + 3; j++)
+            for (int i = -2; i <= nx + 3; i++)
+                for (int k = nz - na + 1; k <= nz + 3; k++) v[i3(nx, nz, k, i, j)] = v[i3(nx, nz, nz - na, i, j)];
+    }
+}
+
+/* ---- tau-method relaxation times from Q, src/swpc_3d/m_medium.f90:193-207 ---- */
+void orc_q_to_tau(int64_t n, int nm, float zeta, float *qp_to_taup, float *qs_to_taus)
+{
+    for (int64_t p = 0; p < n; p++) {
+        qp_to_taup[p] = (float)nm * zeta / qp_to_taup[p];
+        qs_to_taus[p] = (float)nm * zeta / qs_to_taus[p];
+    }
+}
+
+/* ---- relaxed_medium, src/swpc_3d/m_medium.f90:231-271 ---- */
+void orc_relaxed_medium(int64_t n, int nm, const float *ts, float fq_ref, float *lam, float *mu, const float *taup,
+                        const float *taus)
+{
+    if (nm == 0) return;
+    float omega = (float)(2 * PI_D * (double)fq_ref);
+    float complex cc = 0.0f;
+    for (int im = 0; im < nm; im++) {
+        double w = (double)omega * (double)ts[im];
+        double complex z = (I * w) / (1.0 - I * w);
+        cc = cc + (float complex)z;
+    }
+    cc = cc / (float)nm;
+    for (int64_t p = 0; p < n; p++) {
+        float rho_beta2 = mu[p];
+        float rho_alpha2 = lam[p] + 2 * mu[p];
+        float chi_mu = 1.0f / crealf(1.0f / csqrtf(1.0f - cc * taus[p]));
+        float chi_lam = 1.0f / crealf(1.0f / csqrtf(1.0f - cc * taup[p]));
+        mu[p] = rho_beta2 / (chi_mu * chi_mu);
+        lam[p] = rho_alpha2 / (chi_lam * chi_lam) - 2 * mu[p];
+    }
+}
+
+/* ---- surface_detection, src/swpc_3d/m_medium.f90:339-394.  kfs/kob are set
+   on (0:nx+2, 0:ny+2) and 0 elsewhere; band maps on (1:nx, 1:ny). ---- */
+void orc_surface_detection(int nx, int ny, int nz, const float *lam, const float *mu, int32_t *kfs, int32_t *kob,
+                           int32_t *kfs_top, int32_t *kfs_bot, int32_t *kob_top, int32_t *kob_bot)
+{
+    const float epsl = 1.1920928955078125e-07f;
+    const int64_t n2 = (int64_t)(nx + 2 * HALO) * (ny + 2 * HALO);
+    for (int64_t p = 0; p < n2; p++) { kfs[p] = 0; kob[p] = 0; kfs_top[p] = 0; kfs_bot[p] = 0; kob_top[p] = 0; kob_bot[p] = 0; }
+    for (int j = 0; j <= ny + 2; j++)
+        for (int i = 0; i <= nx + 2; i++)
+            for (int k = 1; k <= nz - 1; k++) {
+                if (fabsf(mu[i3(nx, nz, k, i, j)]) < epsl && fabsf(mu[i3(nx, nz, k + 1, i, j)]) > epsl) kob[i2(nx, i, j)] = k;
+                if (fabsf(lam[i3(nx, nz, k, i, j)]) < epsl && fabsf(lam[i3(nx, nz, k + 1, i, j)]) > epsl) kfs[i2(nx, i, j)] = k;
+            }
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            int fmin = 1 << 30, fmax = -(1 << 30), omin = 1 << 30, omax = -(1 << 30);
+            for (int jj = j - 2; jj <= j + 3; jj++)
+                for (int ii = i - 2; ii <= i + 3; ii++) {
+                    int f = kfs[i2(nx, ii, jj)], o = kob[i2(nx, ii, jj)];
+                    if (f < fmin) fmin = f;
+                    if (f > fmax) fmax = f;
+                    if (o < omin) omin = o;
+                    if (o > omax) omax = o;
+                }
+            kfs_top[i2(nx, i, j)] = fmin - 2 > 1 ? fmin - 2 : 1;
+            kfs_bot[i2(nx, i, j)] = fmax + 2 < nz ? fmax + 2 : nz;
+            kob_top[i2(nx, i, j)] = omin - 2 > 1 ? omin - 2 : 1;
+            kob_bot[i2(nx, i, j)] = omax + 2 < nz ? omax + 2 : nz;
+        }
+}
+
+/* ---- kbeg_a, src/swpc_3d/m_global.f90:334-343 (PML columns) ---- */
+void orc_kbeg_a(int nx, int ny, int nz, int na, int32_t *kbeg_a)
+{
+    for (int j = -2; j <= ny + 3; j++)
+        for (int i = -2; i <= nx + 3; i++)
+            kbeg_a[i2(nx, i, j)] = (i <= na || nx - na + 1 <= i || j <= na || ny - na + 1 <= j) ? 1 : nz - na + 1;
+}
+
+/* ---- velocity_minmax, src/swpc_3d/m_medium.f90:396-427 ---- */
+void orc_velocity_minmax(int nx, int ny, int nz, const float *rho, const float *lam, const float *mu,
+                         const int32_t *kfs, float *vmin, float *vmax)
+{
+    float vmax1 = -1, vmin1 = 1e30f;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++)
+            for (int k = kfs[i2(nx, i, j)] + 1; k <= nz; k++) {
+                int64_t p = i3(nx, nz, k, i, j);
+                float vp = sqrtf((lam[p] + 2 * mu[p]) / rho[p]);
+                float vs = sqrtf(mu[p] / rho[p]);
+                if (vp > vmax1) vmax1 = vp;
+                if (vs < 1.1920928955078125e-07f) continue;
+                if (vs < vmin1) vmin1 = vs;
+            }
+    *vmin = vmin1;
+    *vmax = vmax1;
+}
+
+/* ---- damping_profile, src/swpc_3d/m_absorb_p.f90:533-573 ---- */
+void orc_damping_profile(float x, float H, float xbeg0, float xend0, int na, float fcut, float dt, float *g)
+{
+    const int pd = 1, pa = 1, pb = 2;
+    const float cp = 6.0f;
+    (void)pa;
+    float R0 = powf(10.0f, -((log10f((float)na) - 1) / log10f(2.0f)) - 3.0f);
+    float d0 = -((1.0f / (2.0f * H)) * (float)(pd + 1) * cp * logf(R0));
+    float b0 = 7.0f;
+    float a0 = (float)(PI_D * (double)fcut);
+    float xx;
+    if (x <= xbeg0 + H) xx = (xbeg0 + H) - x;
+    else if (x >= xend0 - H) xx = x - (xend0 - H);
+    else xx = 0.0f;
+    float r = fabsf(xx / H);
+    float d = d0 * r;
+    float a = a0 * (1.0f - r);
+    float b = 1.0f + (b0 - 1.0f) * (r * r);
+    (void)pb;
+    float den = 1.0f + (dt / 2.0f) * (a + d / b);
+    g[0] = ((1.0f + (dt / 2.0f) * a) / b) / den;
+    g[1] = (-1.0f / b) / den;
+    g[2] = (1.0f - (dt / 2.0f) * (a + d / b)) / den;
+    g[3] = (d / b) / den;
+}
+
+/* profiles gxc, gxe for cells 1..n: centre x(i) and edge x(i)+h/2
+   (src/swpc_3d/m_absorb_p.f90:79-93).  out_c/out_e are (4, n). */
+void orc_pml_profile_axis(int n, int na, float beg, float h, float fcut, float dt, float *out_c, float *out_e)
+{
+    float H = (float)na * h;
+    float end = beg + (float)n * h;
+    for (int i = 1; i <= n; i++) {
+        float xc = orc_i2x(i, beg, h);
+        orc_damping_profile(xc, H, beg, end, na, fcut, dt, &out_c[4 * (i - 1)]);
+        orc_damping_profile(xc + h / 2.0f, H, beg, end, na, fcut, dt, &out_e[4 * (i - 1)]);
+    }
+}
+
+/* ---- Cerjan sponge, src/swpc_3d/m_absorb_c.f90:27-101.  Arrays carry the
+   3-cell halo (index i + 3).  top != 0 tapers the low side (x and y);
+   z uses top = 0 (no taper at the free surface). ---- */
+void orc_cerjan_profile_axis(int n, int na, float h, int top, float *g_c, float *g_b)
+{
+    const float alpha = 0.09f;
+    float L = (float)na * h;
+    for (int i = -2; i <= n + 3; i++) { g_c[i + OFF] = 1.0f; g_b[i + OFF] = 1.0f; }
+    for (int i = 1; i <= n; i++) {
+        float yc, yb;
+        if (i <= na) {
+            if (!top) continue;
+            yc = 1.0f - (orc_i2x(i, 0.0f, h)) / L;
+            yb = 1.0f - ((orc_i2x(i, 0.0f, h) + h / 2)) / L;
+        } else if (i >= n - na + 1) {
+            yc = 1.0f - (orc_i2x(i, (float)n * h, -h) + h / 2) / L;
+            yb = 1.0f - ((orc_i2x(i, (float)n * h, -h))) / L;
+        } else {
+            continue;
+        }
+        g_c[i + OFF] = expf(-(alpha * (yc * yc)));
+        g_b[i + OFF] = expf(-(alpha * (yb * yb)));
+    }
+}

@@ -1,0 +1,98 @@
+/*
+ * swpc_oracle.h -- CPU restatement of OpenSWPC's 3D velocity-stress time loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle: the checker that the HIP
+ * library is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in openswpc_amd/ links,
+ * imports or executes anything under oracle/.
+ *
+ * Layout follows the reference exactly so that every loop can be read side
+ * by side with the Fortran: 3D arrays are (k,i,j) with k (depth) fastest and
+ * a 3-cell halo on every side (src/swpc_3d/m_global.f90:295-300), memory
+ * variables are (m,k,i,j) over the kernel box (src/swpc_3d/m_kernel.f90:392-397),
+ * (i,j) maps carry the same halo, PML profiles are g(4,n) with n 1-based
+ * (src/swpc_3d/m_absorb_p.f90:75-77).
+ *
+ * Precision variants (ORC_PREC, one shared object each):
+ *   32  -> the reference FP32 build (MP = SP at src/swpc_3d/m_global.f90:30)
+ *   16  -> the reference default "mixed" build (MP = DP; V, S and spatial
+ *          derivatives in double, memory variables / medium / PML auxiliaries
+ *          in single)
+ *   64  -> everything in double: the same discrete scheme in FP64, used to
+ *          state the rounding error of the FP32 runs.
+ * Set-up math (setup.c) is always evaluated with the reference's single
+ * precision semantics so that every variant integrates identical inputs.
+ *
+ * Parity status of each function is listed in the header of setup.c / step.c.
+ */
+#ifndef SWPC_ORACLE_H
+#define SWPC_ORACLE_H
+
+#include <stdint.h>
+
+#ifndef ORC_PREC
+#define ORC_PREC 32
+#endif
+
+#if ORC_PREC == 32
+typedef float mp_t;   /* real(MP)                  */
+typedef float sp_t;   /* real(SP)                  */
+typedef float dr_t;   /* result of real() intrinsic */
+#elif ORC_PREC == 16
+typedef double mp_t;
+typedef float sp_t;
+typedef float dr_t;
+#elif ORC_PREC == 64
+typedef double mp_t;
+typedef double sp_t;
+typedef double dr_t;
+#else
+#error "ORC_PREC must be 32, 16 or 64"
+#endif
+
+#define ORC_HALO 3 /* cells on each side: indices -2..n+3 (m_global.f90:295-300) */
+#define ORC_OFF 2  /* storage offset of index 0 */
+
+typedef struct orc_state {
+    /* grid (m_global.f90:52-58,82-86) */
+    int nx, ny, nz, na, nm;
+    int abc;                      /* 0 = 'pml', 1 = 'cerjan'           */
+    int ibeg_k, iend_k, jbeg_k, jend_k, kbeg_k, kend_k;
+    int kbeg_min;                 /* m_absorb_p.f90:96                 */
+    double dx, dy, dz;            /* given in double, rounded to mp_t */
+    float dt;
+
+    /* 3D fields, (k,i,j) halo 3 */
+    mp_t *Vx, *Vy, *Vz;
+    mp_t *Sxx, *Syy, *Szz, *Syz, *Sxz, *Sxy;
+    /* memory variables (m,k,i,j) on the kernel box */
+    sp_t *Rxx, *Ryy, *Rzz, *Ryz, *Rxz, *Rxy;
+    /* medium (k,i,j) halo 3 */
+    sp_t *rho, *lam, *mu, *taup, *taus;
+    /* (i,j) maps, halo 3 */
+    int32_t *kfs_top, *kfs_bot, *kob_top, *kob_bot, *kbeg_a, *kob;
+    /* viscoelastic coefficients (m_kernel.f90:23-24) */
+    sp_t *c1, *c2, *d1;
+    sp_t d2;
+    /* ADE-CFS PML: profiles g(4,n) and 18 auxiliaries on
+       (kbeg_min:nz, 1:nx, 1:ny)  (m_absorb_p.f90:43-52,99-116) */
+    sp_t *gxc, *gxe, *gyc, *gye, *gzc, *gze;
+    sp_t *axVx, *ayVx, *azVx, *axVy, *ayVy, *azVy, *axVz, *ayVz, *azVz;
+    sp_t *axSxx, *aySxy, *azSxz, *axSxy, *aySyy, *azSyz, *axSxz, *aySyz, *azSzz;
+    /* Cerjan sponge profiles, halo 3 (m_absorb_c.f90:22-23) */
+    sp_t *cgx_c, *cgx_b, *cgy_c, *cgy_b, *cgz_c, *cgz_b;
+} orc_state;
+
+/* index helpers */
+static inline int64_t orc_i3(const orc_state *s, int k, int i, int j)
+{
+    int64_t nzm = s->nz + 2 * ORC_HALO, nxm = s->nx + 2 * ORC_HALO;
+    return ((int64_t)(j + ORC_OFF) * nxm + (i + ORC_OFF)) * nzm + (k + ORC_OFF);
+}
+static inline int64_t orc_i2(const orc_state *s, int i, int j)
+{
+    int64_t nxm = s->nx + 2 * ORC_HALO;
+    return (int64_t)(j + ORC_OFF) * nxm + (i + ORC_OFF);
+}
+
+#endif
