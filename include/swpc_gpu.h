@@ -1,0 +1,165 @@
+/*
+ * swpc_gpu.h -- C ABI of libswpc_gpu, the MI355X (gfx950) implementation of
+ * OpenSWPC's 3D velocity-stress time loop.
+ *
+ * The reference exposes this path as argument-less Fortran module routines
+ * sharing module globals.  Each entry point below replaces one of them and
+ * says which (file:line in the reference tree).  A Fortran caller binds them
+ * with iso_c_binding interfaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative SWPC_E* code on
+ *     failure; swpc_last_error() gives the text.  The library never aborts.
+ *   - Host arrays passed in use the reference layout: 3D arrays are (k,i,j)
+ *     with k fastest and a 3-cell halo on every side (indices -2..n+3,
+ *     src/swpc_3d/m_global.f90:295-300); (i,j) maps carry the same halo.
+ *     The library keeps its own x-fastest layout on the device.
+ *   - Arithmetic is IEEE FP32 in the reference's expression order (the FP32
+ *     build, MP = SP at src/swpc_3d/m_global.f90:30).
+ *   - A context is used by one host thread at a time.  Calls are ordered on
+ *     the context's HIP stream; swpc_get_* calls synchronise it.
+ */
+#ifndef SWPC_GPU_H
+#define SWPC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWPC_OK 0
+#define SWPC_EINVAL -1  /* bad argument                      */
+#define SWPC_ESTATE -2  /* call out of order / missing setup  */
+#define SWPC_EHIP -3    /* HIP runtime error                  */
+#define SWPC_ENOMEM -4  /* device allocation failed           */
+
+#define SWPC_ABC_PML 0    /* abc_type = 'pml'    */
+#define SWPC_ABC_CERJAN 1 /* abc_type = 'cerjan' */
+
+/* field ids for swpc_get_field / swpc_set_field */
+enum swpc_field {
+    SWPC_VX = 0, SWPC_VY, SWPC_VZ,
+    SWPC_SXX, SWPC_SYY, SWPC_SZZ, SWPC_SYZ, SWPC_SXZ, SWPC_SXY,
+    SWPC_NFIELD
+};
+
+/* receiver output kinds (bit flags, m_wav.f90:29-33 sw_wav_*) */
+#define SWPC_WAV_V 1
+#define SWPC_WAV_U 2
+#define SWPC_WAV_STRESS 4
+#define SWPC_WAV_STRAIN 8
+
+typedef struct swpc_ctx swpc_ctx;
+
+typedef struct swpc_desc {
+    int nx, ny, nz;     /* cells (m_global.f90:52)                    */
+    int na;             /* absorber thickness (m_global.f90:82)       */
+    int nm;             /* Zener mechanisms, 0..8 (m_global.f90:31)   */
+    int abc;            /* SWPC_ABC_PML or SWPC_ABC_CERJAN            */
+    double dx, dy, dz;  /* grid widths [km], rounded to float         */
+    float dt;           /* time step [s]                              */
+    int device;         /* HIP device ordinal                         */
+} swpc_desc;
+
+/* Context: allocates all device state (fields, memory variables, PML
+   auxiliaries) zero-initialised.  Replaces the allocations of
+   kernel__setup/memory_allocate (src/swpc_3d/m_kernel.f90:30-73,376-400)
+   and absorb_p__setup (src/swpc_3d/m_absorb_p.f90:95-116). */
+int swpc_create(const swpc_desc *desc, swpc_ctx **out);
+int swpc_destroy(swpc_ctx *ctx);
+const char *swpc_last_error(const swpc_ctx *ctx);
+
+/* Medium and free-surface bands: the arrays medium__setup leaves in module
+   m_global (src/swpc_3d/m_medium.f90:36-229): rho, relaxed lam/mu,
+   tau-method taup/taus, and the 2nd-order band limits kfs_top..kob_bot
+   plus kob (used by kernel__vmax). */
+int swpc_set_medium(swpc_ctx *ctx, const float *rho, const float *lam, const float *mu, const float *taup,
+                    const float *taus, const int32_t *kfs_top, const int32_t *kfs_bot, const int32_t *kob_top,
+                    const int32_t *kob_bot, const int32_t *kob);
+
+/* Zener recursion coefficients c1, c2, d1 (nm each) and d2
+   (src/swpc_3d/m_kernel.f90:58-67). */
+int swpc_set_visco(swpc_ctx *ctx, const float *c1, const float *c2, const float *d1, float d2);
+
+/* ADE-CFS PML profiles g(4,n) at cell centre and edge, n = nx, ny, nz
+   (src/swpc_3d/m_absorb_p.f90:74-93). */
+int swpc_set_pml(swpc_ctx *ctx, const float *gxc, const float *gxe, const float *gyc, const float *gye,
+                 const float *gzc, const float *gze);
+
+/* Cerjan sponge profiles with the 3-cell halo (n+6 each)
+   (src/swpc_3d/m_absorb_c.f90:27-101). */
+int swpc_set_cerjan(swpc_ctx *ctx, const float *gx_c, const float *gx_b, const float *gy_c, const float *gy_b,
+                    const float *gz_c, const float *gz_b);
+
+/* Sources (source__setup, src/swpc_3d/m_source.f90:41-314, after the
+   reference's own host set-up).  bf_mode = 0: moment tensors, comp is
+   (6, nsrc) = mxx myy mzz myz mxz mxy per source, mo the seismic moment of
+   each source; bf_mode = 1: body forces, comp is (3, nsrc) = fx fy fz and mo
+   is ignored.  M0 is the normalisation (total moment or force norm).  stf is
+   (nt, nsrc): the moment rate of each source at t = tbeg + (it - 1/2) dt for
+   moment tensors (m_source.f90:800) or t = tbeg + it dt for body forces
+   (m_source.f90:873).  Sources must lie inside [1,nx]x[1,ny]x[1,nz]. */
+int swpc_set_sources(swpc_ctx *ctx, int nsrc, int bf_mode, const int32_t *isrc, const int32_t *jsrc,
+                     const int32_t *ksrc, const float *mo, const float *comp, float M0, int nt, const float *stf);
+
+/* Receivers (wav__setup, src/swpc_3d/m_wav.f90:46-137 and set_stinfo
+   :141-271, after station mapping on the host).  what = SWPC_WAV_* flags,
+   ntw = samples per trace.  M0 and UC are the output scales of
+   m_wav.f90:517-607. */
+int swpc_set_receivers(swpc_ctx *ctx, int nst, const int32_t *ist, const int32_t *jst, const int32_t *kst,
+                       int what, int ntdec_w, int ntw, float M0, float UC);
+
+/* ---- one time step, fine grained (reference routine for routine) ---- */
+/* kernel__update_stress (src/swpc_3d/m_kernel.f90:142-348) */
+int swpc_kernel_update_stress(swpc_ctx *ctx);
+/* absorb__update_stress (src/swpc_3d/m_absorb.f90:60-75) */
+int swpc_absorb_update_stress(swpc_ctx *ctx);
+/* source__stressglut (src/swpc_3d/m_source.f90:776-848) */
+int swpc_source_stressglut(swpc_ctx *ctx, int it);
+/* kernel__update_vel (src/swpc_3d/m_kernel.f90:75-140) */
+int swpc_kernel_update_vel(swpc_ctx *ctx);
+/* absorb__update_vel (src/swpc_3d/m_absorb.f90:43-58) */
+int swpc_absorb_update_vel(swpc_ctx *ctx);
+/* source__bodyforce (src/swpc_3d/m_source.f90:850-898) */
+int swpc_source_bodyforce(swpc_ctx *ctx, int it);
+/* wav__store (src/swpc_3d/m_wav.f90:397-625) */
+int swpc_wav_store(swpc_ctx *ctx, int it);
+
+/* nsteps whole steps it0 .. it0+nsteps-1 in the order of the main loop
+   (src/swpc_3d/main.f90:119-139): wav__store, stress, glut, vel,
+   body force.  With use_graph != 0 the steps run as replays of a captured
+   HIP graph of `block` steps (falls back to direct launches for the tail). */
+int swpc_step(swpc_ctx *ctx, int it0, int nsteps, int use_graph, int block);
+
+/* kernel__vmax (src/swpc_3d/m_kernel.f90:350-374): max |V| one cell below
+   the ocean bottom, 5 cells inside the absorber. Synchronises. */
+int swpc_kernel_vmax(swpc_ctx *ctx, float out[3]);
+
+/* Field transfer in the reference (k,i,j) halo-3 layout, FP32. */
+int swpc_get_field(swpc_ctx *ctx, int field, float *out);
+int swpc_set_field(swpc_ctx *ctx, int field, const float *in);
+
+/* Seismograms (ntw, ncmp, nst) like wav_vel/wav_disp/wav_stress/wav_strain
+   (src/swpc_3d/m_wav.f90:37-38).  kind = one SWPC_WAV_* flag. */
+int swpc_get_seismograms(swpc_ctx *ctx, int kind, float *out);
+
+/* Wait for all work queued on the context. */
+int swpc_synchronize(swpc_ctx *ctx);
+
+/* Device bytes held by the context (for reporting). */
+int64_t swpc_device_bytes(const swpc_ctx *ctx);
+
+/* Launch-shape tuning knobs (0 = keep current).  kchunk: k-planes marched
+   per thread block in the interior kernels. */
+int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by);
+
+/* The HIP stream the context launches on (hipStream_t as void*), so a
+   caller can time or order work against it. */
+void *swpc_stream(swpc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWPC_GPU_H */

@@ -1,0 +1,653 @@
+// api.hip -- C ABI of libswpc_gpu (declared in include/swpc_gpu.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "swpc_internal.h"
+
+static int fail(swpc_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) return fail((c), SWPC_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+static hipError_t dalloc(swpc_ctx *c, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return e;
+    c->bytes += (long long)bytes;
+    return hipMemsetAsync(*p, 0, bytes, c->stream);
+}
+
+static void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+static void reset_graph(swpc_ctx *c)
+{
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->graph_block = 0;
+}
+
+extern "C" const char *swpc_last_error(const swpc_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+extern "C" int64_t swpc_device_bytes(const swpc_ctx *ctx) { return ctx ? ctx->bytes : 0; }
+
+extern "C" void *swpc_stream(swpc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
+{
+    if (!d || !out) return SWPC_EINVAL;
+    *out = nullptr;
+    if (d->nx < 4 || d->ny < 4 || d->nz < 4 || d->na < 0 || d->nm < 0 || d->nm > SWPC_MAXNM)
+        return SWPC_EINVAL;
+    if (!(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0) || !(d->dt > 0)) return SWPC_EINVAL;
+    if (d->abc != SWPC_ABC_PML && d->abc != SWPC_ABC_CERJAN) return SWPC_EINVAL;
+    if (d->abc == SWPC_ABC_PML && (d->nx <= 2 * d->na || d->ny <= 2 * d->na || d->nz <= d->na)) return SWPC_EINVAL;
+
+    swpc_ctx *c = new swpc_ctx();
+    memset(c, 0, sizeof(*c));
+    c->desc = *d;
+    c->device = d->device;
+    c->kchunk = 64;
+    c->by = 4;
+    if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return SWPC_EHIP;
+    }
+    Grid &g = c->g;
+    g.nx = d->nx; g.ny = d->ny; g.nz = d->nz; g.na = d->na; g.nm = d->nm;
+    // first interior column i = na+1 on a 128-byte boundary; at least 3 halo cells below i = 1
+    g.ox = (int)((32 - (d->na % 32)) % 32);
+    while (g.ox < 3) g.ox += 32;
+    g.pitch = (int)round_up(g.ox + d->nx + 3, 32);
+    g.nyp = d->ny + 6;
+    g.nzp = d->nz + 6;
+    g.plane = (long long)g.pitch * g.nyp;
+    g.n = g.plane * g.nzp;
+    if (d->abc == SWPC_ABC_PML) {
+        g.ib = d->na + 1; g.ie = d->nx - d->na;
+        g.jb = d->na + 1; g.je = d->ny - d->na;
+        g.kb = 1; g.ke = d->nz - d->na;
+    } else {
+        g.ib = 1; g.ie = d->nx; g.jb = 1; g.je = d->ny; g.kb = 1; g.ke = d->nz;
+    }
+    g.nxk = g.ie - g.ib + 1;
+    g.nyk = g.je - g.jb + 1;
+    g.nzk = g.ke - g.kb + 1;
+    g.rpitch = (int)round_up(g.nxk, 32);
+    g.rplane = (long long)g.rpitch * g.nyk;
+    g.rn = g.rplane * g.nzk;
+
+    int rc = SWPC_OK;
+    const size_t fb = (size_t)g.n * sizeof(float);
+    const size_t mb = (size_t)g.plane * sizeof(int);
+#define ALLOC(ptr, bytes)                                                                   \
+    do {                                                                                    \
+        if (dalloc(c, (void **)&(ptr), (bytes)) != hipSuccess) { rc = SWPC_ENOMEM; goto bad; } \
+    } while (0)
+    for (int q = 0; q < SWPC_NFIELD; ++q) ALLOC(c->fields[q], fb);
+    ALLOC(c->rho, fb);
+    ALLOC(c->lam, fb);
+    ALLOC(c->mu, fb);
+    ALLOC(c->taup, fb);
+    ALLOC(c->taus, fb);
+    ALLOC(c->kfs_top, mb);
+    ALLOC(c->kfs_bot, mb);
+    ALLOC(c->kob_top, mb);
+    ALLOC(c->kob_bot, mb);
+    ALLOC(c->kob, mb);
+    if (d->nm > 0) ALLOC(c->R, (size_t)6 * d->nm * g.rn * sizeof(float));
+    if (d->abc == SWPC_ABC_PML && d->na > 0) {
+        const int na = d->na, nx = d->nx, ny = d->ny, nz = d->nz;
+        PmlBox b[5] = {
+            {1, 1, 1, nx, na, nz, 0},
+            {1, ny - na + 1, 1, nx, na, nz, 0},
+            {1, na + 1, 1, na, ny - 2 * na, nz, 0},
+            {nx - na + 1, na + 1, 1, na, ny - 2 * na, nz, 0},
+            {na + 1, na + 1, nz - na + 1, nx - 2 * na, ny - 2 * na, na, 0},
+        };
+        long long off = 0;
+        for (int q = 0; q < 5; ++q) {
+            b[q].off = off;
+            off += (long long)b[q].nxb * b[q].nyb * b[q].nzb;
+            c->box[q] = b[q];
+        }
+        c->npml = off;
+        ALLOC(c->ade, (size_t)A_N * c->npml * sizeof(float));
+        ALLOC(c->gxc, (size_t)nx * sizeof(float4));
+        ALLOC(c->gxe, (size_t)nx * sizeof(float4));
+        ALLOC(c->gyc, (size_t)ny * sizeof(float4));
+        ALLOC(c->gye, (size_t)ny * sizeof(float4));
+        ALLOC(c->gzc, (size_t)nz * sizeof(float4));
+        ALLOC(c->gze, (size_t)nz * sizeof(float4));
+    }
+    if (d->abc == SWPC_ABC_CERJAN) {
+        ALLOC(c->cgx_c, (size_t)(d->nx + 6) * sizeof(float));
+        ALLOC(c->cgx_b, (size_t)(d->nx + 6) * sizeof(float));
+        ALLOC(c->cgy_c, (size_t)(d->ny + 6) * sizeof(float));
+        ALLOC(c->cgy_b, (size_t)(d->ny + 6) * sizeof(float));
+        ALLOC(c->cgz_c, (size_t)(d->nz + 6) * sizeof(float));
+        ALLOC(c->cgz_b, (size_t)(d->nz + 6) * sizeof(float));
+    }
+    c->staging_n = (long long)(d->nx + 6) * (d->ny + 6) * (d->nz + 6);
+    ALLOC(c->staging, (size_t)c->staging_n * sizeof(float));
+    ALLOC(c->d_vmax, 4 * sizeof(float));
+    ALLOC(c->d_it, 4 * sizeof(int));
+#undef ALLOC
+    {
+        // 4th-order coefficients of both band states (m_kernel.f90:43-54,106-111)
+        const float dx = (float)d->dx, dy = (float)d->dy, dz = (float)d->dz;
+        const float h[3] = {dx, dy, dz};
+        float *re40[3] = {c->coef.re40x, c->coef.re40y, c->coef.re40z};
+        float *re41[3] = {c->coef.re41x, c->coef.re41y, c->coef.re41z};
+        for (int a = 0; a < 3; ++a) {
+            const float rc40 = 17.0f / 16.0f / h[a], rc41 = 1.0f / 48.0f / h[a];
+            const float rd40 = -1.0f / 16.0f / h[a], rd41 = -1.0f / 48.0f / h[a];
+            re40[a][0] = rc40 + (-1.0f) * rd40;
+            re41[a][0] = rc41 + (-1.0f) * rd41;
+            re40[a][1] = rc40 + (1.0f) * rd40;
+            re41[a][1] = rc41 + (1.0f) * rd41;
+        }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = SWPC_EHIP; goto bad; }
+    *out = c;
+    return SWPC_OK;
+bad:
+    swpc_destroy(c);
+    return rc;
+}
+
+extern "C" int swpc_destroy(swpc_ctx *c)
+{
+    if (!c) return SWPC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    reset_graph(c);
+    for (int q = 0; q < SWPC_NFIELD; ++q) dfree(c->fields[q]);
+    dfree(c->rho); dfree(c->lam); dfree(c->mu); dfree(c->taup); dfree(c->taus);
+    dfree(c->kfs_top); dfree(c->kfs_bot); dfree(c->kob_top); dfree(c->kob_bot); dfree(c->kob);
+    dfree(c->R); dfree(c->ade);
+    dfree(c->gxc); dfree(c->gxe); dfree(c->gyc); dfree(c->gye); dfree(c->gzc); dfree(c->gze);
+    dfree(c->cgx_c); dfree(c->cgx_b); dfree(c->cgy_c); dfree(c->cgy_b); dfree(c->cgz_c); dfree(c->cgz_b);
+    dfree(c->d_targets); dfree(c->d_contrib); dfree(c->d_comp); dfree(c->d_mo); dfree(c->d_stf);
+    dfree(c->d_ist); dfree(c->d_jst); dfree(c->d_kst); dfree(c->d_uacc); dfree(c->d_eacc);
+    for (int q = 0; q < 4; ++q) dfree(c->d_wav[q]);
+    dfree(c->staging); dfree(c->d_vmax); dfree(c->d_it);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SWPC_OK;
+}
+
+// upload one reference-layout (k,i,j) halo-3 array into a device array
+static int upload_ref(swpc_ctx *c, const float *host, float *dev)
+{
+    HIPCHK(c, hipMemcpyAsync(c->staging, host, (size_t)c->staging_n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_ref_to_dev(c, c->staging, dev, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+static int upload_map(swpc_ctx *c, const int32_t *host, int *dev)
+{
+    const Grid &g = c->g;
+    std::vector<int> buf((size_t)g.plane, 0);
+    const int nxr = g.nx + 6;
+    for (int j = -2; j <= g.ny + 3; ++j)
+        for (int i = -2; i <= g.nx + 3; ++i) buf[(size_t)g2idx(g, i, j)] = host[(size_t)(j + 2) * nxr + (i + 2)];
+    HIPCHK(c, hipMemcpyAsync(dev, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_medium(swpc_ctx *c, const float *rho, const float *lam, const float *mu, const float *taup,
+                               const float *taus, const int32_t *kfs_top, const int32_t *kfs_bot,
+                               const int32_t *kob_top, const int32_t *kob_bot, const int32_t *kob)
+{
+    if (!c) return SWPC_EINVAL;
+    if (!rho || !lam || !mu || !taup || !taus || !kfs_top || !kfs_bot || !kob_top || !kob_bot || !kob)
+        return fail(c, SWPC_EINVAL, "swpc_set_medium: null array");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = upload_ref(c, rho, c->rho)) || (rc = upload_ref(c, lam, c->lam)) || (rc = upload_ref(c, mu, c->mu)) ||
+        (rc = upload_ref(c, taup, c->taup)) || (rc = upload_ref(c, taus, c->taus)))
+        return rc;
+    if ((rc = upload_map(c, kfs_top, c->kfs_top)) || (rc = upload_map(c, kfs_bot, c->kfs_bot)) ||
+        (rc = upload_map(c, kob_top, c->kob_top)) || (rc = upload_map(c, kob_bot, c->kob_bot)) ||
+        (rc = upload_map(c, kob, c->kob)))
+        return rc;
+    c->has_medium = true;
+    reset_graph(c);
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_visco(swpc_ctx *c, const float *c1, const float *c2, const float *d1, float d2)
+{
+    if (!c) return SWPC_EINVAL;
+    const int nm = c->desc.nm;
+    if (nm > 0 && (!c1 || !c2 || !d1)) return fail(c, SWPC_EINVAL, "swpc_set_visco: null array");
+    for (int m = 0; m < nm; ++m) {
+        c->visco.c1[m] = c1[m];
+        c->visco.c2[m] = c2[m];
+        c->visco.d1[m] = d1[m];
+    }
+    c->visco.d2 = d2;
+    c->has_visco = true;
+    reset_graph(c);
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_pml(swpc_ctx *c, const float *gxc, const float *gxe, const float *gyc, const float *gye,
+                            const float *gzc, const float *gze)
+{
+    if (!c) return SWPC_EINVAL;
+    if (c->desc.abc != SWPC_ABC_PML) return fail(c, SWPC_ESTATE, "swpc_set_pml: context uses the Cerjan sponge");
+    if (c->npml == 0) { c->has_pml = true; return SWPC_OK; }
+    if (!gxc || !gxe || !gyc || !gye || !gzc || !gze) return fail(c, SWPC_EINVAL, "swpc_set_pml: null array");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Grid &g = c->g;
+    // g(4,n) is four consecutive floats per cell: the float4 layout
+    HIPCHK(c, hipMemcpyAsync(c->gxc, gxc, (size_t)g.nx * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->gxe, gxe, (size_t)g.nx * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->gyc, gyc, (size_t)g.ny * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->gye, gye, (size_t)g.ny * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->gzc, gzc, (size_t)g.nz * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->gze, gze, (size_t)g.nz * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_pml = true;
+    reset_graph(c);
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_cerjan(swpc_ctx *c, const float *gx_c, const float *gx_b, const float *gy_c,
+                               const float *gy_b, const float *gz_c, const float *gz_b)
+{
+    if (!c) return SWPC_EINVAL;
+    if (c->desc.abc != SWPC_ABC_CERJAN) return fail(c, SWPC_ESTATE, "swpc_set_cerjan: context uses the PML");
+    if (!gx_c || !gx_b || !gy_c || !gy_b || !gz_c || !gz_b) return fail(c, SWPC_EINVAL, "swpc_set_cerjan: null array");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Grid &g = c->g;
+    HIPCHK(c, hipMemcpyAsync(c->cgx_c, gx_c, (size_t)(g.nx + 6) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cgx_b, gx_b, (size_t)(g.nx + 6) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cgy_c, gy_c, (size_t)(g.ny + 6) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cgy_b, gy_b, (size_t)(g.ny + 6) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cgz_c, gz_c, (size_t)(g.nz + 6) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cgz_b, gz_b, (size_t)(g.nz + 6) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_cerjan = true;
+    reset_graph(c);
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_sources(swpc_ctx *c, int nsrc, int bf_mode, const int32_t *isrc, const int32_t *jsrc,
+                                const int32_t *ksrc, const float *mo, const float *comp, float M0, int nt,
+                                const float *stf)
+{
+    if (!c) return SWPC_EINVAL;
+    if (nsrc < 0 || nt < 0) return fail(c, SWPC_EINVAL, "swpc_set_sources: negative count");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_targets); dfree(c->d_contrib); dfree(c->d_comp); dfree(c->d_mo); dfree(c->d_stf);
+    c->d_targets = nullptr; c->d_contrib = nullptr; c->d_comp = nullptr; c->d_mo = nullptr; c->d_stf = nullptr;
+    c->nsrc = 0;
+    c->ntarget = 0;
+    reset_graph(c);
+    if (nsrc == 0) return SWPC_OK;
+    if (!isrc || !jsrc || !ksrc || !comp || !stf || (!bf_mode && !mo) || !(M0 != 0.0f))
+        return fail(c, SWPC_EINVAL, "swpc_set_sources: null array or zero M0");
+    const Grid &g = c->g;
+    for (int n = 0; n < nsrc; ++n) {
+        if (isrc[n] < 2 || isrc[n] > g.nx || jsrc[n] < 2 || jsrc[n] > g.ny || ksrc[n] < 2 || ksrc[n] > g.nz)
+            return fail(c, SWPC_EINVAL, "swpc_set_sources: source %d at (%d,%d,%d) outside [2,nx]x[2,ny]x[2,nz]", n,
+                        isrc[n], jsrc[n], ksrc[n]);
+    }
+    const int ncomp = bf_mode ? 3 : 6;
+    // targets grouped by (field, cell) in first-touch order, contributions in source order
+    std::map<std::pair<int, long long>, int> where;
+    std::vector<std::vector<SrcContrib>> lists;
+    std::vector<std::pair<int, long long>> keys;
+    auto add = [&](int field, int i, int j, int k, SrcContrib ct) {
+        const std::pair<int, long long> key(field, gidx(g, i, j, k));
+        auto it = where.find(key);
+        int id;
+        if (it == where.end()) {
+            id = (int)lists.size();
+            where[key] = id;
+            lists.emplace_back();
+            keys.push_back(key);
+        } else {
+            id = it->second;
+        }
+        lists[id].push_back(ct);
+    };
+    for (int n = 0; n < nsrc; ++n) {
+        const int ii = isrc[n], jj = jsrc[n], kk = ksrc[n];
+        SrcContrib ct;
+        memset(&ct, 0, sizeof(ct));
+        ct.src = n;
+        if (!bf_mode) {  // m_source.f90:808-839
+            ct.div4 = 0;
+            ct.comp = 0; add(SWPC_SXX, ii, jj, kk, ct);
+            ct.comp = 1; add(SWPC_SYY, ii, jj, kk, ct);
+            ct.comp = 2; add(SWPC_SZZ, ii, jj, kk, ct);
+            ct.div4 = 1;
+            ct.comp = 5;
+            add(SWPC_SXY, ii, jj, kk, ct); add(SWPC_SXY, ii, jj - 1, kk, ct);
+            add(SWPC_SXY, ii - 1, jj, kk, ct); add(SWPC_SXY, ii - 1, jj - 1, kk, ct);
+            ct.comp = 4;
+            add(SWPC_SXZ, ii, jj, kk, ct); add(SWPC_SXZ, ii, jj, kk - 1, ct);
+            add(SWPC_SXZ, ii - 1, jj, kk, ct); add(SWPC_SXZ, ii - 1, jj, kk - 1, ct);
+            ct.comp = 3;
+            add(SWPC_SYZ, ii, jj, kk, ct); add(SWPC_SYZ, ii, jj, kk - 1, ct);
+            add(SWPC_SYZ, ii, jj - 1, kk, ct); add(SWPC_SYZ, ii, jj - 1, kk - 1, ct);
+        } else {  // m_source.f90:880-890
+            const long long p0 = gidx(g, ii, jj, kk);
+            ct.comp = 0;
+            ct.rho_a = p0; ct.rho_b = gidx(g, ii + 1, jj, kk); add(SWPC_VX, ii, jj, kk, ct);
+            ct.rho_a = p0; ct.rho_b = gidx(g, ii - 1, jj, kk); add(SWPC_VX, ii - 1, jj, kk, ct);
+            ct.comp = 1;
+            ct.rho_a = p0; ct.rho_b = gidx(g, ii, jj + 1, kk); add(SWPC_VY, ii, jj, kk, ct);
+            ct.rho_a = p0; ct.rho_b = gidx(g, ii, jj - 1, kk); add(SWPC_VY, ii, jj - 1, kk, ct);
+            ct.comp = 2;
+            ct.rho_a = p0; ct.rho_b = gidx(g, ii, jj, kk + 1); add(SWPC_VZ, ii, jj, kk, ct);
+            ct.rho_a = p0; ct.rho_b = gidx(g, ii, jj, kk - 1); add(SWPC_VZ, ii, jj, kk - 1, ct);
+        }
+    }
+    std::vector<SrcTarget> tg(lists.size());
+    std::vector<SrcContrib> ct;
+    for (size_t q = 0; q < lists.size(); ++q) {
+        tg[q].field = keys[q].first;
+        tg[q].idx = keys[q].second;
+        tg[q].start = (int)ct.size();
+        tg[q].count = (int)lists[q].size();
+        ct.insert(ct.end(), lists[q].begin(), lists[q].end());
+    }
+    // normalisation by M0 in single precision (m_source.f90:297-303) and
+    // dt_dxyz = dt / (dx*dy*dz) (m_source.f90:306)
+    std::vector<float> cm((size_t)ncomp * nsrc), mon((size_t)nsrc, 1.0f);
+    for (int q = 0; q < ncomp; ++q)
+        for (int n = 0; n < nsrc; ++n) {
+            const float v = comp[(size_t)q * nsrc + n];
+            cm[(size_t)q * nsrc + n] = bf_mode ? v / M0 : v;
+        }
+    if (!bf_mode)
+        for (int n = 0; n < nsrc; ++n) mon[n] = mo[n] / M0;
+    const float dx = (float)c->desc.dx, dy = (float)c->desc.dy, dz = (float)c->desc.dz;
+    c->dt_dxyz = c->desc.dt / (dx * dy * dz);
+    if (dalloc(c, (void **)&c->d_targets, tg.size() * sizeof(SrcTarget)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_contrib, ct.size() * sizeof(SrcContrib)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_comp, cm.size() * sizeof(float)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_mo, mon.size() * sizeof(float)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_stf, (size_t)nt * nsrc * sizeof(float) + 4) != hipSuccess)
+        return fail(c, SWPC_ENOMEM, "swpc_set_sources: device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_targets, tg.data(), tg.size() * sizeof(SrcTarget), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_contrib, ct.data(), ct.size() * sizeof(SrcContrib), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_comp, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mo, mon.data(), mon.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (nt > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_stf, stf, (size_t)nt * nsrc * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nsrc = nsrc;
+    c->bf_mode = bf_mode ? 1 : 0;
+    c->src_nt = nt;
+    c->ntarget = (int)tg.size();
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_receivers(swpc_ctx *c, int nst, const int32_t *ist, const int32_t *jst, const int32_t *kst,
+                                  int what, int ntdec_w, int ntw, float M0, float UC)
+{
+    if (!c) return SWPC_EINVAL;
+    if (nst < 0 || (nst > 0 && (!ist || !jst || !kst || ntdec_w < 1 || ntw < 1)))
+        return fail(c, SWPC_EINVAL, "swpc_set_receivers: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_ist); dfree(c->d_jst); dfree(c->d_kst); dfree(c->d_uacc); dfree(c->d_eacc);
+    c->d_ist = c->d_jst = c->d_kst = nullptr;
+    c->d_uacc = c->d_eacc = nullptr;
+    for (int q = 0; q < 4; ++q) { dfree(c->d_wav[q]); c->d_wav[q] = nullptr; }
+    c->nst = 0;
+    reset_graph(c);
+    if (nst == 0) return SWPC_OK;
+    const Grid &g = c->g;
+    for (int n = 0; n < nst; ++n)
+        if (ist[n] < 1 || ist[n] > g.nx || jst[n] < 1 || jst[n] > g.ny || kst[n] < 1 || kst[n] > g.nz)
+            return fail(c, SWPC_EINVAL, "swpc_set_receivers: station %d outside the grid", n);
+    if (dalloc(c, (void **)&c->d_ist, nst * sizeof(int)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_jst, nst * sizeof(int)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_kst, nst * sizeof(int)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_uacc, 3 * nst * sizeof(float)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_eacc, 6 * nst * sizeof(float)) != hipSuccess)
+        return fail(c, SWPC_ENOMEM, "swpc_set_receivers: device allocation failed");
+    const int ncmp[4] = {3, 3, 6, 6};
+    for (int q = 0; q < 4; ++q)
+        if (what & (1 << q))
+            if (dalloc(c, (void **)&c->d_wav[q], (size_t)ntw * ncmp[q] * nst * sizeof(float)) != hipSuccess)
+                return fail(c, SWPC_ENOMEM, "swpc_set_receivers: device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_ist, ist, nst * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_jst, jst, nst * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_kst, kst, nst * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nst = nst;
+    c->wav_what = what;
+    c->ntdec_w = ntdec_w;
+    c->ntw = ntw;
+    c->wav_M0 = M0;
+    c->wav_UC = UC;
+    return SWPC_OK;
+}
+
+static int ready(swpc_ctx *c)
+{
+    if (!c->has_medium) return fail(c, SWPC_ESTATE, "medium not set");
+    if (c->desc.nm > 0 && !c->has_visco) return fail(c, SWPC_ESTATE, "visco coefficients not set");
+    if (c->desc.abc == SWPC_ABC_PML && !c->has_pml) return fail(c, SWPC_ESTATE, "PML profiles not set");
+    if (c->desc.abc == SWPC_ABC_CERJAN && !c->has_cerjan) return fail(c, SWPC_ESTATE, "Cerjan profiles not set");
+    return SWPC_OK;
+}
+
+extern "C" int swpc_kernel_update_stress(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_kernel_stress(c));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_absorb_update_stress(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = ready(c);
+    if (rc) return rc;
+    if (c->desc.abc == SWPC_ABC_PML) HIPCHK(c, launch_pml_stress(c));
+    else HIPCHK(c, launch_cerjan_stress(c));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_kernel_update_vel(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_kernel_vel(c));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_absorb_update_vel(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = ready(c);
+    if (rc) return rc;
+    if (c->desc.abc == SWPC_ABC_PML) HIPCHK(c, launch_pml_vel(c));
+    else HIPCHK(c, launch_cerjan_vel(c));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_source_stressglut(swpc_ctx *c, int it)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, launch_stressglut(c, it, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_source_bodyforce(swpc_ctx *c, int it)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, launch_bodyforce(c, it, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_wav_store(swpc_ctx *c, int it)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, launch_wav_store(c, it, nullptr));
+    return SWPC_OK;
+}
+
+// one step in the order of src/swpc_3d/main.f90:122-138.  d_it != nullptr:
+// kernels read the step number from device memory and the step ends by
+// advancing it (graph-capturable).
+static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it)
+{
+    hipError_t e;
+    if ((e = launch_wav_store(c, it, d_it)) != hipSuccess) return e;
+    if ((e = launch_kernel_stress(c)) != hipSuccess) return e;
+    if (c->desc.abc == SWPC_ABC_PML) e = launch_pml_stress(c);
+    else e = launch_cerjan_stress(c);
+    if (e != hipSuccess) return e;
+    if ((e = launch_stressglut(c, it, d_it)) != hipSuccess) return e;
+    if ((e = launch_kernel_vel(c)) != hipSuccess) return e;
+    if (c->desc.abc == SWPC_ABC_PML) e = launch_pml_vel(c);
+    else e = launch_cerjan_vel(c);
+    if (e != hipSuccess) return e;
+    if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess) return e;
+    if (d_it) e = launch_step_advance(c);
+    return e;
+}
+
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+extern "C" int swpc_step(swpc_ctx *c, int it0, int nsteps, int use_graph, int block)
+{
+    if (!c) return SWPC_EINVAL;
+    if (nsteps < 0) return fail(c, SWPC_EINVAL, "swpc_step: negative step count");
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    int done = 0;
+    if (use_graph && block > 0 && nsteps >= block) {
+        if (!c->graph_exec || c->graph_block != block) {
+            reset_graph(c);
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int b = 0; b < block && e == hipSuccess; ++b) e = enqueue_step(c, 0, c->d_it);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if (e != hipSuccess) return fail(c, SWPC_EHIP, "graph capture: %s", hipGetErrorString(e));
+            if (e2 != hipSuccess) return fail(c, SWPC_EHIP, "graph capture end: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                c->graph_exec = nullptr;
+                return fail(c, SWPC_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+            }
+            c->graph_block = block;
+        }
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->d_it, it0);
+        HIPCHK(c, hipGetLastError());
+        for (; done + block <= nsteps; done += block) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    }
+    for (; done < nsteps; ++done) HIPCHK(c, enqueue_step(c, it0 + done, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_kernel_vmax(swpc_ctx *c, float out[3])
+{
+    if (!c || !out) return SWPC_EINVAL;
+    if (!c->has_medium) return fail(c, SWPC_ESTATE, "medium not set");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_vmax(c));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_vmax, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_get_field(swpc_ctx *c, int field, float *out)
+{
+    if (!c || !out || field < 0 || field >= SWPC_NFIELD) return SWPC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_dev_to_ref(c, c->fields[field], c->staging, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->staging, (size_t)c->staging_n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_field(swpc_ctx *c, int field, const float *in)
+{
+    if (!c || !in || field < 0 || field >= SWPC_NFIELD) return SWPC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    return upload_ref(c, in, c->fields[field]);
+}
+
+extern "C" int swpc_get_seismograms(swpc_ctx *c, int kind, float *out)
+{
+    if (!c || !out) return SWPC_EINVAL;
+    int q = -1;
+    for (int b = 0; b < 4; ++b)
+        if (kind == (1 << b)) q = b;
+    if (q < 0) return fail(c, SWPC_EINVAL, "swpc_get_seismograms: kind must be one SWPC_WAV_* flag");
+    if (c->nst == 0 || !c->d_wav[q]) return fail(c, SWPC_ESTATE, "swpc_get_seismograms: output not enabled");
+    const int ncmp[4] = {3, 3, 6, 6};
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_wav[q], (size_t)c->ntw * ncmp[q] * c->nst * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_synchronize(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by)
+{
+    if (!c) return SWPC_EINVAL;
+    if (kchunk < 0 || by < 0 || (by > 0 && (256 % (by * 64) != 0 && by * 64 > 1024)))
+        return fail(c, SWPC_EINVAL, "swpc_set_tuning: bad value");
+    if (by > 4) return fail(c, SWPC_EINVAL, "swpc_set_tuning: by must be 1..4 (launch bound 256)");
+    if (kchunk > 0) c->kchunk = kchunk;
+    if (by > 0) c->by = by;
+    reset_graph(c);
+    return SWPC_OK;
+}

@@ -1,0 +1,169 @@
+// swpc_internal.h -- device layout and context of libswpc_gpu (gfx950).
+//
+// Device layout (all FP32, x fastest):
+//   3D halo arrays (fields V3/S6, medium rho/lam/mu/taup/taus):
+//     idx(i,j,k) = (k+2)*plane + (j+2)*pitch + (i-1+ox),  i,j,k in -2..n+3
+//     ox is chosen so that i = na+1 (first interior column) starts a 128-byte
+//     line; pitch is a multiple of 32 floats.
+//   Memory variables: 6*NM arrays on the kernel box only,
+//     ridx(i,j,k) = ((k-kb)*nyk + (j-jb))*rpitch + (i-ib).
+//   PML auxiliaries: 18 arrays over the PML cells only, indexed by a compact
+//     cell number running over five boxes (two y slabs, two x slabs, bottom).
+//   (i,j) maps: (j+2)*pitch + (i-1+ox).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/swpc_gpu.h"
+
+#define SWPC_MAXNM 8
+
+struct Grid {
+    int nx, ny, nz, na, nm;
+    int ox, pitch, nyp, nzp;  // nyp = ny+6 rows, nzp = nz+6 planes
+    long long plane, n;       // floats per plane / per array
+    // kernel (interior) box, m_global.f90:345-376
+    int ib, ie, jb, je, kb, ke;
+    int nxk, nyk, nzk, rpitch;
+    long long rplane, rn;
+};
+
+__host__ __device__ static inline long long gidx(const Grid &g, int i, int j, int k)
+{
+    return (long long)(k + 2) * g.plane + (long long)(j + 2) * g.pitch + (i - 1 + g.ox);
+}
+__host__ __device__ static inline long long g2idx(const Grid &g, int i, int j)
+{
+    return (long long)(j + 2) * g.pitch + (i - 1 + g.ox);
+}
+
+// one of the five PML boxes (m_global.f90:334-343 kbeg_a partition)
+struct PmlBox {
+    int i0, j0, k0;       // first cell
+    int nxb, nyb, nzb;    // extent
+    long long off;        // first compact cell number
+};
+
+// 4th-order coefficients for the two band states:
+// [0] outside the band (isign = -1), [1] inside (isign = +1)
+struct Coef4 {
+    float re40x[2], re41x[2], re40y[2], re41y[2], re40z[2], re41z[2];
+};
+
+struct FieldPtrs {
+    float *Vx, *Vy, *Vz;
+    float *Sxx, *Syy, *Szz, *Syz, *Sxz, *Sxy;
+};
+
+struct MediumPtrs {
+    const float *rho, *lam, *mu, *taup, *taus;
+    const int *kfs_top, *kfs_bot, *kob_top, *kob_bot, *kob;
+};
+
+struct ViscoCoef {
+    float c1[SWPC_MAXNM], c2[SWPC_MAXNM], d1[SWPC_MAXNM];
+    float d2;
+};
+
+// ADE auxiliaries, order of m_absorb_p.f90:47-52
+enum {
+    A_axVx = 0, A_ayVx, A_azVx, A_axVy, A_ayVy, A_azVy, A_axVz, A_ayVz, A_azVz,
+    A_axSxx, A_aySxy, A_azSxz, A_axSxy, A_aySyy, A_azSyz, A_axSxz, A_aySyz, A_azSzz,
+    A_N
+};
+
+struct PmlArgs {
+    Grid g;
+    FieldPtrs f;
+    MediumPtrs m;
+    float *ade;               // A_N arrays of npml floats
+    long long npml;
+    PmlBox box[5];
+    const float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;  // g(1:4, n), index n-1
+    float r20x, r20y, r20z;   // 1/dx etc. (m_absorb_p.f90:70-72)
+    float dt;
+};
+
+struct SrcTarget {
+    long long idx;   // device index in the field array
+    int field;       // SWPC_* field id
+    int start, count;
+};
+struct SrcContrib {
+    int src;         // source number
+    int comp;        // MT: 0..5 = xx yy zz yz xz xy ; BF: 0..2 = fx fy fz
+    int div4;        // MT shear: divide by 4 (m_source.f90:815-839)
+    int pad;
+    long long rho_a, rho_b;  // BF: the two densities of the staggered buoyancy
+};
+
+struct swpc_ctx {
+    swpc_desc desc;
+    Grid g;
+    int device;
+    hipStream_t stream;
+    char err[512];
+
+    float *fields[SWPC_NFIELD];
+    float *rho, *lam, *mu, *taup, *taus;
+    int *kfs_top, *kfs_bot, *kob_top, *kob_bot, *kob;
+    float *R;          // 6*nm*rn
+    ViscoCoef visco;
+    bool has_medium, has_visco, has_pml, has_cerjan;
+
+    // PML
+    PmlBox box[5];
+    long long npml;
+    float *ade;
+    float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
+    // Cerjan
+    float *cgx_c, *cgx_b, *cgy_c, *cgy_b, *cgz_c, *cgz_b;
+
+    Coef4 coef;
+
+    // sources
+    int nsrc, bf_mode, src_nt;
+    int ntarget;
+    SrcTarget *d_targets;
+    SrcContrib *d_contrib;
+    float *d_comp;     // (ncomp, nsrc), normalised by M0 (m_source.f90:297-303)
+    float *d_mo;       // nsrc, mo/M0
+    float *d_stf;      // (nt, nsrc)
+    float dt_dxyz;
+
+    // receivers
+    int nst, wav_what, ntdec_w, ntw;
+    float wav_M0, wav_UC;
+    int *d_ist, *d_jst, *d_kst;
+    float *d_uacc, *d_eacc;
+    float *d_wav[4];   // V, U, stress, strain: (ntw, ncmp, nst)
+
+    // graphs and device step counter
+    int *d_it;
+    hipGraphExec_t graph_exec;
+    int graph_block;
+
+    // staging for layout transposes
+    float *staging;
+    long long staging_n;
+    float *d_vmax;
+
+    int kchunk, by;
+    long long bytes;
+};
+
+// ---- launchers (stencil.hip / aux.hip) ----
+hipError_t launch_kernel_stress(swpc_ctx *c);
+hipError_t launch_kernel_vel(swpc_ctx *c);
+hipError_t launch_pml_stress(swpc_ctx *c);
+hipError_t launch_pml_vel(swpc_ctx *c);
+hipError_t launch_cerjan_stress(swpc_ctx *c);
+hipError_t launch_cerjan_vel(swpc_ctx *c);
+hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it);
+hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it);
+hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it);
+hipError_t launch_step_advance(swpc_ctx *c);
+hipError_t launch_vmax(swpc_ctx *c);
+hipError_t launch_ref_to_dev(swpc_ctx *c, const float *src_ref, float *dst, hipStream_t s);
+hipError_t launch_dev_to_ref(swpc_ctx *c, const float *src, float *dst_ref, hipStream_t s);

@@ -1,0 +1,216 @@
+"""ctypes binding of libswpc_gpu (include/swpc_gpu.h).
+
+``SwpcGPU`` mirrors the reference's hot-path routines one for one
+(src/swpc_3d/main.f90:119-139):
+
+    kernel__update_stress  -> SwpcGPU.kernel_update_stress()
+    absorb__update_stress  -> SwpcGPU.absorb_update_stress()
+    source__stressglut(it) -> SwpcGPU.source_stressglut(it)
+    kernel__update_vel     -> SwpcGPU.kernel_update_vel()
+    absorb__update_vel     -> SwpcGPU.absorb_update_vel()
+    source__bodyforce(it)  -> SwpcGPU.source_bodyforce(it)
+    wav__store(it)         -> SwpcGPU.wav_store(it)
+    kernel__vmax           -> SwpcGPU.kernel_vmax()
+
+Arrays cross the boundary in the reference layout: 3D (k,i,j) with k
+fastest and a 3-cell halo, i.e. numpy shape (ny+6, nx+6, nz+6); (i,j) maps
+as (ny+6, nx+6).  There is no CPU fallback: if the HIP library is missing
+or no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libswpc_gpu.so")
+
+FIELDS = {"Vx": 0, "Vy": 1, "Vz": 2, "Sxx": 3, "Syy": 4, "Szz": 5, "Syz": 6, "Sxz": 7, "Sxy": 8}
+WAV = {"vel": 1, "disp": 2, "stress": 4, "strain": 8}
+NCMP = {"vel": 3, "disp": 3, "stress": 6, "strain": 6}
+ABC = {"pml": 0, "cerjan": 1}
+
+
+class SwpcError(RuntimeError):
+    pass
+
+
+class _Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("na", C.c_int), ("nm", C.c_int),
+                ("abc", C.c_int), ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("dt", C.c_float), ("device", C.c_int)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libswpc_gpu.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SwpcError(f"{path} is missing: build it with openswpc_amd/build.py (no CPU fallback exists)")
+    L = C.CDLL(path)
+    vp, i, f = C.c_void_p, C.c_int, C.c_float
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L.swpc_create.argtypes = [C.POINTER(_Desc), C.POINTER(vp)]
+    L.swpc_destroy.argtypes = [vp]
+    L.swpc_last_error.argtypes = [vp]
+    L.swpc_last_error.restype = C.c_char_p
+    L.swpc_set_medium.argtypes = [vp, fp, fp, fp, fp, fp, ip, ip, ip, ip, ip]
+    L.swpc_set_visco.argtypes = [vp, fp, fp, fp, f]
+    L.swpc_set_pml.argtypes = [vp, fp, fp, fp, fp, fp, fp]
+    L.swpc_set_cerjan.argtypes = [vp, fp, fp, fp, fp, fp, fp]
+    L.swpc_set_sources.argtypes = [vp, i, i, ip, ip, ip, fp, fp, f, i, fp]
+    L.swpc_set_receivers.argtypes = [vp, i, ip, ip, ip, i, i, i, f, f]
+    for nm in ("swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_kernel_update_vel",
+               "swpc_absorb_update_vel", "swpc_synchronize"):
+        getattr(L, nm).argtypes = [vp]
+    for nm in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store"):
+        getattr(L, nm).argtypes = [vp, i]
+    L.swpc_step.argtypes = [vp, i, i, i, i]
+    L.swpc_kernel_vmax.argtypes = [vp, fp]
+    L.swpc_get_field.argtypes = [vp, i, fp]
+    L.swpc_set_field.argtypes = [vp, i, fp]
+    L.swpc_get_seismograms.argtypes = [vp, i, fp]
+    L.swpc_device_bytes.argtypes = [vp]
+    L.swpc_device_bytes.restype = C.c_int64
+    L.swpc_set_tuning.argtypes = [vp, i, i]
+    L.swpc_stream.argtypes = [vp]
+    L.swpc_stream.restype = vp
+    _lib = L
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class SwpcGPU:
+    """One device-resident 3D simulation (one context of libswpc_gpu)."""
+
+    def __init__(self, nx, ny, nz, na, nm, abc="pml", dx=0.5, dy=0.5, dz=0.5, dt=0.01, device=0):
+        self.L = load_library()
+        d = _Desc(nx=nx, ny=ny, nz=nz, na=na, nm=nm, abc=ABC[abc], dx=dx, dy=dy, dz=dz, dt=dt, device=device)
+        h = C.c_void_p()
+        rc = self.L.swpc_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise SwpcError(f"swpc_create failed ({rc})")
+        self.h = h
+        self.nx, self.ny, self.nz, self.na, self.nm = nx, ny, nz, na, nm
+        self.abc = abc
+        self.nst = 0
+        self.ntw = 0
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self.L.swpc_last_error(self.h)
+            raise SwpcError(f"libswpc_gpu error {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.swpc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up (the arrays the reference keeps in its modules) ----
+    def set_medium(self, rho, lam, mu, taup, taus, kfs_top, kfs_bot, kob_top, kob_bot, kob):
+        self._chk(self.L.swpc_set_medium(self.h, _f32(rho), _f32(lam), _f32(mu), _f32(taup), _f32(taus),
+                                         _i32(kfs_top), _i32(kfs_bot), _i32(kob_top), _i32(kob_bot), _i32(kob)))
+
+    def set_visco(self, c1, c2, d1, d2):
+        self._chk(self.L.swpc_set_visco(self.h, _f32(c1), _f32(c2), _f32(d1), float(d2)))
+
+    def set_pml(self, gxc, gxe, gyc, gye, gzc, gze):
+        self._chk(self.L.swpc_set_pml(self.h, _f32(gxc), _f32(gxe), _f32(gyc), _f32(gye), _f32(gzc), _f32(gze)))
+
+    def set_cerjan(self, gx_c, gx_b, gy_c, gy_b, gz_c, gz_b):
+        self._chk(self.L.swpc_set_cerjan(self.h, _f32(gx_c), _f32(gx_b), _f32(gy_c), _f32(gy_b), _f32(gz_c),
+                                         _f32(gz_b)))
+
+    def set_sources(self, isrc, jsrc, ksrc, mo, comp, M0, stf, bf_mode=False):
+        isrc, jsrc, ksrc = _i32(isrc), _i32(jsrc), _i32(ksrc)
+        n = int(isrc.size)
+        stf = _f32(stf).reshape(-1, max(n, 1)) if n else np.zeros((1, 1), np.float32)
+        comp = _f32(comp) if n else np.zeros(6, np.float32)
+        mo = _f32(mo) if (n and not bf_mode) else np.ones(max(n, 1), np.float32)
+        self._chk(self.L.swpc_set_sources(self.h, n, 1 if bf_mode else 0, isrc if n else np.zeros(1, np.int32),
+                                          jsrc if n else np.zeros(1, np.int32), ksrc if n else np.zeros(1, np.int32),
+                                          mo, comp, float(M0), int(stf.shape[0]) if n else 0, stf))
+
+    def set_receivers(self, ist, jst, kst, what, ntdec_w, ntw, M0, UC):
+        ist, jst, kst = _i32(ist), _i32(jst), _i32(kst)
+        self._chk(self.L.swpc_set_receivers(self.h, int(ist.size), ist, jst, kst, int(what), int(ntdec_w), int(ntw),
+                                            float(M0), float(UC)))
+        self.nst, self.ntw = int(ist.size), int(ntw)
+
+    # ---- time loop, routine for routine ----
+    def kernel_update_stress(self):
+        self._chk(self.L.swpc_kernel_update_stress(self.h))
+
+    def absorb_update_stress(self):
+        self._chk(self.L.swpc_absorb_update_stress(self.h))
+
+    def source_stressglut(self, it):
+        self._chk(self.L.swpc_source_stressglut(self.h, int(it)))
+
+    def kernel_update_vel(self):
+        self._chk(self.L.swpc_kernel_update_vel(self.h))
+
+    def absorb_update_vel(self):
+        self._chk(self.L.swpc_absorb_update_vel(self.h))
+
+    def source_bodyforce(self, it):
+        self._chk(self.L.swpc_source_bodyforce(self.h, int(it)))
+
+    def wav_store(self, it):
+        self._chk(self.L.swpc_wav_store(self.h, int(it)))
+
+    def step(self, it0, nsteps, use_graph=False, block=100):
+        self._chk(self.L.swpc_step(self.h, int(it0), int(nsteps), 1 if use_graph else 0, int(block)))
+
+    def synchronize(self):
+        self._chk(self.L.swpc_synchronize(self.h))
+
+    def kernel_vmax(self):
+        out = np.zeros(3, np.float32)
+        self._chk(self.L.swpc_kernel_vmax(self.h, out))
+        return out
+
+    # ---- data out / in ----
+    def get_field(self, name):
+        out = np.empty((self.ny + 6, self.nx + 6, self.nz + 6), np.float32)
+        self._chk(self.L.swpc_get_field(self.h, FIELDS[name], out))
+        return out
+
+    def set_field(self, name, arr):
+        a = _f32(arr)
+        assert a.shape == (self.ny + 6, self.nx + 6, self.nz + 6)
+        self._chk(self.L.swpc_set_field(self.h, FIELDS[name], a))
+
+    def seismograms(self, kind="vel"):
+        out = np.empty((self.nst, NCMP[kind], self.ntw), np.float32)
+        self._chk(self.L.swpc_get_seismograms(self.h, WAV[kind], out))
+        return out
+
+    def device_bytes(self):
+        return int(self.L.swpc_device_bytes(self.h))
+
+    def set_tuning(self, kchunk=0, by=0):
+        self._chk(self.L.swpc_set_tuning(self.h, int(kchunk), int(by)))
+
+    def stream(self):
+        return self.L.swpc_stream(self.h)
