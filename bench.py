@@ -1,0 +1,315 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3D viscoelastic velocity-stress loop on one MI355X.
+
+Workload (default, ``--workload production``): 1024 x 1024 x 512 cells,
+dx = dy = dz = 0.25 km, 3 Zener mechanisms, 20-cell ADE-CFS PML, free surface
+on top, layered background with a depth gradient Vs 2.0 -> 4.5 km/s
+(Vp = sqrt(3) Vs, Qs = 50 Vs, Qp = 2 Qs) plus seeded von Karman heterogeneity
+(eps 5 %, a = 5 km, kappa = 0.5, seed 3), one seeded moment-tensor source at
+10 km depth with a Kupper wavelet of 1 s, 256 seeded surface receivers.
+``--workload plain`` is the 128^3 example grid.
+
+A step is one pass of the time loop (receivers, stress interior + PML,
+stress glut, velocity interior + PML); steps run as replays of a HIP graph
+of whole steps.  Inputs are resident in HBM before the timed region; the
+timed region is bracketed by HIP events on the library's stream.
+
+Prints one JSON line (metric, value = Gcell-updates/s, roofline of the
+dominant kernel, step roofline, CPU baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+METRIC = "Gcell-updates/s & %HBM roofline: plain 3D, batched Green's fns, adjoint grad"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def production_spec(nx=1024, ny=1024, nz=512, seed_medium=3, seed_st=4, seed_src=5, with_xi=True):
+    from openswpc_amd.host import ModelSpec
+    from openswpc_amd.rmedia import von_karman
+    dx = 0.25
+    vs0, vs1 = 2.0, 4.5
+    layers = [[0.0, 2.3, vs0 * math.sqrt(3), vs0, 100.0 * vs0, 50.0 * vs0],
+              [60.0, 3.2, vs1 * math.sqrt(3), vs1, 100.0 * vs1, 50.0 * vs1]]
+    # dt = 0.85 x the 4th-order stability limit for the fastest perturbed Vp
+    vmax = vs1 * math.sqrt(3) * 1.15
+    hh = 1.0 / math.sqrt(3.0 / dx**2)
+    dt = float(np.float32(math.floor(0.85 * (6.0 / 7.0) * hh / vmax * 1e4) / 1e4))
+    rng = np.random.default_rng(seed_src)
+    strike, dip, rake = rng.uniform(0, 360), rng.uniform(0, 90), rng.uniform(-180, 180)
+    src = [dict(x=0.0, y=0.0, z=10.0, t0=0.0, tr=1.0, mo=1e15, strike=float(strike), dip=float(dip), rake=float(rake))]
+    rs = np.random.default_rng(seed_st)
+    half = nx * dx / 2 - 30 * dx
+    st = [(float(x), float(y), 0.0, "fsb") for x, y in rs.uniform(-half, half, size=(256, 2))]
+    xi = None
+    if with_xi:
+        t = time.perf_counter()
+        xi = von_karman((ny + 6, nx + 6, nz + 6), dx, dx, dx, a=5.0, eps=0.05, kappa=0.5, seed=seed_medium)
+        log(f"[bench] von Karman field {xi.shape} in {time.perf_counter() - t:.1f}s std={xi.std():.4f}")
+    return ModelSpec(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx, dt=dt, nt=3000, na=20, nm=3, zbeg=-1.0,
+                     vmodel="lgm", layers=layers, xi=xi, sources=src, stations=st, ntdec_w=1, wav_what=1,
+                     fq_min=0.05, fq_max=5.0, fq_ref=1.0)
+
+
+def plain_spec(seed_src=1, seed_st=2):
+    from openswpc_amd.host import ModelSpec
+    rng = np.random.default_rng(seed_src)
+    strike, dip, rake = rng.uniform(0, 360), rng.uniform(0, 90), rng.uniform(-180, 180)
+    src = [dict(x=0.0, y=0.0, z=10.0, t0=0.0, tr=1.0, mo=1e15, strike=float(strike), dip=float(dip), rake=float(rake))]
+    rs = np.random.default_rng(seed_st)
+    st = [(float(x), float(y), 0.0, "fsb") for x, y in rs.uniform(-21, 21, size=(16, 2))]
+    layers = [[0.0, 2.7, 5.5, 3.2, 320.0, 160.0], [5.0, 2.9, 6.5, 3.7, 370.0, 185.0]]
+    return ModelSpec(nx=128, ny=128, nz=128, dx=0.5, dy=0.5, dz=0.5, dt=0.02, nt=1000, na=20, nm=3, xbeg=-32.0,
+                     ybeg=-32.0, zbeg=-2.0, vmodel="lhm", layers=layers, sources=src, stations=st,
+                     fq_min=0.05, fq_max=5.0, fq_ref=1.0)
+
+
+def cell_counts(nx, ny, nz, na):
+    n_int = (nx - 2 * na) * (ny - 2 * na) * (nz - na)
+    return n_int, nx * ny * nz - n_int
+
+
+def bytes_model(nm):
+    """Algorithmic bytes per cell and phase (FP32; every array read or written
+    once per phase, stencil neighbours free)."""
+    stress_int = 4 * (3 + 4 + 2 * 6 + 2 * 6 * nm)   # V3 read; lam,mu,taup,taus; S6 RMW; R(6 nm) RMW
+    vel_int = 4 * (6 + 1 + 2 * 3)                   # S6 read; rho; V3 RMW
+    stress_pml = 4 * (3 + 2 + 2 * 6 + 2 * 9)        # V3; lam, mu; S6 RMW; 9 ADE RMW
+    vel_pml = 4 * (6 + 1 + 2 * 3 + 2 * 9)           # S6; rho; V3 RMW; 9 ADE RMW
+    return stress_int, vel_int, stress_pml, vel_pml
+
+
+# ---------------------------------------------------------------------------
+# HIP events on the library's stream
+# ---------------------------------------------------------------------------
+class HipEvents:
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle (restated reference scheme, FP32, OpenMP)
+# ---------------------------------------------------------------------------
+def cpu_baseline(target_s=12.0):
+    from oracle import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    nx, ny, nz = 192, 192, 128
+    cfg = O.Config(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, dt=0.0115, nt=400, na=20, nm=3, zbeg=-1.0,
+                   vmodel="lgm", layers=[[0.0, 2.3, 3.464, 2.0, 200, 100], [60.0, 3.2, 7.794, 4.5, 450, 225]],
+                   sources=[dict(x=0.0, y=0.0, z=10.0, t0=0.0, tr=1.0, mo=1e15, strike=30.0, dip=60.0, rake=90.0)])
+    m = O.build_model(cfg)
+    run = O.OracleRun(m, 32)
+    run.run(2)                                   # warm-up (thread pool, page faults)
+    t = time.perf_counter()
+    run.run(2)
+    per = (time.perf_counter() - t) / 2
+    steps = max(4, min(int(target_s / max(per, 1e-6)), cfg.nt - 4))
+    t = time.perf_counter()
+    run.run(steps)
+    dt = time.perf_counter() - t
+    val = nx * ny * nz * steps / dt / 1e9
+    return {"value": round(val, 5), "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP32 restatement (gcc -O2 OpenMP, {cores} threads), {nx}x{ny}x{nz} NM=3 PML, "
+                      f"{steps} steps, {dt:.1f} s"}
+
+
+def read_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from committed rocprofv3
+    --pmc summaries (FETCH_SIZE + WRITE_SIZE), if present."""
+    p = os.path.join(ROOT, "profiles", f"stress_interior_{workload}_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=["production", "plain"], default="production")
+    ap.add_argument("--block", type=int, default=50, help="steps per captured HIP graph")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per kernel with events")
+    ap.add_argument("--kchunk", type=int, default=0)
+    ap.add_argument("--by", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the single-GPU path is the whole "
+                                   "domain on one device; multi-GPU sharding is not built (see DESIGN.md)",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    from openswpc_amd.host import build_inputs, make_simulation
+
+    t0 = time.perf_counter()
+    spec = production_spec() if args.workload == "production" else plain_spec()
+    inp = build_inputs(spec)
+    spec.xi = None
+    log(f"[bench] host set-up {time.perf_counter() - t0:.1f}s dt={spec.dt} vmin={inp.scalars['vmin']:.3f} "
+        f"vmax={inp.scalars['vmax']:.3f}")
+    t0 = time.perf_counter()
+    sim = make_simulation(inp)
+    if args.kchunk or args.by:
+        sim.set_tuning(args.kchunk, args.by)
+    del inp
+    log(f"[bench] device upload {time.perf_counter() - t0:.1f}s, {sim.device_bytes() / 2**30:.1f} GiB on device")
+
+    ev = HipEvents()
+    stream = sim.stream()
+    use_graph = not args.no_graph
+    block = min(args.block, max(args.steps, 1))
+    it = 1
+    sim.step(it, args.warmup, use_graph=use_graph, block=min(block, max(args.warmup, 1)))
+    it += args.warmup
+    sim.synchronize()
+
+    e0, e1 = ev.create(), ev.create()
+    w0 = time.perf_counter()
+    ev.record(e0, stream)
+    sim.step(it, args.steps, use_graph=use_graph, block=block)
+    ev.record(e1, stream)
+    total_ms = ev.elapsed_ms(e0, e1)
+    wall = time.perf_counter() - w0
+    it += args.steps
+    ms_per_step = total_ms / args.steps
+
+    # per-kernel durations on the same stream (reference order of one step)
+    names = ["stress_interior", "stress_pml", "vel_interior", "vel_pml"]
+    acc = {n: 0.0 for n in names}
+    evs = [ev.create() for _ in range(6)]
+    for s in range(args.kernel_steps):
+        sim.wav_store(it)
+        ev.record(evs[0], stream)
+        sim.kernel_update_stress()
+        ev.record(evs[1], stream)
+        sim.absorb_update_stress()
+        ev.record(evs[2], stream)
+        sim.source_stressglut(it)
+        ev.record(evs[3], stream)
+        sim.kernel_update_vel()
+        ev.record(evs[4], stream)
+        sim.absorb_update_vel()
+        e5 = evs[5]
+        ev.record(e5, stream)
+        acc["stress_interior"] += ev.elapsed_ms(evs[0], evs[1])
+        acc["stress_pml"] += ev.elapsed_ms(evs[1], evs[2])
+        acc["vel_interior"] += ev.elapsed_ms(evs[3], evs[4])
+        acc["vel_pml"] += ev.elapsed_ms(evs[4], e5)
+        it += 1
+    kms = {n: acc[n] / args.kernel_steps for n in names}
+    vmax = sim.kernel_vmax()
+
+    nx, ny, nz, na, nm = spec.nx, spec.ny, spec.nz, spec.na, spec.nm
+    n_int, n_pml = cell_counts(nx, ny, nz, na)
+    b_si, b_vi, b_sp, b_vp = bytes_model(nm)
+    bytes_step = n_int * (b_si + b_vi) + n_pml * (b_sp + b_vp)
+    bytes_launch = n_int * b_si
+    achieved = bytes_launch / (kms["stress_interior"] * 1e-3)
+    value = nx * ny * nz / (ms_per_step * 1e-3) / 1e9
+    step_bw = bytes_step / (ms_per_step * 1e-3)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # the baseline must not hide the GPU result
+            cpu = {"error": str(e)}
+
+    workload = "production" if args.workload == "production" else "plain"
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("plain 3D viscoelastic FDM, 1024x1024x512 cells, dx=0.25 km, NM=3, 20-cell ADE-CFS PML, "
+                         "layered gradient Vs 2.0->4.5 km/s + von Karman (eps 5%, a 5 km, kappa 0.5, seed 3), "
+                         "1 moment-tensor source, 256 surface receivers" if workload == "production" else
+                         "plain 3D viscoelastic FDM, 128^3 cells, dx=0.5 km, NM=3, 20-cell ADE-CFS PML, two-layer "
+                         "medium, 1 moment-tensor source, 16 surface receivers"),
+            "grid": [nx, ny, nz], "dt": spec.dt, "nm": nm, "na": na, "graph_block": block if use_graph else 0,
+            "parallelism": "single GPU, whole domain resident in HBM",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_stress_interior<3> (kernel__update_stress)",
+            "achieved": round(achieved / 1e9, 2),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": read_traffic(workload),
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": round(kms["stress_interior"], 4),
+        },
+        "step_roofline": {
+            "bytes_per_step": bytes_step,
+            "achieved": round(step_bw / 1e9, 2),
+            "unit": "GB/s",
+            "frac": round(step_bw / HBM_PEAK, 4),
+            "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
+        },
+        "cpu_baseline": cpu,
+        "wall_s_timed": round(wall, 3),
+        "vmax_end": [float(v) for v in vmax],
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

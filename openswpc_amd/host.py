@@ -1,0 +1,228 @@
+"""Host-side set-up for libswpc_gpu, backed by the Fortran library
+libswpc_host.so (openswpc_amd/host/swpc_host.f90).
+
+``ModelSpec`` describes a run the way an OpenSWPC input.inf does (grid,
+layered medium, sources, stations); ``build_inputs`` runs the reference's
+set-up sequence (medium__setup -> kernel__setup -> source__setup ->
+absorb__setup, src/swpc_3d/main.f90:70-78) through the Fortran routines and
+returns the kernel inputs; ``make_simulation`` loads them into a SwpcGPU
+context.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(HERE, "lib", "libswpc_host.so")
+HALO = 3
+OFF = 2
+
+STF_CODES = {"boxcar": 0, "triangle": 1, "herrmann": 2, "kupper": 3, "cosine": 4, "texp": 5}
+ZSW = {"dep": 0, "fsb": 1, "obb": 2}
+UC_SP = np.float32(1.0000001095e-15)   # 10.0**(-15) in single precision (m_global.f90:29)
+
+_h = None
+
+
+def host_lib() -> C.CDLL:
+    global _h
+    if _h is not None:
+        return _h
+    if not os.path.exists(HOST_LIB):
+        raise RuntimeError(f"{HOST_LIB} is missing: build it with `make -C openswpc_amd/host`")
+    L = C.CDLL(HOST_LIB)
+    f, i = C.c_float, C.c_int
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L.swpch_i2x.restype = f
+    L.swpch_i2x.argtypes = [i, f, f]
+    L.swpch_x2i.restype = i
+    L.swpch_x2i.argtypes = [f, f, f]
+    L.swpch_stable_dt.restype = f
+    L.swpch_stable_dt.argtypes = [f, f, f, f]
+    L.swpch_vmodel_layers.argtypes = [i, i, i, i, f, f, f, i, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_int)]
+    L.swpch_perturb.argtypes = [i, i, i, fp, f, f, f, fp, fp, fp]
+    L.swpch_medium_setup.argtypes = [i, i, i, i, i, f, f, f, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_float),
+                                     ip, ip, ip, ip, ip, ip, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.swpch_kernel_coef.argtypes = [i, fp, f, fp, fp, fp, C.POINTER(C.c_float)]
+    L.swpch_pml_profile.argtypes = [i, i, f, f, f, f, fp, fp]
+    L.swpch_cerjan_profile.argtypes = [i, i, f, i, fp, fp]
+    L.swpch_sdr2moment.argtypes = [f, f, f, fp]
+    L.swpch_momentrate.restype = f
+    L.swpch_momentrate.argtypes = [f, i, f, f]
+    L.swpch_stf_table.argtypes = [i, i, i, fp, fp, f, f, f, fp]
+    L.swpch_station_index.argtypes = [i, fp, fp, fp, ip, i, i, i, f, f, f, f, f, f, ip, ip, ip, ip, ip, ip]
+    _h = L
+    return L
+
+
+@dataclass
+class ModelSpec:
+    nx: int
+    ny: int
+    nz: int
+    dx: float = 0.5
+    dy: float = 0.5
+    dz: float = 0.5
+    dt: float = 0.02
+    nt: int = 1000
+    na: int = 20
+    nm: int = 3
+    abc: str = "pml"
+    xbeg: float | None = None
+    ybeg: float | None = None
+    zbeg: float = -2.0
+    tbeg: float = 0.0
+    fq_min: float = 0.05
+    fq_max: float = 5.0
+    fq_ref: float = 1.0
+    vmodel: str = "lhm"          # 'lhm' (homogeneous layers) | 'lgm' (linear gradients)
+    layers: list = field(default_factory=list)   # rows: depth rho vp vs qp qs
+    vcut: float = 0.0
+    xi: np.ndarray | None = None  # optional relative perturbation, (k,i,j) halo layout
+    sources: list = field(default_factory=list)
+    bf_mode: bool = False
+    stftype: str = "kupper"
+    stations: list = field(default_factory=list)   # (x, y, z, zsw)
+    ntdec_w: int = 1
+    wav_what: int = 1
+
+    def __post_init__(self):
+        if self.xbeg is None:
+            self.xbeg = -self.nx / 2 * float(np.float32(self.dx))
+        if self.ybeg is None:
+            self.ybeg = -self.ny / 2 * float(np.float32(self.dy))
+
+
+@dataclass
+class Inputs:
+    spec: ModelSpec
+    arrays: dict
+    scalars: dict
+
+
+def build_inputs(spec: ModelSpec) -> Inputs:
+    L = host_lib()
+    nx, ny, nz, na, nm = spec.nx, spec.ny, spec.nz, spec.na, spec.nm
+    shp = (ny + 6, nx + 6, nz + 6)
+    rho, lam, mu, taup, taus = (np.zeros(shp, np.float32) for _ in range(5))
+    layers = np.ascontiguousarray(np.asarray(spec.layers, np.float32).reshape(-1))
+    rc = C.c_int()
+    kind = {"lhm": 0, "lgm": 1}[spec.vmodel]
+    f32 = np.float32
+    L.swpch_vmodel_layers(kind, nx, ny, nz, spec.zbeg, spec.dz, spec.vcut, len(spec.layers), layers,
+                          rho, lam, mu, taup, taus, C.byref(rc))
+    if rc.value != 0:
+        raise ValueError(f"velocity model rejected (rc={rc.value})")
+    if spec.xi is not None:
+        dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
+        vmax_cfl = (f32(6) / f32(7)) * f32(1) / np.sqrt(1 / dx32**2 + 1 / dy32**2 + 1 / dz32**2) / f32(spec.dt)
+        L.swpch_perturb(nx, ny, nz, np.ascontiguousarray(spec.xi, np.float32), f32(spec.vcut), f32(vmax_cfl),
+                        f32(1.0), rho, lam, mu)
+    ts = np.zeros(max(nm, 1), np.float32)
+    zeta = C.c_float()
+    kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot = (np.zeros((ny + 6, nx + 6), np.int32) for _ in range(6))
+    vmin, vmax = C.c_float(), C.c_float()
+    L.swpch_medium_setup(nx, ny, nz, na, nm, spec.fq_min, spec.fq_max, spec.fq_ref, rho, lam, mu, taup, taus, ts,
+                         C.byref(zeta), kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot, C.byref(vmin), C.byref(vmax))
+    c1, c2, d1 = (np.zeros(max(nm, 1), np.float32) for _ in range(3))
+    d2 = C.c_float()
+    L.swpch_kernel_coef(nm, ts, spec.dt, c1, c2, d1, C.byref(d2))
+
+    # sources
+    srcs = spec.sources
+    nsrc = len(srcs)
+    dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
+    isrc = np.array([L.swpch_x2i(s["x"], spec.xbeg, dx32) for s in srcs], np.int32)
+    jsrc = np.array([L.swpch_x2i(s["y"], spec.ybeg, dy32) for s in srcs], np.int32)
+    ksrc = np.array([L.swpch_x2i(s["z"], spec.zbeg, dz32) for s in srcs], np.int32)
+    t0 = np.array([s["t0"] for s in srcs], np.float32)
+    tr = np.array([s["tr"] for s in srcs], np.float32)
+    fcut = f32(0)
+    for v in tr:
+        fcut = max(fcut, f32(1) / v)
+    UC = UC_SP
+    if spec.bf_mode:
+        comp = np.array([[s["fx"] for s in srcs], [s["fy"] for s in srcs], [s["fz"] for s in srcs]], np.float32)
+        M0 = f32(math.sqrt(float(np.sum(comp * comp, dtype=f32)))) if nsrc else f32(1)
+        UC = f32(UC * f32(1000))
+        mo = np.ones(nsrc, np.float32)
+        shift = 0.0
+    else:
+        mo = np.array([s["mo"] for s in srcs], np.float32)
+        comp = np.zeros((6, nsrc), np.float32)
+        for n, s in enumerate(srcs):
+            if "strike" in s:
+                m6 = np.zeros(6, np.float32)
+                L.swpch_sdr2moment(s["strike"], s["dip"], s["rake"], m6)
+                comp[:, n] = m6
+            else:
+                comp[:, n] = [s["mxx"], s["myy"], s["mzz"], s["myz"], s["mxz"], s["mxy"]]
+        M0 = f32(np.sum(mo, dtype=f32)) if nsrc else f32(1)
+        shift = 0.5
+    stf = np.zeros((spec.nt, max(nsrc, 1)), np.float32)
+    if nsrc:
+        L.swpch_stf_table(spec.nt, nsrc, STF_CODES[spec.stftype], t0, tr, spec.tbeg, spec.dt, shift, stf)
+    stf = np.ascontiguousarray(stf[:, :nsrc])
+
+    # absorbers
+    arrays = dict(rho=rho, lam=lam, mu=mu, taup=taup, taus=taus, kfs=kfs, kob=kob, kfs_top=kfs_top,
+                  kfs_bot=kfs_bot, kob_top=kob_top, kob_bot=kob_bot, ts=ts[:nm].copy(), c1=c1, c2=c2, d1=d1,
+                  isrc=isrc, jsrc=jsrc, ksrc=ksrc, mo=mo, comp=np.ascontiguousarray(comp), stf=stf)
+    if spec.abc == "pml":
+        for ax, n, beg, h in (("x", nx, spec.xbeg, dx32), ("y", ny, spec.ybeg, dy32), ("z", nz, spec.zbeg, dz32)):
+            gc = np.zeros(4 * n, np.float32)
+            ge = np.zeros(4 * n, np.float32)
+            L.swpch_pml_profile(n, na, beg, h, fcut, spec.dt, gc, ge)
+            arrays[f"g{ax}c"] = gc
+            arrays[f"g{ax}e"] = ge
+    else:
+        for ax, n, h, top in (("x", nx, dx32, 1), ("y", ny, dy32, 1), ("z", nz, dz32, 0)):
+            g_c = np.zeros(n + 6, np.float32)
+            g_b = np.zeros(n + 6, np.float32)
+            L.swpch_cerjan_profile(n, na, h, top, g_c, g_b)
+            arrays[f"cg{ax}_c"] = g_c
+            arrays[f"cg{ax}_b"] = g_b
+
+    # stations
+    nst = len(spec.stations)
+    if nst:
+        xs = np.array([s[0] for s in spec.stations], np.float32)
+        ys = np.array([s[1] for s in spec.stations], np.float32)
+        zs = np.array([s[2] for s in spec.stations], np.float32)
+        zsw = np.array([ZSW[s[3]] for s in spec.stations], np.int32)
+        ist, jst, kst, inside = (np.zeros(nst, np.int32) for _ in range(4))
+        L.swpch_station_index(nst, xs, ys, zs, zsw, nx, ny, nz, spec.xbeg, spec.ybeg, spec.zbeg, dx32, dy32, dz32,
+                              kfs, kob, ist, jst, kst, inside)
+        keep = inside.astype(bool)
+        arrays.update(ist=ist[keep].copy(), jst=jst[keep].copy(), kst=kst[keep].copy())
+    else:
+        arrays.update(ist=np.zeros(0, np.int32), jst=np.zeros(0, np.int32), kst=np.zeros(0, np.int32))
+    ntw = int(math.floor(float(f32(spec.nt - 1) / f32(spec.ntdec_w) + f32(1.0))))
+    scalars = dict(zeta=float(zeta.value), d2=float(d2.value), vmin=float(vmin.value), vmax=float(vmax.value),
+                   fcut=float(fcut), M0=float(M0), UC=float(UC), ntw=ntw, nsrc=nsrc)
+    return Inputs(spec=spec, arrays=arrays, scalars=scalars)
+
+
+def make_simulation(inp: Inputs, device: int = 0):
+    """Create a SwpcGPU context loaded with the inputs of ``build_inputs``."""
+    from .gpu import SwpcGPU
+    s, a, sc = inp.spec, inp.arrays, inp.scalars
+    g = SwpcGPU(s.nx, s.ny, s.nz, s.na, s.nm, s.abc, s.dx, s.dy, s.dz, s.dt, device=device)
+    g.set_medium(a["rho"], a["lam"], a["mu"], a["taup"], a["taus"], a["kfs_top"], a["kfs_bot"], a["kob_top"],
+                 a["kob_bot"], a["kob"])
+    g.set_visco(a["c1"], a["c2"], a["d1"], sc["d2"])
+    if s.abc == "pml":
+        g.set_pml(a["gxc"], a["gxe"], a["gyc"], a["gye"], a["gzc"], a["gze"])
+    else:
+        g.set_cerjan(a["cgx_c"], a["cgx_b"], a["cgy_c"], a["cgy_b"], a["cgz_c"], a["cgz_b"])
+    if sc["nsrc"]:
+        g.set_sources(a["isrc"], a["jsrc"], a["ksrc"], a["mo"], a["comp"], sc["M0"], a["stf"], bf_mode=s.bf_mode)
+    if a["ist"].size:
+        g.set_receivers(a["ist"], a["jst"], a["kst"], s.wav_what, s.ntdec_w, sc["ntw"], sc["M0"], sc["UC"])
+    return g
