@@ -126,6 +126,13 @@ int swpc_source_bodyforce(swpc_ctx *ctx, int it);
 /* wav__store (src/swpc_3d/m_wav.f90:397-625) */
 int swpc_wav_store(swpc_ctx *ctx, int it);
 
+/* Both updates of one phase as the time loop issues them: kernel__update_stress
+   followed by absorb__update_stress (src/swpc_3d/main.f90:126-127), and
+   kernel__update_vel followed by absorb__update_vel (main.f90:132-133).  With
+   the PML each is one fused launch over the whole grid. */
+int swpc_update_stress(swpc_ctx *ctx);
+int swpc_update_vel(swpc_ctx *ctx);
+
 /* nsteps whole steps it0 .. it0+nsteps-1 in the order of the main loop
    (src/swpc_3d/main.f90:119-139): wav__store, stress, glut, vel,
    body force.  With use_graph != 0 the steps run as replays of a captured
@@ -150,9 +157,11 @@ int swpc_synchronize(swpc_ctx *ctx);
 /* Device bytes held by the context (for reporting). */
 int64_t swpc_device_bytes(const swpc_ctx *ctx);
 
-/* Launch-shape tuning knobs (0 = keep current).  kchunk: k-planes marched
-   per thread block in the interior kernels. */
-int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by);
+/* Launch-shape tuning knobs.  kchunk: k-planes marched per thread block
+   (0 = keep); by: rows per 64-wide tile, 1, 2 or 4 (0 = keep); variant:
+   kernel variant flags, bit 0 = non-temporal memory-variable/stress streams
+   (-1 = keep). */
+int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a
    caller can time or order work against it. */

@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <utility>
 #include <vector>
@@ -52,6 +53,52 @@ static void reset_graph(swpc_ctx *c)
     c->graph_block = 0;
 }
 
+// Tiles (64 x by columns, one k chunk) that hold PML cells: side columns
+// over the full depth, interior columns over the bottom na planes.
+static int build_pml_tiles(swpc_ctx *c)
+{
+    dfree(c->d_pml_tiles);
+    c->d_pml_tiles = nullptr;
+    c->npml_tiles = 0;
+    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0) return SWPC_OK;
+    const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz, na = c->g.na, by = c->by, kc = c->kchunk;
+    std::vector<int> t;
+    auto add = [&](int i0, int w, int j0, int k0, int k1) {
+        t.push_back(i0 | (w << 20)); t.push_back(j0); t.push_back(k0); t.push_back(k1);
+    };
+    // x slabs (side columns of interior rows), width na, several rows per block
+    const int slab_rows = (64 * by) / std::max(na, 1);
+    const bool slab_tiles = na > 0 && na <= 64 && slab_rows > 0;
+    for (int k0 = 1; k0 <= nz; k0 += kc) {
+        const int k1 = std::min(nz, k0 + kc - 1);
+        // full-row tiles: y-slab rows over the full depth, interior rows over
+        // the bottom na planes (x-slab columns of interior rows excluded)
+        for (int j0 = 1; j0 <= ny; j0 += by) {
+            const int jl = std::min(j0 + by - 1, ny);
+            const bool yslab = j0 <= na || jl > ny - na;
+            const bool inner_rows = jl > na && j0 <= ny - na;
+            for (int i0 = 1; i0 <= nx; i0 += 64) {
+                const int il = std::min(i0 + 63, nx);
+                const bool inner_cols = il > na && i0 <= nx - na;
+                if (yslab || (inner_rows && inner_cols && k1 >= nz - na + 1)) add(i0, 64, j0, k0, k1);
+            }
+        }
+        if (slab_tiles)
+            for (int j0 = na + 1; j0 <= ny - na; j0 += slab_rows) {
+                add(1, na, j0, k0, k1);
+                add(nx - na + 1, na, j0, k0, k1);
+            }
+    }
+    if (!slab_tiles && na > 0) return fail(c, SWPC_EINVAL, "PML thickness na must be 1..64");
+    if (t.empty()) return SWPC_OK;
+    if (dalloc(c, (void **)&c->d_pml_tiles, t.size() * sizeof(int)) != hipSuccess)
+        return fail(c, SWPC_ENOMEM, "PML tile list allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_pml_tiles, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->npml_tiles = (int)(t.size() / 4);
+    return SWPC_OK;
+}
+
 extern "C" const char *swpc_last_error(const swpc_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
 extern "C" int64_t swpc_device_bytes(const swpc_ctx *ctx) { return ctx ? ctx->bytes : 0; }
@@ -74,15 +121,16 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     c->device = d->device;
     c->kchunk = 64;
     c->by = 4;
+    c->variant = 1;
     if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;
         return SWPC_EHIP;
     }
     Grid &g = c->g;
     g.nx = d->nx; g.ny = d->ny; g.nz = d->nz; g.na = d->na; g.nm = d->nm;
-    // first interior column i = na+1 on a 128-byte boundary; at least 3 halo cells below i = 1
-    g.ox = (int)((32 - (d->na % 32)) % 32);
-    while (g.ox < 3) g.ox += 32;
+    // i = 1 on a 128-byte boundary: every 64-cell tile row of the full-width
+    // kernels starts a cache line; 32 halo cells below i = 1 (3 are used)
+    g.ox = 32;
     g.pitch = (int)round_up(g.ox + d->nx + 3, 32);
     g.nyp = d->ny + 6;
     g.nzp = d->nz + 6;
@@ -98,7 +146,9 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     g.nxk = g.ie - g.ib + 1;
     g.nyk = g.je - g.jb + 1;
     g.nzk = g.ke - g.kb + 1;
-    g.rpitch = (int)round_up(g.nxk, 32);
+    // memory variables keep full-width rows (i = 1 at offset 0) so that they
+    // align with the field rows; only interior cells are ever touched
+    g.rpitch = (int)round_up(d->nx, 32);
     g.rplane = (long long)g.rpitch * g.nyk;
     g.rn = g.rplane * g.nzk;
 
@@ -174,6 +224,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
         }
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = SWPC_EHIP; goto bad; }
+    if ((rc = build_pml_tiles(c)) != SWPC_OK) goto bad;
     *out = c;
     return SWPC_OK;
 bad:
@@ -190,7 +241,7 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     for (int q = 0; q < SWPC_NFIELD; ++q) dfree(c->fields[q]);
     dfree(c->rho); dfree(c->lam); dfree(c->mu); dfree(c->taup); dfree(c->taus);
     dfree(c->kfs_top); dfree(c->kfs_bot); dfree(c->kob_top); dfree(c->kob_bot); dfree(c->kob);
-    dfree(c->R); dfree(c->ade);
+    dfree(c->R); dfree(c->ade); dfree(c->d_pml_tiles);
     dfree(c->gxc); dfree(c->gxe); dfree(c->gyc); dfree(c->gye); dfree(c->gzc); dfree(c->gze);
     dfree(c->cgx_c); dfree(c->cgx_b); dfree(c->cgy_c); dfree(c->cgy_b); dfree(c->cgz_c); dfree(c->cgz_b);
     dfree(c->d_targets); dfree(c->d_contrib); dfree(c->d_comp); dfree(c->d_mo); dfree(c->d_stf);
@@ -507,6 +558,34 @@ extern "C" int swpc_absorb_update_vel(swpc_ctx *c)
     return SWPC_OK;
 }
 
+extern "C" int swpc_update_stress(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = ready(c);
+    if (rc) return rc;
+    if (c->desc.abc == SWPC_ABC_PML) {
+        HIPCHK(c, launch_stress(c, 3));
+    } else {
+        HIPCHK(c, launch_stress(c, 1));
+        HIPCHK(c, launch_cerjan_stress(c));
+    }
+    return SWPC_OK;
+}
+
+extern "C" int swpc_update_vel(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = ready(c);
+    if (rc) return rc;
+    if (c->desc.abc == SWPC_ABC_PML) {
+        HIPCHK(c, launch_vel(c, 3));
+    } else {
+        HIPCHK(c, launch_vel(c, 1));
+        HIPCHK(c, launch_cerjan_vel(c));
+    }
+    return SWPC_OK;
+}
+
 extern "C" int swpc_source_stressglut(swpc_ctx *c, int it)
 {
     if (!c) return SWPC_EINVAL;
@@ -533,17 +612,17 @@ extern "C" int swpc_wav_store(swpc_ctx *c, int it)
 // advancing it (graph-capturable).
 static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it)
 {
+    // kernel__update_stress and absorb__update_stress touch disjoint cells
+    // and only read V: with the PML both run in one launch; the Cerjan taper
+    // is a separate sweep over the updated stress, as in the reference.
     hipError_t e;
+    const bool pml = c->desc.abc == SWPC_ABC_PML;
     if ((e = launch_wav_store(c, it, d_it)) != hipSuccess) return e;
-    if ((e = launch_kernel_stress(c)) != hipSuccess) return e;
-    if (c->desc.abc == SWPC_ABC_PML) e = launch_pml_stress(c);
-    else e = launch_cerjan_stress(c);
-    if (e != hipSuccess) return e;
+    if ((e = launch_stress(c, pml ? 3 : 1)) != hipSuccess) return e;
+    if (!pml && (e = launch_cerjan_stress(c)) != hipSuccess) return e;
     if ((e = launch_stressglut(c, it, d_it)) != hipSuccess) return e;
-    if ((e = launch_kernel_vel(c)) != hipSuccess) return e;
-    if (c->desc.abc == SWPC_ABC_PML) e = launch_pml_vel(c);
-    else e = launch_cerjan_vel(c);
-    if (e != hipSuccess) return e;
+    if ((e = launch_vel(c, pml ? 3 : 1)) != hipSuccess) return e;
+    if (!pml && (e = launch_cerjan_vel(c)) != hipSuccess) return e;
     if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess) return e;
     if (d_it) e = launch_step_advance(c);
     return e;
@@ -640,14 +719,16 @@ extern "C" int swpc_synchronize(swpc_ctx *c)
     return SWPC_OK;
 }
 
-extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by)
+extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by, int variant)
 {
     if (!c) return SWPC_EINVAL;
-    if (kchunk < 0 || by < 0 || (by > 0 && (256 % (by * 64) != 0 && by * 64 > 1024)))
-        return fail(c, SWPC_EINVAL, "swpc_set_tuning: bad value");
-    if (by > 4) return fail(c, SWPC_EINVAL, "swpc_set_tuning: by must be 1..4 (launch bound 256)");
+    if (kchunk < 0 || by < 0 || by == 3 || by > 4)
+        return fail(c, SWPC_EINVAL, "swpc_set_tuning: kchunk >= 0, by in {1, 2, 4} (launch bound 256)");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (kchunk > 0) c->kchunk = kchunk;
     if (by > 0) c->by = by;
+    if (variant >= 0) c->variant = variant;
     reset_graph(c);
-    return SWPC_OK;
+    return build_pml_tiles(c);
 }

@@ -1,18 +1,23 @@
 // stencil.hip -- velocity/stress updates of OpenSWPC's 3D loop on gfx950.
 //
-// Interior (m_kernel) kernels march along z: each thread owns one (i,j)
-// column of a 64 x BY tile and walks kchunk planes, keeping the z-neighbours
-// of its column in registers (a rotating queue), while x/y neighbours come
-// from the plane currently streamed by the tile (L1/L2 resident).  Every
-// arithmetic expression keeps the reference's FP32 evaluation order; the
-// library is compiled with -ffp-contract=off so no FMA changes the rounding.
+// Each phase is two z-marching kernels: the interior (4th order + Zener
+// memory variables, m_kernel) and the ADE-CFS PML (2nd order, m_absorb_p).
+// A thread owns one (i,j) column of a 64 x BY tile of full-width rows whose
+// first cell sits on a 128-byte line, and marches a chunk of planes along z,
+// keeping the z-neighbours of its column in a rotating register queue while
+// the x/y neighbours come from the plane the tile is streaming (L1/L2
+// resident).  Loop bodies are branch free so every load of an iteration is
+// issued at once.  Every arithmetic expression keeps the reference's FP32
+// evaluation order; the library is compiled with -ffp-contract=off so no FMA
+// changes the rounding.
 //
-// PML kernels (m_absorb_p) run one thread per PML cell over a compact
-// numbering of the five PML boxes; their 18 auxiliary arrays exist only
-// there.  Cerjan tapers (m_absorb_c) are plain per-cell kernels.
+// Cerjan tapers (m_absorb_c) are separate per-cell sweeps, as in the
+// reference.
 #include "swpc_internal.h"
 
 #define BX 64
+#define MASK_INT 1
+#define MASK_PML 2
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // Give each XCD a contiguous run of logical tiles so tiles that share halo
@@ -40,7 +45,23 @@ __device__ static inline float hmean4(float a, float b, float c, float d, float 
 
 #define EPSL 1.1920928955078125e-07f  // epsilon(1.0)
 
-struct KArgs {
+// Streams read or written once per step (memory variables, stress, the
+// per-cell medium) can bypass cache retention so the L2 keeps the V and mu
+// rows that neighbouring tiles re-read.
+template <bool NT>
+__device__ static inline float ld1(const float *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT>
+__device__ static inline void st1(float *p, float v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+struct StepArgs {
     Grid g;
     FieldPtrs f;
     MediumPtrs m;
@@ -48,15 +69,28 @@ struct KArgs {
     ViscoCoef v;
     Coef4 c;
     float dt;
-    int kchunk, nbx, nby, nbz, by;
+    // PML
+    float *ade;
+    long long npml;
+    PmlBox box[5];
+    const float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
+    float r20x, r20y, r20z;
+    const int4 *tiles;   // PML tiles: (i0, j0, k0, k1)
+    int ntiles;
+    // interior launch shape
+    int kchunk, bx0, nbx, nby, nbz, by;
 };
 
 // ---------------------------------------------------------------------------
-// kernel__update_stress (m_kernel.f90:142-348): normal and shear parts fused
-// (they read V and write disjoint arrays).
+// Interior (m_kernel).  Tiles of 64 x BY columns over full-width aligned rows;
+// lanes outside [ib,ie] x [jb,je] idle.  Loop bodies are branch free so all
+// loads of an iteration issue together.
 // ---------------------------------------------------------------------------
-template <int NM>
-__global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
+
+// kernel__update_stress, m_kernel.f90:142-348 (normal and shear parts fused:
+// both read V and write disjoint arrays)
+template <int NM, bool NT>
+__global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
 {
     const Grid &g = a.g;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -64,9 +98,9 @@ __global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
     const int t = bid / a.nbx;
     const int byy = t % a.nby;
     const int bz = t / a.nby;
-    const int i = g.ib + bx * BX + threadIdx.x;
+    const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
-    if (i > g.ie || j > g.je) return;
+    if (i < g.ib || i > g.ie || j > g.je) return;
     const int k0 = g.kb + bz * a.kchunk;
     const int k1 = min(g.ke, k0 + a.kchunk - 1);
 
@@ -91,15 +125,14 @@ __global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
     float *__restrict__ Sxy = a.f.Sxy;
     const float dt = a.dt;
     const float d2 = a.v.d2;
+    const long long rn = g.rn;
 
     long long p = gidx(g, i, j, k0);
     float vx_m1 = Vx[p - P], vx_0 = Vx[p], vx_p1 = Vx[p + P];
     float vy_m1 = Vy[p - P], vy_0 = Vy[p], vy_p1 = Vy[p + P];
     float vz_m2 = Vz[p - 2 * P], vz_m1 = Vz[p - P], vz_0 = Vz[p];
     float mu00 = mu[p], mu10 = mu[p + 1], mu01 = mu[p + Y], mu11 = mu[p + 1 + Y];
-
-    long long r = ((long long)(k0 - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - g.ib);
-    const long long rn = g.rn;
+    long long r = ((long long)(k0 - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - 1);
 
     for (int k = k0; k <= k1; ++k, p += P, r += g.rplane) {
         const float vx_p2 = Vx[p + 2 * P], vy_p2 = Vy[p + 2 * P], vz_p1 = Vz[p + P];
@@ -120,15 +153,15 @@ __global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
         const float vz_im1 = Vz[p - 1], vz_ip1 = Vz[p + 1], vz_ip2 = Vz[p + 2];
         const float vz_jm1 = Vz[p - Y], vz_jp1 = Vz[p + Y], vz_jp2 = Vz[p + 2 * Y];
 
-        // ---- normal components, m_kernel.f90:198-238 ----
+        // normal components, m_kernel.f90:198-238
         const float dxVx = (vx_0 - vx_im1) * re40x - (vx_ip1 - vx_im2) * re41x;
         const float dyVy = (vy_0 - vy_jm1) * re40y - (vy_jp1 - vy_jm2) * re41y;
         const float dzVz = (vz_0 - vz_m1) * re40z - (vz_p1 - vz_m2) * re41z;
 
         const float mu2 = 2.0f * mu00;
-        const float lam2mu = lam[p] + mu2;
-        const float taup1 = taup[p];
-        const float taus1 = taus[p];
+        const float lam2mu = ld1<NT>(lam + p) + mu2;
+        const float taup1 = ld1<NT>(taup + p);
+        const float taus1 = ld1<NT>(taus + p);
 
         const float d3v3 = dxVx + dyVy + dzVz;
         const float dyVy_dzVz = dyVy + dzVz;
@@ -142,23 +175,23 @@ __global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
             float *ryy = a.R + (1 * NM + m) * rn + r;
             float *rzz = a.R + (2 * NM + m) * rn + r;
             const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-            const float nxx = c1 * *rxx - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
-            const float nyy = c1 * *ryy - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
-            const float nzz = c1 * *rzz - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
-            *rxx = nxx;
-            *ryy = nyy;
-            *rzz = nzz;
+            const float nxx = c1 * ld1<NT>(rxx) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
+            const float nyy = c1 * ld1<NT>(ryy) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
+            const float nzz = c1 * ld1<NT>(rzz) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
+            st1<NT>(rxx, nxx);
+            st1<NT>(ryy, nyy);
+            st1<NT>(rzz, nzz);
             Rxx_n = Rxx_n + d1 * nxx;
             Ryy_n = Ryy_n + d1 * nyy;
             Rzz_n = Rzz_n + d1 * nzz;
         }
         const float taup_plus1 = 1.0f + taup1 * (1.0f + d2);
         const float taus_plus1 = 1.0f + taus1 * (1.0f + d2);
-        Sxx[p] = Sxx[p] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
-        Syy[p] = Syy[p] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
-        Szz[p] = Szz[p] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
+        st1<NT>(Sxx + p, ld1<NT>(Sxx + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
+        st1<NT>(Syy + p, ld1<NT>(Syy + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
+        st1<NT>(Szz + p, ld1<NT>(Szz + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
 
-        // ---- shear components, m_kernel.f90:285-333 ----
+        // shear components, m_kernel.f90:285-333
         const float dxVy_dyVx = (vy_ip1 - vy_0) * re40x - (vy_ip2 - vy_im1) * re41x
                               + (vx_jp1 - vx_0) * re40y - (vx_jp2 - vx_jm1) * re41y;
         const float dxVz_dzVx = (vz_ip1 - vz_0) * re40x - (vz_ip2 - vz_im1) * re41x
@@ -177,19 +210,19 @@ __global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
             float *rxz = a.R + (4 * NM + m) * rn + r;
             float *rxy = a.R + (5 * NM + m) * rn + r;
             const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-            const float nyz = c1 * *ryz - c2 * muyz * taus1 * dyVz_dzVy * dt;
-            const float nxz = c1 * *rxz - c2 * muxz * taus1 * dxVz_dzVx * dt;
-            const float nxy = c1 * *rxy - c2 * muxy * taus1 * dxVy_dyVx * dt;
-            *ryz = nyz;
-            *rxz = nxz;
-            *rxy = nxy;
+            const float nyz = c1 * ld1<NT>(ryz) - c2 * muyz * taus1 * dyVz_dzVy * dt;
+            const float nxz = c1 * ld1<NT>(rxz) - c2 * muxz * taus1 * dxVz_dzVx * dt;
+            const float nxy = c1 * ld1<NT>(rxy) - c2 * muxy * taus1 * dxVy_dyVx * dt;
+            st1<NT>(ryz, nyz);
+            st1<NT>(rxz, nxz);
+            st1<NT>(rxy, nxy);
             Ryz_n = Ryz_n + d1 * nyz;
             Rxz_n = Rxz_n + d1 * nxz;
             Rxy_n = Rxy_n + d1 * nxy;
         }
-        Syz[p] = Syz[p] + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt;
-        Sxz[p] = Sxz[p] + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
-        Sxy[p] = Sxy[p] + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt;
+        st1<NT>(Syz + p, ld1<NT>(Syz + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
+        st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
+        st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
 
         // rotate the z queues
         vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
@@ -200,10 +233,8 @@ __global__ void __launch_bounds__(256) k_stress_interior(KArgs a)
     }
 }
 
-// ---------------------------------------------------------------------------
-// kernel__update_vel (m_kernel.f90:75-140)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_vel_interior(KArgs a)
+// kernel__update_vel, m_kernel.f90:75-140
+__global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
 {
     const Grid &g = a.g;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -211,9 +242,9 @@ __global__ void __launch_bounds__(256) k_vel_interior(KArgs a)
     const int t = bid / a.nbx;
     const int byy = t % a.nby;
     const int bz = t / a.nby;
-    const int i = g.ib + bx * BX + threadIdx.x;
+    const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
-    if (i > g.ie || j > g.je) return;
+    if (i < g.ib || i > g.ie || j > g.je) return;
     const int k0 = g.kb + bz * a.kchunk;
     const int k1 = min(g.ke, k0 + a.kchunk - 1);
 
@@ -282,7 +313,204 @@ __global__ void __launch_bounds__(256) k_vel_interior(KArgs a)
     }
 }
 
-static void kargs_fill(swpc_ctx *c, KArgs &a)
+// ---------------------------------------------------------------------------
+// ADE-CFS PML (m_absorb_p).  Tiles of 64 x BY columns from a host list cover
+// the PML cells: side columns (kbeg_a = 1) over their full depth, interior
+// columns over the bottom na planes (m_global.f90:334-343).  Each lane marches
+// the part of its column that is PML and lies in the tile's k chunk.  The 18
+// auxiliaries are numbered compactly box by box; the auxiliary of plane k in
+// a column is abase + k*astride.
+// ---------------------------------------------------------------------------
+struct PmlLane {
+    int i, j, kb, ke;
+    long long abase, astride;
+};
+
+__device__ static inline bool pml_lane(const StepArgs &a, PmlLane &L)
+{
+    const Grid &g = a.g;
+    const int4 tl = a.tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    // tile: x = first column | (width << 20), y = first row, z..w = k chunk.
+    // width 64: full rows; width na: one x slab, several rows per wave.
+    const int w = tl.x >> 20;
+    const int lane = threadIdx.y * blockDim.x + threadIdx.x;
+    const int rows = (blockDim.x * blockDim.y) / w;
+    if (lane >= rows * w) return false;
+    L.i = (tl.x & 0xFFFFF) + lane % w;
+    L.j = tl.y + lane / w;
+    if (L.i > g.nx || L.j > g.ny) return false;
+    const int na = g.na;
+    int q;
+    if (L.j <= na) q = 0;
+    else if (L.j > g.ny - na) q = 1;
+    else if (L.i <= na) q = 2;
+    else if (L.i > g.nx - na) q = 3;
+    else q = 4;
+    // each PML cell belongs to exactly one tile: x-slab cells of interior rows
+    // to the slab tiles, every other PML cell to the full-row tiles
+    if ((w == 64) == (q == 2 || q == 3)) return false;
+    const PmlBox b = q == 0 ? a.box[0] : q == 1 ? a.box[1] : q == 2 ? a.box[2] : q == 3 ? a.box[3] : a.box[4];
+    L.kb = max(tl.z, b.k0);
+    L.ke = min(tl.w, g.nz);
+    L.astride = (long long)b.nxb * b.nyb;
+    L.abase = b.off + (long long)(L.j - b.j0) * b.nxb + (L.i - b.i0) - (long long)b.k0 * L.astride;
+    return L.kb <= L.ke;
+}
+
+// absorb_p__update_stress, m_absorb_p.f90:448-519
+__global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
+{
+    PmlLane L;
+    if (!pml_lane(a, L)) return;
+    const Grid &g = a.g;
+    const int i = L.i, j = L.j;
+    const long long P = g.plane;
+    const int Y = g.pitch;
+    const float *__restrict__ Vx = a.f.Vx;
+    const float *__restrict__ Vy = a.f.Vy;
+    const float *__restrict__ Vz = a.f.Vz;
+    const float *__restrict__ mu = a.m.mu;
+    const float *__restrict__ lam = a.m.lam;
+    float *__restrict__ ade = a.ade;
+    const float dt = a.dt;
+    const long long n = a.npml;
+    const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
+    const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
+
+    long long p = gidx(g, i, j, L.kb);
+    long long q = L.abase + (long long)L.kb * L.astride;
+    float vx_0 = Vx[p], vy_0 = Vy[p], vz_m1 = Vz[p - P];
+    float mu00 = mu[p], mu10 = mu[p + 1], mu01 = mu[p + Y], mu11 = mu[p + 1 + Y];
+
+    for (int k = L.kb; k <= L.ke; ++k, p += P, q += L.astride) {
+        const float vx_p1 = Vx[p + P], vy_p1 = Vy[p + P], vz_0 = Vz[p];
+        const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
+        const float vx_im1 = Vx[p - 1], vy_jm1 = Vy[p - Y];
+        const float vy_ip1 = Vy[p + 1], vz_ip1 = Vz[p + 1];
+        const float vx_jp1 = Vx[p + Y], vz_jp1 = Vz[p + Y];
+        const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
+        const float lam0 = lam[p];
+        float *__restrict__ A = ade + q;
+        const float axVx = A[A_axVx * n], ayVy = A[A_ayVy * n], azVz = A[A_azVz * n];
+        const float ayVx = A[A_ayVx * n], azVx = A[A_azVx * n];
+        const float axVy = A[A_axVy * n], azVy = A[A_azVy * n];
+        const float axVz = A[A_axVz * n], ayVz = A[A_ayVz * n];
+
+        // normal components
+        const float dxVx = (vx_0 - vx_im1) * a.r20x;
+        const float dyVy = (vy_0 - vy_jm1) * a.r20y;
+        const float dzVz = (vz_0 - vz_m1) * a.r20z;
+        const float lam2mu_R = (lam0 + 2.0f * mu00);
+        const float lam_R = lam2mu_R - 2.0f * mu00;
+        const float dxVx_ade = gxc.x * dxVx + gxc.y * axVx;
+        const float dyVy_ade = gyc.x * dyVy + gyc.y * ayVy;
+        const float dzVz_ade = gzc.x * dzVz + gzc.y * azVz;
+        a.f.Sxx[p] = a.f.Sxx[p] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
+        a.f.Syy[p] = a.f.Syy[p] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
+        a.f.Szz[p] = a.f.Szz[p] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
+        A[A_axVx * n] = gxc.z * axVx + gxc.w * dxVx * dt;
+        A[A_ayVy * n] = gyc.z * ayVy + gyc.w * dyVy * dt;
+        A[A_azVz * n] = gzc.z * azVz + gzc.w * dzVz * dt;
+
+        // shear components
+        const float dxVy = (vy_ip1 - vy_0) * a.r20x;
+        const float dxVz = (vz_ip1 - vz_0) * a.r20x;
+        const float dyVx = (vx_jp1 - vx_0) * a.r20y;
+        const float dyVz = (vz_jp1 - vz_0) * a.r20y;
+        const float dzVx = (vx_p1 - vx_0) * a.r20z;
+        const float dzVy = (vy_p1 - vy_0) * a.r20z;
+        const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
+        const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
+        const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
+        a.f.Syz[p] = a.f.Syz[p] + muyz * (gye.x * dyVz + gze.x * dzVy + gye.y * ayVz + gze.y * azVy) * dt;
+        a.f.Sxz[p] = a.f.Sxz[p] + muxz * (gxe.x * dxVz + gze.x * dzVx + gxe.y * axVz + gze.y * azVx) * dt;
+        a.f.Sxy[p] = a.f.Sxy[p] + muxy * (gxe.x * dxVy + gye.x * dyVx + gxe.y * axVy + gye.y * ayVx) * dt;
+        A[A_ayVx * n] = gye.z * ayVx + gye.w * dyVx * dt;
+        A[A_azVx * n] = gze.z * azVx + gze.w * dzVx * dt;
+        A[A_axVy * n] = gxe.z * axVy + gxe.w * dxVy * dt;
+        A[A_azVy * n] = gze.z * azVy + gze.w * dzVy * dt;
+        A[A_axVz * n] = gxe.z * axVz + gxe.w * dxVz * dt;
+        A[A_ayVz * n] = gye.z * ayVz + gye.w * dyVz * dt;
+
+        vx_0 = vx_p1; vy_0 = vy_p1; vz_m1 = vz_0;
+        mu00 = mu00n; mu10 = mu10n; mu01 = mu01n;
+        mu11 = mu[p + P + 1 + Y];
+    }
+}
+
+// absorb_p__update_vel, m_absorb_p.f90:261-306
+__global__ void __launch_bounds__(256) k_vel_pml(StepArgs a)
+{
+    PmlLane L;
+    if (!pml_lane(a, L)) return;
+    const Grid &g = a.g;
+    const int i = L.i, j = L.j;
+    const long long P = g.plane;
+    const int Y = g.pitch;
+    const float *__restrict__ Sxx = a.f.Sxx;
+    const float *__restrict__ Syy = a.f.Syy;
+    const float *__restrict__ Szz = a.f.Szz;
+    const float *__restrict__ Syz = a.f.Syz;
+    const float *__restrict__ Sxz = a.f.Sxz;
+    const float *__restrict__ Sxy = a.f.Sxy;
+    const float *__restrict__ rho = a.m.rho;
+    float *__restrict__ ade = a.ade;
+    const float dt = a.dt;
+    const long long n = a.npml;
+    const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
+    const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
+
+    long long p = gidx(g, i, j, L.kb);
+    long long q = L.abase + (long long)L.kb * L.astride;
+    float szz_0 = Szz[p], syz_m1 = Syz[p - P], sxz_m1 = Sxz[p - P], rho0 = rho[p];
+
+    for (int k = L.kb; k <= L.ke; ++k, p += P, q += L.astride) {
+        const float szz_p1 = Szz[p + P], syz_0 = Syz[p], sxz_0 = Sxz[p], rho_kp = rho[p + P];
+        const float sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1];
+        const float syy_0 = Syy[p], syy_jp1 = Syy[p + Y];
+        const float syz_jm1 = Syz[p - Y], sxz_im1 = Sxz[p - 1];
+        const float sxy_0 = Sxy[p], sxy_im1 = Sxy[p - 1], sxy_jm1 = Sxy[p - Y];
+        const float rho_ip = rho[p + 1], rho_jp = rho[p + Y];
+        const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
+        float *__restrict__ A = ade + q;
+        const float axSxx = A[A_axSxx * n], aySxy = A[A_aySxy * n], azSxz = A[A_azSxz * n];
+        const float axSxy = A[A_axSxy * n], aySyy = A[A_aySyy * n], azSyz = A[A_azSyz * n];
+        const float axSxz = A[A_axSxz * n], aySyz = A[A_aySyz * n], azSzz = A[A_azSzz * n];
+
+        const float dxSxx = (sxx_ip1 - sxx_0) * a.r20x;
+        const float dySyy = (syy_jp1 - syy_0) * a.r20y;
+        const float dzSzz = (szz_p1 - szz_0) * a.r20z;
+        const float dySyz = (syz_0 - syz_jm1) * a.r20y;
+        const float dzSyz = (syz_0 - syz_m1) * a.r20z;
+        const float dxSxz = (sxz_0 - sxz_im1) * a.r20x;
+        const float dzSxz = (sxz_0 - sxz_m1) * a.r20z;
+        const float dxSxy = (sxy_0 - sxy_im1) * a.r20x;
+        const float dySxy = (sxy_0 - sxy_jm1) * a.r20y;
+        const float bx = 2.0f / (rho0 + rho_ip);
+        const float by = 2.0f / (rho0 + rho_jp);
+        const float bz = 2.0f / (rho0 + rho_kp);
+
+        a.f.Vx[p] = a.f.Vx[p] + bx * (gxe.x * dxSxx + gyc.x * dySxy + gzc.x * dzSxz
+                                      + gxe.y * axSxx + gyc.y * aySxy + gzc.y * azSxz) * dt;
+        a.f.Vy[p] = a.f.Vy[p] + by * (gxc.x * dxSxy + gye.x * dySyy + gzc.x * dzSyz
+                                      + gxc.y * axSxy + gye.y * aySyy + gzc.y * azSyz) * dt;
+        a.f.Vz[p] = a.f.Vz[p] + bz * (gxc.x * dxSxz + gyc.x * dySyz + gze.x * dzSzz
+                                      + gxc.y * axSxz + gyc.y * aySyz + gze.y * azSzz) * dt;
+        A[A_axSxx * n] = gxe.z * axSxx + gxe.w * dxSxx * dt;
+        A[A_aySxy * n] = gyc.z * aySxy + gyc.w * dySxy * dt;
+        A[A_azSxz * n] = gzc.z * azSxz + gzc.w * dzSxz * dt;
+        A[A_axSxy * n] = gxc.z * axSxy + gxc.w * dxSxy * dt;
+        A[A_aySyy * n] = gye.z * aySyy + gye.w * dySyy * dt;
+        A[A_azSyz * n] = gzc.z * azSyz + gzc.w * dzSyz * dt;
+        A[A_axSxz * n] = gxc.z * axSxz + gxc.w * dxSxz * dt;
+        A[A_aySyz * n] = gyc.z * aySyz + gyc.w * dySyz * dt;
+        A[A_azSzz * n] = gze.z * azSzz + gze.w * dzSzz * dt;
+
+        szz_0 = szz_p1; syz_m1 = syz_0; sxz_m1 = sxz_0; rho0 = rho_kp;
+    }
+}
+
+static void step_args(swpc_ctx *c, StepArgs &a)
 {
     a.g = c->g;
     a.f.Vx = c->fields[SWPC_VX]; a.f.Vy = c->fields[SWPC_VY]; a.f.Vz = c->fields[SWPC_VZ];
@@ -295,204 +523,6 @@ static void kargs_fill(swpc_ctx *c, KArgs &a)
     a.v = c->visco;
     a.c = c->coef;
     a.dt = c->desc.dt;
-    a.by = c->by;
-    a.kchunk = c->kchunk;
-    a.nbx = (c->g.nxk + BX - 1) / BX;
-    a.nby = (c->g.nyk + a.by - 1) / a.by;
-    a.nbz = (c->g.nzk + a.kchunk - 1) / a.kchunk;
-}
-
-hipError_t launch_kernel_stress(swpc_ctx *c)
-{
-    if (c->g.nxk <= 0 || c->g.nyk <= 0 || c->g.nzk <= 0) return hipSuccess;
-    KArgs a;
-    kargs_fill(c, a);
-    dim3 block(BX, a.by);
-    dim3 grid(a.nbx * a.nby * a.nbz);
-    switch (c->g.nm) {
-    case 0: hipLaunchKernelGGL(k_stress_interior<0>, grid, block, 0, c->stream, a); break;
-    case 1: hipLaunchKernelGGL(k_stress_interior<1>, grid, block, 0, c->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_stress_interior<2>, grid, block, 0, c->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_stress_interior<3>, grid, block, 0, c->stream, a); break;
-    case 4: hipLaunchKernelGGL(k_stress_interior<4>, grid, block, 0, c->stream, a); break;
-    case 5: hipLaunchKernelGGL(k_stress_interior<5>, grid, block, 0, c->stream, a); break;
-    case 6: hipLaunchKernelGGL(k_stress_interior<6>, grid, block, 0, c->stream, a); break;
-    case 7: hipLaunchKernelGGL(k_stress_interior<7>, grid, block, 0, c->stream, a); break;
-    case 8: hipLaunchKernelGGL(k_stress_interior<8>, grid, block, 0, c->stream, a); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_kernel_vel(swpc_ctx *c)
-{
-    if (c->g.nxk <= 0 || c->g.nyk <= 0 || c->g.nzk <= 0) return hipSuccess;
-    KArgs a;
-    kargs_fill(c, a);
-    dim3 block(BX, a.by);
-    dim3 grid(a.nbx * a.nby * a.nbz);
-    hipLaunchKernelGGL(k_vel_interior, grid, block, 0, c->stream, a);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// ADE-CFS PML (m_absorb_p.f90)
-// ---------------------------------------------------------------------------
-__device__ static inline bool pml_cell(const PmlArgs &a, long long t, int &i, int &j, int &k)
-{
-    if (t >= a.npml) return false;
-    PmlBox b = a.box[0];
-#pragma unroll
-    for (int q = 1; q < 5; ++q)
-        if (t >= a.box[q].off) b = a.box[q];
-    const long long l = t - b.off;
-    const long long plane = (long long)b.nxb * b.nyb;
-    const int kk = (int)(l / plane);
-    const int rem = (int)(l - (long long)kk * plane);
-    const int jj = rem / b.nxb;
-    const int ii = rem - jj * b.nxb;
-    i = b.i0 + ii;
-    j = b.j0 + jj;
-    k = b.k0 + kk;
-    return true;
-}
-
-// absorb_p__update_vel, m_absorb_p.f90:261-306
-__global__ void __launch_bounds__(256) k_pml_vel(PmlArgs a)
-{
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int i, j, k;
-    if (!pml_cell(a, t, i, j, k)) return;
-    const Grid &g = a.g;
-    const long long P = g.plane;
-    const int Y = g.pitch;
-    const long long p = gidx(g, i, j, k);
-    const float *Sxx = a.f.Sxx, *Syy = a.f.Syy, *Szz = a.f.Szz, *Syz = a.f.Syz, *Sxz = a.f.Sxz, *Sxy = a.f.Sxy;
-    const float *rho = a.m.rho;
-    const float dt = a.dt;
-    const long long n = a.npml;
-
-    const float dxSxx = (Sxx[p + 1] - Sxx[p]) * a.r20x;
-    const float dySyy = (Syy[p + Y] - Syy[p]) * a.r20y;
-    const float dzSzz = (Szz[p + P] - Szz[p]) * a.r20z;
-    const float dySyz = (Syz[p] - Syz[p - Y]) * a.r20y;
-    const float dzSyz = (Syz[p] - Syz[p - P]) * a.r20z;
-    const float dxSxz = (Sxz[p] - Sxz[p - 1]) * a.r20x;
-    const float dzSxz = (Sxz[p] - Sxz[p - P]) * a.r20z;
-    const float dxSxy = (Sxy[p] - Sxy[p - 1]) * a.r20x;
-    const float dySxy = (Sxy[p] - Sxy[p - Y]) * a.r20y;
-
-    const float r0 = rho[p];
-    const float bx = 2.0f / (r0 + rho[p + 1]);
-    const float by = 2.0f / (r0 + rho[p + Y]);
-    const float bz = 2.0f / (r0 + rho[p + P]);
-
-    const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
-    const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
-    const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
-
-    float *ade = a.ade;
-    float axSxx = ade[A_axSxx * n + t], aySxy = ade[A_aySxy * n + t], azSxz = ade[A_azSxz * n + t];
-    float axSxy = ade[A_axSxy * n + t], aySyy = ade[A_aySyy * n + t], azSyz = ade[A_azSyz * n + t];
-    float axSxz = ade[A_axSxz * n + t], aySyz = ade[A_aySyz * n + t], azSzz = ade[A_azSzz * n + t];
-
-    a.f.Vx[p] = a.f.Vx[p] + bx * (gxe.x * dxSxx + gyc.x * dySxy + gzc.x * dzSxz
-                                  + gxe.y * axSxx + gyc.y * aySxy + gzc.y * azSxz) * dt;
-    a.f.Vy[p] = a.f.Vy[p] + by * (gxc.x * dxSxy + gye.x * dySyy + gzc.x * dzSyz
-                                  + gxc.y * axSxy + gye.y * aySyy + gzc.y * azSyz) * dt;
-    a.f.Vz[p] = a.f.Vz[p] + bz * (gxc.x * dxSxz + gyc.x * dySyz + gze.x * dzSzz
-                                  + gxc.y * axSxz + gyc.y * aySyz + gze.y * azSzz) * dt;
-
-    ade[A_axSxx * n + t] = gxe.z * axSxx + gxe.w * dxSxx * dt;
-    ade[A_aySxy * n + t] = gyc.z * aySxy + gyc.w * dySxy * dt;
-    ade[A_azSxz * n + t] = gzc.z * azSxz + gzc.w * dzSxz * dt;
-    ade[A_axSxy * n + t] = gxc.z * axSxy + gxc.w * dxSxy * dt;
-    ade[A_aySyy * n + t] = gye.z * aySyy + gye.w * dySyy * dt;
-    ade[A_azSyz * n + t] = gzc.z * azSyz + gzc.w * dzSyz * dt;
-    ade[A_axSxz * n + t] = gxc.z * axSxz + gxc.w * dxSxz * dt;
-    ade[A_aySyz * n + t] = gyc.z * aySyz + gyc.w * dySyz * dt;
-    ade[A_azSzz * n + t] = gze.z * azSzz + gze.w * dzSzz * dt;
-}
-
-// absorb_p__update_stress, m_absorb_p.f90:448-519 (both k loops of a column
-// touch disjoint arrays, so one thread does both for its cell)
-__global__ void __launch_bounds__(256) k_pml_stress(PmlArgs a)
-{
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int i, j, k;
-    if (!pml_cell(a, t, i, j, k)) return;
-    const Grid &g = a.g;
-    const long long P = g.plane;
-    const int Y = g.pitch;
-    const long long p = gidx(g, i, j, k);
-    const float *Vx = a.f.Vx, *Vy = a.f.Vy, *Vz = a.f.Vz;
-    const float *mu = a.m.mu;
-    const float dt = a.dt;
-    const long long n = a.npml;
-    float *ade = a.ade;
-
-    const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
-    const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
-    const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
-
-    // normal components
-    {
-        const float vx0 = Vx[p], vy0 = Vy[p], vz0 = Vz[p];
-        const float dxVx = (vx0 - Vx[p - 1]) * a.r20x;
-        const float dyVy = (vy0 - Vy[p - Y]) * a.r20y;
-        const float dzVz = (vz0 - Vz[p - P]) * a.r20z;
-        const float mu0 = mu[p];
-        const float lam2mu_R = (a.m.lam[p] + 2.0f * mu0);
-        const float lam_R = lam2mu_R - 2.0f * mu0;
-        const float axVx = ade[A_axVx * n + t], ayVy = ade[A_ayVy * n + t], azVz = ade[A_azVz * n + t];
-        const float dxVx_ade = gxc.x * dxVx + gxc.y * axVx;
-        const float dyVy_ade = gyc.x * dyVy + gyc.y * ayVy;
-        const float dzVz_ade = gzc.x * dzVz + gzc.y * azVz;
-        a.f.Sxx[p] = a.f.Sxx[p] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
-        a.f.Syy[p] = a.f.Syy[p] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
-        a.f.Szz[p] = a.f.Szz[p] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
-        ade[A_axVx * n + t] = gxc.z * axVx + gxc.w * dxVx * dt;
-        ade[A_ayVy * n + t] = gyc.z * ayVy + gyc.w * dyVy * dt;
-        ade[A_azVz * n + t] = gzc.z * azVz + gzc.w * dzVz * dt;
-    }
-    // shear components
-    {
-        const float vx0 = Vx[p], vy0 = Vy[p], vz0 = Vz[p];
-        const float dxVy = (Vy[p + 1] - vy0) * a.r20x;
-        const float dxVz = (Vz[p + 1] - vz0) * a.r20x;
-        const float dyVx = (Vx[p + Y] - vx0) * a.r20y;
-        const float dyVz = (Vz[p + Y] - vz0) * a.r20y;
-        const float dzVx = (Vx[p + P] - vx0) * a.r20z;
-        const float dzVy = (Vy[p + P] - vy0) * a.r20z;
-        const float m000 = mu[p], m100 = mu[p + P], m010 = mu[p + 1], m110 = mu[p + P + 1];
-        const float m001 = mu[p + Y], m101 = mu[p + P + Y], m011 = mu[p + 1 + Y];
-        const float muxz = hmean4(m000, m100, m010, m110, EPSL);
-        const float muxy = hmean4(m000, m010, m001, m011, EPSL);
-        const float muyz = hmean4(m000, m100, m001, m101, EPSL);
-        const float ayVx = ade[A_ayVx * n + t], azVx = ade[A_azVx * n + t];
-        const float axVy = ade[A_axVy * n + t], azVy = ade[A_azVy * n + t];
-        const float axVz = ade[A_axVz * n + t], ayVz = ade[A_ayVz * n + t];
-        a.f.Syz[p] = a.f.Syz[p] + muyz * (gye.x * dyVz + gze.x * dzVy + gye.y * ayVz + gze.y * azVy) * dt;
-        a.f.Sxz[p] = a.f.Sxz[p] + muxz * (gxe.x * dxVz + gze.x * dzVx + gxe.y * axVz + gze.y * azVx) * dt;
-        a.f.Sxy[p] = a.f.Sxy[p] + muxy * (gxe.x * dxVy + gye.x * dyVx + gxe.y * axVy + gye.y * ayVx) * dt;
-        ade[A_ayVx * n + t] = gye.z * ayVx + gye.w * dyVx * dt;
-        ade[A_azVx * n + t] = gze.z * azVx + gze.w * dzVx * dt;
-        ade[A_axVy * n + t] = gxe.z * axVy + gxe.w * dxVy * dt;
-        ade[A_azVy * n + t] = gze.z * azVy + gze.w * dzVy * dt;
-        ade[A_axVz * n + t] = gxe.z * axVz + gxe.w * dxVz * dt;
-        ade[A_ayVz * n + t] = gye.z * ayVz + gye.w * dyVz * dt;
-    }
-}
-
-static void pmlargs_fill(swpc_ctx *c, PmlArgs &a)
-{
-    a.g = c->g;
-    a.f.Vx = c->fields[SWPC_VX]; a.f.Vy = c->fields[SWPC_VY]; a.f.Vz = c->fields[SWPC_VZ];
-    a.f.Sxx = c->fields[SWPC_SXX]; a.f.Syy = c->fields[SWPC_SYY]; a.f.Szz = c->fields[SWPC_SZZ];
-    a.f.Syz = c->fields[SWPC_SYZ]; a.f.Sxz = c->fields[SWPC_SXZ]; a.f.Sxy = c->fields[SWPC_SXY];
-    a.m.rho = c->rho; a.m.lam = c->lam; a.m.mu = c->mu; a.m.taup = c->taup; a.m.taus = c->taus;
-    a.m.kfs_top = c->kfs_top; a.m.kfs_bot = c->kfs_bot; a.m.kob_top = c->kob_top; a.m.kob_bot = c->kob_bot;
-    a.m.kob = c->kob;
     a.ade = c->ade;
     a.npml = c->npml;
     for (int q = 0; q < 5; ++q) a.box[q] = c->box[q];
@@ -500,28 +530,67 @@ static void pmlargs_fill(swpc_ctx *c, PmlArgs &a)
     a.r20x = 1.0f / (float)c->desc.dx;
     a.r20y = 1.0f / (float)c->desc.dy;
     a.r20z = 1.0f / (float)c->desc.dz;
-    a.dt = c->desc.dt;
+    a.tiles = (const int4 *)c->d_pml_tiles;
+    a.ntiles = c->npml_tiles;
+    a.by = c->by;
+    a.kchunk = c->kchunk;
+    const Grid &g = c->g;
+    a.bx0 = (g.ib - 1) / BX;
+    a.nbx = (g.ie - 1) / BX - a.bx0 + 1;
+    a.nby = (g.nyk + a.by - 1) / a.by;
+    a.nbz = (g.nzk + a.kchunk - 1) / a.kchunk;
 }
 
-hipError_t launch_pml_vel(swpc_ctx *c)
+hipError_t launch_stress(swpc_ctx *c, int mask)
 {
-    if (c->npml <= 0) return hipSuccess;
-    PmlArgs a;
-    pmlargs_fill(c, a);
-    const long long nb = (c->npml + 255) / 256;
-    hipLaunchKernelGGL(k_pml_vel, dim3((unsigned)nb), dim3(256), 0, c->stream, a);
-    return hipGetLastError();
+    StepArgs a;
+    step_args(c, a);
+    dim3 block(BX, a.by);
+    if ((mask & MASK_INT) && c->g.nxk > 0 && c->g.nyk > 0 && c->g.nzk > 0) {
+        dim3 grid(a.nbx * a.nby * a.nbz);
+        const bool nt = (c->variant & 1) != 0;
+#define SWPC_STRESS_CASE(NMV)                                                                         \
+    case NMV:                                                                                         \
+        if (nt) hipLaunchKernelGGL((k_stress_int<NMV, true>), grid, block, 0, c->stream, a);        \
+        else hipLaunchKernelGGL((k_stress_int<NMV, false>), grid, block, 0, c->stream, a);          \
+        break;
+        switch (c->g.nm) {
+            SWPC_STRESS_CASE(0) SWPC_STRESS_CASE(1) SWPC_STRESS_CASE(2) SWPC_STRESS_CASE(3) SWPC_STRESS_CASE(4)
+            SWPC_STRESS_CASE(5) SWPC_STRESS_CASE(6) SWPC_STRESS_CASE(7) SWPC_STRESS_CASE(8)
+        default: return hipErrorInvalidValue;
+        }
+#undef SWPC_STRESS_CASE
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if ((mask & MASK_PML) && c->npml > 0 && c->npml_tiles > 0) {
+        hipLaunchKernelGGL(k_stress_pml, dim3(c->npml_tiles), block, 0, c->stream, a);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
-hipError_t launch_pml_stress(swpc_ctx *c)
+hipError_t launch_vel(swpc_ctx *c, int mask)
 {
-    if (c->npml <= 0) return hipSuccess;
-    PmlArgs a;
-    pmlargs_fill(c, a);
-    const long long nb = (c->npml + 255) / 256;
-    hipLaunchKernelGGL(k_pml_stress, dim3((unsigned)nb), dim3(256), 0, c->stream, a);
-    return hipGetLastError();
+    StepArgs a;
+    step_args(c, a);
+    dim3 block(BX, a.by);
+    if ((mask & MASK_INT) && c->g.nxk > 0 && c->g.nyk > 0 && c->g.nzk > 0) {
+        hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), block, 0, c->stream, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if ((mask & MASK_PML) && c->npml > 0 && c->npml_tiles > 0) {
+        hipLaunchKernelGGL(k_vel_pml, dim3(c->npml_tiles), block, 0, c->stream, a);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
+
+hipError_t launch_kernel_stress(swpc_ctx *c) { return launch_stress(c, MASK_INT); }
+hipError_t launch_pml_stress(swpc_ctx *c) { return launch_stress(c, MASK_PML); }
+hipError_t launch_kernel_vel(swpc_ctx *c) { return launch_vel(c, MASK_INT); }
+hipError_t launch_pml_vel(swpc_ctx *c) { return launch_vel(c, MASK_PML); }
 
 // ---------------------------------------------------------------------------
 // Cerjan sponge (m_absorb_c.f90:113-198).  Profiles carry the 3-cell halo:

@@ -3,10 +3,9 @@
 // Device layout (all FP32, x fastest):
 //   3D halo arrays (fields V3/S6, medium rho/lam/mu/taup/taus):
 //     idx(i,j,k) = (k+2)*plane + (j+2)*pitch + (i-1+ox),  i,j,k in -2..n+3
-//     ox is chosen so that i = na+1 (first interior column) starts a 128-byte
-//     line; pitch is a multiple of 32 floats.
-//   Memory variables: 6*NM arrays on the kernel box only,
-//     ridx(i,j,k) = ((k-kb)*nyk + (j-jb))*rpitch + (i-ib).
+//     ox = 32 puts i = 1 on a 128-byte line; pitch is a multiple of 32 floats.
+//   Memory variables: 6*NM arrays over the kernel rows j in [jb,je], planes
+//     k in [kb,ke], with full-width rows: ridx(i,j,k) = ((k-kb)*nyk + (j-jb))*rpitch + (i-1).
 //   PML auxiliaries: 18 arrays over the PML cells only, indexed by a compact
 //     cell number running over five boxes (two y slabs, two x slabs, bottom).
 //   (i,j) maps: (j+2)*pitch + (i-1+ox).
@@ -73,17 +72,6 @@ enum {
     A_N
 };
 
-struct PmlArgs {
-    Grid g;
-    FieldPtrs f;
-    MediumPtrs m;
-    float *ade;               // A_N arrays of npml floats
-    long long npml;
-    PmlBox box[5];
-    const float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;  // g(1:4, n), index n-1
-    float r20x, r20y, r20z;   // 1/dx etc. (m_absorb_p.f90:70-72)
-    float dt;
-};
 
 struct SrcTarget {
     long long idx;   // device index in the field array
@@ -117,6 +105,8 @@ struct swpc_ctx {
     long long npml;
     float *ade;
     float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
+    int *d_pml_tiles;  // int4 (i0, j0, k0, k1) per PML tile
+    int npml_tiles;
     // Cerjan
     float *cgx_c, *cgx_b, *cgy_c, *cgy_b, *cgz_c, *cgz_b;
 
@@ -150,10 +140,13 @@ struct swpc_ctx {
     float *d_vmax;
 
     int kchunk, by;
+    int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel
     long long bytes;
 };
 
 // ---- launchers (stencil.hip / aux.hip) ----
+hipError_t launch_stress(swpc_ctx *c, int mask);  // mask: 1 interior, 2 PML, 3 both
+hipError_t launch_vel(swpc_ctx *c, int mask);
 hipError_t launch_kernel_stress(swpc_ctx *c);
 hipError_t launch_kernel_vel(swpc_ctx *c);
 hipError_t launch_pml_stress(swpc_ctx *c);

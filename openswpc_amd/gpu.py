@@ -68,7 +68,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_set_sources.argtypes = [vp, i, i, ip, ip, ip, fp, fp, f, i, fp]
     L.swpc_set_receivers.argtypes = [vp, i, ip, ip, ip, i, i, i, f, f]
     for nm in ("swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_kernel_update_vel",
-               "swpc_absorb_update_vel", "swpc_synchronize"):
+               "swpc_absorb_update_vel", "swpc_update_stress", "swpc_update_vel", "swpc_synchronize"):
         getattr(L, nm).argtypes = [vp]
     for nm in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store"):
         getattr(L, nm).argtypes = [vp, i]
@@ -79,7 +79,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_get_seismograms.argtypes = [vp, i, fp]
     L.swpc_device_bytes.argtypes = [vp]
     L.swpc_device_bytes.restype = C.c_int64
-    L.swpc_set_tuning.argtypes = [vp, i, i]
+    L.swpc_set_tuning.argtypes = [vp, i, i, i]
     L.swpc_stream.argtypes = [vp]
     L.swpc_stream.restype = vp
     _lib = L
@@ -164,6 +164,14 @@ class SwpcGPU:
     def absorb_update_stress(self):
         self._chk(self.L.swpc_absorb_update_stress(self.h))
 
+    def update_stress(self):
+        """kernel__update_stress + absorb__update_stress in one fused launch"""
+        self._chk(self.L.swpc_update_stress(self.h))
+
+    def update_vel(self):
+        """kernel__update_vel + absorb__update_vel in one fused launch"""
+        self._chk(self.L.swpc_update_vel(self.h))
+
     def source_stressglut(self, it):
         self._chk(self.L.swpc_source_stressglut(self.h, int(it)))
 
@@ -209,8 +217,8 @@ class SwpcGPU:
     def device_bytes(self):
         return int(self.L.swpc_device_bytes(self.h))
 
-    def set_tuning(self, kchunk=0, by=0):
-        self._chk(self.L.swpc_set_tuning(self.h, int(kchunk), int(by)))
+    def set_tuning(self, kchunk=0, by=0, variant=-1):
+        self._chk(self.L.swpc_set_tuning(self.h, int(kchunk), int(by), int(variant)))
 
     def stream(self):
         return self.L.swpc_stream(self.h)

@@ -5,12 +5,12 @@ import numpy as np
 from oracle import oracle as O
 
 
-def gpu_from_model(m, kchunk=0, by=0):
+def gpu_from_model(m, kchunk=0, by=0, variant=-1):
     from openswpc_amd.gpu import SwpcGPU
     cfg = m.cfg
     g = SwpcGPU(cfg.nx, cfg.ny, cfg.nz, cfg.na, cfg.nm, cfg.abc, cfg.dx, cfg.dy, cfg.dz, cfg.dt)
-    if kchunk or by:
-        g.set_tuning(kchunk, by)
+    if kchunk or by or variant >= 0:
+        g.set_tuning(kchunk, by, variant)
     g.set_medium(m.rho, m.lam, m.mu, m.taup, m.taus, m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)
     g.set_visco(m.c1, m.c2, m.d1, m.d2)
     if cfg.abc == "pml":

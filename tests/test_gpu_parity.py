@@ -116,12 +116,27 @@ def test_vmax_matches(orc):
     assert np.array_equal(v_g, v_o), (v_g, v_o)
 
 
-@pytest.mark.parametrize("kchunk,by", [(7, 1), (16, 2), (200, 4)])
-def test_launch_shapes_same_result(orc, kchunk, by):
+@pytest.mark.parametrize("kchunk,by,variant", [(7, 1, 0), (16, 2, 1), (200, 4, 1), (64, 4, 0)])
+def test_launch_shapes_same_result(orc, kchunk, by, variant):
     cfg = small_config(nx=44, ny=40, nz=36)
     m = O.build_model(cfg)
     run = O.OracleRun(m, 32)
     run.run(20)
-    g = gpu_from_model(m, kchunk=kchunk, by=by)
+    g = gpu_from_model(m, kchunk=kchunk, by=by, variant=variant)
     g.step(1, 20)
-    _check(g, run, f"kchunk={kchunk} by={by}")
+    _check(g, run, f"kchunk={kchunk} by={by} variant={variant}")
+
+
+@pytest.mark.parametrize("na,by", [(7, 4), (5, 2), (13, 1)])
+def test_odd_sizes_and_absorber_widths(orc, na, by):
+    """PML tiles where na is not a multiple of the tile height and the grid
+    is not a multiple of the tile width: every PML cell updated exactly once."""
+    cfg = small_config(nx=37, ny=41, nz=29, na=na, nt=25,
+                       sources=[dict(x=0.3, y=-0.6, z=4.2, t0=0.0, tr=0.6, mo=1e15, strike=30.0, dip=60.0, rake=90.0)],
+                       stations=[(1.1, 1.3, 0.0, 'fsb')])
+    m = O.build_model(cfg)
+    run = O.OracleRun(m, 32)
+    run.run(25)
+    g = gpu_from_model(m, kchunk=8, by=by, variant=1)
+    g.step(1, 25)
+    _check(g, run, f"na={na} by={by}")
