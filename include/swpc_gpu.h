@@ -159,8 +159,10 @@ int64_t swpc_device_bytes(const swpc_ctx *ctx);
 
 /* Launch-shape tuning knobs.  kchunk: k-planes marched per thread block
    (0 = keep); by: rows per 64-wide tile, 1, 2 or 4 (0 = keep); variant:
-   kernel variant flags, bit 0 = non-temporal memory-variable/stress streams
-   (-1 = keep). */
+   kernel variant flags (-1 = keep; default 1): bit 0 = non-temporal loads and
+   stores for the single-use streams of the stress update, bit 1 = side-column
+   PML kernels on a second stream alongside the interior ones.  All variants
+   give bit-identical results. */
 int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a

@@ -63,30 +63,26 @@ static int build_pml_tiles(swpc_ctx *c)
     if (c->desc.abc != SWPC_ABC_PML || c->npml == 0) return SWPC_OK;
     const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz, na = c->g.na, by = c->by, kc = c->kchunk;
     std::vector<int> t;
-    auto add = [&](int i0, int w, int j0, int k0, int k1) {
-        t.push_back(i0 | (w << 20)); t.push_back(j0); t.push_back(k0); t.push_back(k1);
+    // tile = (i0 | width << 20 | xslab << 28, j0, k0, k1)
+    auto add = [&](int i0, int w, int xslab, int j0, int k0, int k1) {
+        t.push_back(i0 | (w << 20) | (xslab << 28)); t.push_back(j0); t.push_back(k0); t.push_back(k1);
     };
     // x slabs (side columns of interior rows), width na, several rows per block
     const int slab_rows = (64 * by) / std::max(na, 1);
     const bool slab_tiles = na > 0 && na <= 64 && slab_rows > 0;
     for (int k0 = 1; k0 <= nz; k0 += kc) {
         const int k1 = std::min(nz, k0 + kc - 1);
-        // full-row tiles: y-slab rows over the full depth, interior rows over
-        // the bottom na planes (x-slab columns of interior rows excluded)
+        // y-slab rows as full-row tiles over the full depth; the bottom band
+        // of interior columns is marched by the interior kernels
         for (int j0 = 1; j0 <= ny; j0 += by) {
             const int jl = std::min(j0 + by - 1, ny);
-            const bool yslab = j0 <= na || jl > ny - na;
-            const bool inner_rows = jl > na && j0 <= ny - na;
-            for (int i0 = 1; i0 <= nx; i0 += 64) {
-                const int il = std::min(i0 + 63, nx);
-                const bool inner_cols = il > na && i0 <= nx - na;
-                if (yslab || (inner_rows && inner_cols && k1 >= nz - na + 1)) add(i0, 64, j0, k0, k1);
-            }
+            if (j0 <= na || jl > ny - na)
+                for (int i0 = 1; i0 <= nx; i0 += 64) add(i0, 64, 0, j0, k0, k1);
         }
         if (slab_tiles)
             for (int j0 = na + 1; j0 <= ny - na; j0 += slab_rows) {
-                add(1, na, j0, k0, k1);
-                add(nx - na + 1, na, j0, k0, k1);
+                add(1, na, 1, j0, k0, k1);
+                add(nx - na + 1, na, 1, j0, k0, k1);
             }
     }
     if (!slab_tiles && na > 0) return fail(c, SWPC_EINVAL, "PML thickness na must be 1..64");
@@ -122,8 +118,11 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     c->kchunk = 64;
     c->by = 4;
     c->variant = 1;
-    if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-        delete c;
+    if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipStreamCreate(&c->stream2) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        swpc_destroy(c);
         return SWPC_EHIP;
     }
     Grid &g = c->g;
@@ -248,6 +247,10 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     dfree(c->d_ist); dfree(c->d_jst); dfree(c->d_kst); dfree(c->d_uacc); dfree(c->d_eacc);
     for (int q = 0; q < 4; ++q) dfree(c->d_wav[q]);
     dfree(c->staging); dfree(c->d_vmax); dfree(c->d_it);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWPC_OK;
@@ -564,9 +567,9 @@ extern "C" int swpc_update_stress(swpc_ctx *c)
     int rc = ready(c);
     if (rc) return rc;
     if (c->desc.abc == SWPC_ABC_PML) {
-        HIPCHK(c, launch_stress(c, 3));
+        HIPCHK(c, launch_stress(c, 3, nullptr));
     } else {
-        HIPCHK(c, launch_stress(c, 1));
+        HIPCHK(c, launch_stress(c, 1, nullptr));
         HIPCHK(c, launch_cerjan_stress(c));
     }
     return SWPC_OK;
@@ -578,9 +581,9 @@ extern "C" int swpc_update_vel(swpc_ctx *c)
     int rc = ready(c);
     if (rc) return rc;
     if (c->desc.abc == SWPC_ABC_PML) {
-        HIPCHK(c, launch_vel(c, 3));
+        HIPCHK(c, launch_vel(c, 3, nullptr));
     } else {
-        HIPCHK(c, launch_vel(c, 1));
+        HIPCHK(c, launch_vel(c, 1, nullptr));
         HIPCHK(c, launch_cerjan_vel(c));
     }
     return SWPC_OK;
@@ -617,11 +620,28 @@ static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it)
     // is a separate sweep over the updated stress, as in the reference.
     hipError_t e;
     const bool pml = c->desc.abc == SWPC_ABC_PML;
+    const bool fork = pml && (c->variant & 2) && c->npml_tiles > 0;
     if ((e = launch_wav_store(c, it, d_it)) != hipSuccess) return e;
-    if ((e = launch_stress(c, pml ? 3 : 1)) != hipSuccess) return e;
+    if (fork) {
+        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
+    }
+    if ((e = launch_stress(c, pml ? 3 : 1, fork ? c->stream2 : nullptr)) != hipSuccess) return e;
+    if (fork) {
+        if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+    }
     if (!pml && (e = launch_cerjan_stress(c)) != hipSuccess) return e;
     if ((e = launch_stressglut(c, it, d_it)) != hipSuccess) return e;
-    if ((e = launch_vel(c, pml ? 3 : 1)) != hipSuccess) return e;
+    if (fork) {
+        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
+    }
+    if ((e = launch_vel(c, pml ? 3 : 1, fork ? c->stream2 : nullptr)) != hipSuccess) return e;
+    if (fork) {
+        if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+    }
     if (!pml && (e = launch_cerjan_vel(c)) != hipSuccess) return e;
     if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess) return e;
     if (d_it) e = launch_step_advance(c);

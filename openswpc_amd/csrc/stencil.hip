@@ -75,22 +75,166 @@ struct StepArgs {
     PmlBox box[5];
     const float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
     float r20x, r20y, r20z;
-    const int4 *tiles;   // PML tiles: (i0, j0, k0, k1)
+    const int4 *tiles;   // side-column PML tiles: (i0 | width << 20, j0, k0, k1)
     int ntiles;
+    int has_pml, mask;
     // interior launch shape
     int kchunk, bx0, nbx, nby, nbz, by;
 };
 
 // ---------------------------------------------------------------------------
-// Interior (m_kernel).  Tiles of 64 x BY columns over full-width aligned rows;
-// lanes outside [ib,ie] x [jb,je] idle.  Loop bodies are branch free so all
-// loads of an iteration issue together.
+// ADE-CFS PML cell updates (m_absorb_p).  A column's auxiliaries are numbered
+// compactly box by box; the auxiliary of plane k is q = abase + k*astride.
+// ---------------------------------------------------------------------------
+__device__ static inline PmlBox pick_box(const StepArgs &a, int q)
+{
+    return q == 0 ? a.box[0] : q == 1 ? a.box[1] : q == 2 ? a.box[2] : q == 3 ? a.box[3] : a.box[4];
+}
+
+// absorb_p__update_stress for one cell, m_absorb_p.f90:455-517.  The caller
+// passes its z-queue values: V at k (and k+1 for Vx, Vy; k-1 for Vz) and mu
+// on planes k, k+1.
+template <bool NT>
+__device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, long long q, int k, float vx_0,
+                                                float vx_p1, float vy_0, float vy_p1, float vz_m1, float vz_0,
+                                                float mu00, float mu00n, float mu10, float mu10n, float mu01,
+                                                float mu01n, float mu11, float4 gxc, float4 gxe, float4 gyc,
+                                                float4 gye)
+{
+    const long long P = a.g.plane;
+    const int Y = a.g.pitch;
+    const float *__restrict__ Vx = a.f.Vx;
+    const float *__restrict__ Vy = a.f.Vy;
+    const float *__restrict__ Vz = a.f.Vz;
+    const long long n = a.npml;
+    float *__restrict__ A = a.ade + q;
+    const float dt = a.dt;
+    (void)P;
+    const float vx_im1 = Vx[p - 1], vy_jm1 = Vy[p - Y];
+    const float vy_ip1 = Vy[p + 1], vz_ip1 = Vz[p + 1];
+    const float vx_jp1 = Vx[p + Y], vz_jp1 = Vz[p + Y];
+    const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
+    const float lam0 = ld1<NT>(a.m.lam + p);
+    const float axVx = ld1<NT>(A + A_axVx * n), ayVy = ld1<NT>(A + A_ayVy * n), azVz = ld1<NT>(A + A_azVz * n);
+    const float ayVx = ld1<NT>(A + A_ayVx * n), azVx = ld1<NT>(A + A_azVx * n);
+    const float axVy = ld1<NT>(A + A_axVy * n), azVy = ld1<NT>(A + A_azVy * n);
+    const float axVz = ld1<NT>(A + A_axVz * n), ayVz = ld1<NT>(A + A_ayVz * n);
+    float *__restrict__ Sxx = a.f.Sxx;
+    float *__restrict__ Syy = a.f.Syy;
+    float *__restrict__ Szz = a.f.Szz;
+    float *__restrict__ Syz = a.f.Syz;
+    float *__restrict__ Sxz = a.f.Sxz;
+    float *__restrict__ Sxy = a.f.Sxy;
+
+    // normal components
+    const float dxVx = (vx_0 - vx_im1) * a.r20x;
+    const float dyVy = (vy_0 - vy_jm1) * a.r20y;
+    const float dzVz = (vz_0 - vz_m1) * a.r20z;
+    const float lam2mu_R = (lam0 + 2.0f * mu00);
+    const float lam_R = lam2mu_R - 2.0f * mu00;
+    const float dxVx_ade = gxc.x * dxVx + gxc.y * axVx;
+    const float dyVy_ade = gyc.x * dyVy + gyc.y * ayVy;
+    const float dzVz_ade = gzc.x * dzVz + gzc.y * azVz;
+    st1<NT>(Sxx + p, ld1<NT>(Sxx + p) + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt);
+    st1<NT>(Syy + p, ld1<NT>(Syy + p) + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt);
+    st1<NT>(Szz + p, ld1<NT>(Szz + p) + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt);
+    st1<NT>(A + A_axVx * n, gxc.z * axVx + gxc.w * dxVx * dt);
+    st1<NT>(A + A_ayVy * n, gyc.z * ayVy + gyc.w * dyVy * dt);
+    st1<NT>(A + A_azVz * n, gzc.z * azVz + gzc.w * dzVz * dt);
+
+    // shear components
+    const float dxVy = (vy_ip1 - vy_0) * a.r20x;
+    const float dxVz = (vz_ip1 - vz_0) * a.r20x;
+    const float dyVx = (vx_jp1 - vx_0) * a.r20y;
+    const float dyVz = (vz_jp1 - vz_0) * a.r20y;
+    const float dzVx = (vx_p1 - vx_0) * a.r20z;
+    const float dzVy = (vy_p1 - vy_0) * a.r20z;
+    const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
+    const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
+    const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
+    st1<NT>(Syz + p, ld1<NT>(Syz + p) + muyz * (gye.x * dyVz + gze.x * dzVy + gye.y * ayVz + gze.y * azVy) * dt);
+    st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + muxz * (gxe.x * dxVz + gze.x * dzVx + gxe.y * axVz + gze.y * azVx) * dt);
+    st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + muxy * (gxe.x * dxVy + gye.x * dyVx + gxe.y * axVy + gye.y * ayVx) * dt);
+    st1<NT>(A + A_ayVx * n, gye.z * ayVx + gye.w * dyVx * dt);
+    st1<NT>(A + A_azVx * n, gze.z * azVx + gze.w * dzVx * dt);
+    st1<NT>(A + A_axVy * n, gxe.z * axVy + gxe.w * dxVy * dt);
+    st1<NT>(A + A_azVy * n, gze.z * azVy + gze.w * dzVy * dt);
+    st1<NT>(A + A_axVz * n, gxe.z * axVz + gxe.w * dxVz * dt);
+    st1<NT>(A + A_ayVz * n, gye.z * ayVz + gye.w * dyVz * dt);
+}
+
+// absorb_p__update_vel for one cell, m_absorb_p.f90:267-304.  Queue values:
+// Szz on k, k+1; Syz, Sxz on k-1, k; rho on k, k+1.
+__device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, long long q, int k, float szz_0,
+                                             float szz_p1, float syz_m1, float syz_0, float sxz_m1, float sxz_0,
+                                             float rho0, float rho_kp, float4 gxc, float4 gxe, float4 gyc,
+                                             float4 gye)
+{
+    const int Y = a.g.pitch;
+    const float *__restrict__ Sxx = a.f.Sxx;
+    const float *__restrict__ Syy = a.f.Syy;
+    const float *__restrict__ Syz = a.f.Syz;
+    const float *__restrict__ Sxz = a.f.Sxz;
+    const float *__restrict__ Sxy = a.f.Sxy;
+    const float *__restrict__ rho = a.m.rho;
+    const long long n = a.npml;
+    float *__restrict__ A = a.ade + q;
+    const float dt = a.dt;
+    const float sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1];
+    const float syy_0 = Syy[p], syy_jp1 = Syy[p + Y];
+    const float syz_jm1 = Syz[p - Y], sxz_im1 = Sxz[p - 1];
+    const float sxy_0 = Sxy[p], sxy_im1 = Sxy[p - 1], sxy_jm1 = Sxy[p - Y];
+    const float rho_ip = rho[p + 1], rho_jp = rho[p + Y];
+    const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
+    const float axSxx = ld1<true>(A + A_axSxx * n), aySxy = ld1<true>(A + A_aySxy * n), azSxz = ld1<true>(A + A_azSxz * n);
+    const float axSxy = ld1<true>(A + A_axSxy * n), aySyy = ld1<true>(A + A_aySyy * n), azSyz = ld1<true>(A + A_azSyz * n);
+    const float axSxz = ld1<true>(A + A_axSxz * n), aySyz = ld1<true>(A + A_aySyz * n), azSzz = ld1<true>(A + A_azSzz * n);
+
+    const float dxSxx = (sxx_ip1 - sxx_0) * a.r20x;
+    const float dySyy = (syy_jp1 - syy_0) * a.r20y;
+    const float dzSzz = (szz_p1 - szz_0) * a.r20z;
+    const float dySyz = (syz_0 - syz_jm1) * a.r20y;
+    const float dzSyz = (syz_0 - syz_m1) * a.r20z;
+    const float dxSxz = (sxz_0 - sxz_im1) * a.r20x;
+    const float dzSxz = (sxz_0 - sxz_m1) * a.r20z;
+    const float dxSxy = (sxy_0 - sxy_im1) * a.r20x;
+    const float dySxy = (sxy_0 - sxy_jm1) * a.r20y;
+    const float bx = 2.0f / (rho0 + rho_ip);
+    const float by = 2.0f / (rho0 + rho_jp);
+    const float bz = 2.0f / (rho0 + rho_kp);
+
+    float *__restrict__ Vx = a.f.Vx;
+    float *__restrict__ Vy = a.f.Vy;
+    float *__restrict__ Vz = a.f.Vz;
+    st1<true>(Vx + p, ld1<true>(Vx + p) + bx * (gxe.x * dxSxx + gyc.x * dySxy + gzc.x * dzSxz
+                                                + gxe.y * axSxx + gyc.y * aySxy + gzc.y * azSxz) * dt);
+    st1<true>(Vy + p, ld1<true>(Vy + p) + by * (gxc.x * dxSxy + gye.x * dySyy + gzc.x * dzSyz
+                                                + gxc.y * axSxy + gye.y * aySyy + gzc.y * azSyz) * dt);
+    st1<true>(Vz + p, ld1<true>(Vz + p) + bz * (gxc.x * dxSxz + gyc.x * dySyz + gze.x * dzSzz
+                                                + gxc.y * axSxz + gyc.y * aySyz + gze.y * azSzz) * dt);
+    st1<true>(A + A_axSxx * n, gxe.z * axSxx + gxe.w * dxSxx * dt);
+    st1<true>(A + A_aySxy * n, gyc.z * aySxy + gyc.w * dySxy * dt);
+    st1<true>(A + A_azSxz * n, gzc.z * azSxz + gzc.w * dzSxz * dt);
+    st1<true>(A + A_axSxy * n, gxc.z * axSxy + gxc.w * dxSxy * dt);
+    st1<true>(A + A_aySyy * n, gye.z * aySyy + gye.w * dySyy * dt);
+    st1<true>(A + A_azSyz * n, gzc.z * azSyz + gzc.w * dzSyz * dt);
+    st1<true>(A + A_axSxz * n, gxc.z * axSxz + gxc.w * dxSxz * dt);
+    st1<true>(A + A_aySyz * n, gyc.z * aySyz + gyc.w * dySyz * dt);
+    st1<true>(A + A_azSzz * n, gze.z * azSzz + gze.w * dzSzz * dt);
+}
+
+// ---------------------------------------------------------------------------
+// Interior columns (m_kernel, and the bottom PML band of the same columns).
+// Tiles of 64 x BY columns over full-width aligned rows; lanes outside
+// [ib,ie] x [jb,je] idle.  Chunks run over k = 1..nz: planes 1..ke take the
+// interior update, planes ke+1..nz (the bottom PML, kbeg_a = nz-na+1) the
+// ADE-CFS update, in two branch-free loops sharing the z queues.
 // ---------------------------------------------------------------------------
 
 // kernel__update_stress, m_kernel.f90:142-348 (normal and shear parts fused:
 // both read V and write disjoint arrays)
 template <int NM, bool NT>
-__global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
+__device__ __forceinline__ void stress_int_body(const StepArgs &a)
 {
     const Grid &g = a.g;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -101,12 +245,13 @@ __global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
     const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
-    const int k0 = g.kb + bz * a.kchunk;
-    const int k1 = min(g.ke, k0 + a.kchunk - 1);
-
-    const long long c2 = g2idx(g, i, j);
-    const int kft = a.m.kfs_top[c2], kfb = a.m.kfs_bot[c2];
-    const int kot = a.m.kob_top[c2], kobb = a.m.kob_bot[c2];
+    const int k0 = 1 + bz * a.kchunk;
+    const int k1 = min(g.nz, k0 + a.kchunk - 1);
+    const int ka1 = (a.mask & MASK_INT) ? min(k1, g.ke) : k0 - 1;      // interior planes k0..ka1
+    const int kb0 = max(k0, g.ke + 1);
+    const int kb1 = (a.has_pml && (a.mask & MASK_PML)) ? k1 : kb0 - 1;  // bottom PML planes kb0..kb1
+    if (ka1 < k0 && kb1 < kb0) return;
+    const int ks = ka1 >= k0 ? k0 : kb0;
 
     const long long P = g.plane;
     const int Y = g.pitch;
@@ -127,113 +272,143 @@ __global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
     const float d2 = a.v.d2;
     const long long rn = g.rn;
 
-    long long p = gidx(g, i, j, k0);
+    long long p = gidx(g, i, j, ks);
     float vx_m1 = Vx[p - P], vx_0 = Vx[p], vx_p1 = Vx[p + P];
     float vy_m1 = Vy[p - P], vy_0 = Vy[p], vy_p1 = Vy[p + P];
     float vz_m2 = Vz[p - 2 * P], vz_m1 = Vz[p - P], vz_0 = Vz[p];
     float mu00 = mu[p], mu10 = mu[p + 1], mu01 = mu[p + Y], mu11 = mu[p + 1 + Y];
-    long long r = ((long long)(k0 - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - 1);
 
-    for (int k = k0; k <= k1; ++k, p += P, r += g.rplane) {
-        const float vx_p2 = Vx[p + 2 * P], vy_p2 = Vy[p + 2 * P], vz_p1 = Vz[p + P];
-        const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
+    if (ka1 >= k0) {
+        const long long c2 = g2idx(g, i, j);
+        const int kft = a.m.kfs_top[c2], kfb = a.m.kfs_bot[c2];
+        const int kot = a.m.kob_top[c2], kobb = a.m.kob_bot[c2];
+        long long r = ((long long)(k0 - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - 1);
+        for (int k = k0; k <= ka1; ++k, p += P, r += g.rplane) {
+            const float vx_p2 = Vx[p + 2 * P], vy_p2 = Vy[p + 2 * P], vz_p1 = Vz[p + P];
+            const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
 
-        const bool in = in_band(k, kft, kfb, kot, kobb);
-        const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
-        const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
-        const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
-        const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
-        const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
-        const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
+            const bool in = in_band(k, kft, kfb, kot, kobb);
+            const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
+            const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
+            const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
+            const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
+            const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
+            const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
 
-        const float vx_im2 = Vx[p - 2], vx_im1 = Vx[p - 1], vx_ip1 = Vx[p + 1];
-        const float vx_jm1 = Vx[p - Y], vx_jp1 = Vx[p + Y], vx_jp2 = Vx[p + 2 * Y];
-        const float vy_jm2 = Vy[p - 2 * Y], vy_jm1 = Vy[p - Y], vy_jp1 = Vy[p + Y];
-        const float vy_im1 = Vy[p - 1], vy_ip1 = Vy[p + 1], vy_ip2 = Vy[p + 2];
-        const float vz_im1 = Vz[p - 1], vz_ip1 = Vz[p + 1], vz_ip2 = Vz[p + 2];
-        const float vz_jm1 = Vz[p - Y], vz_jp1 = Vz[p + Y], vz_jp2 = Vz[p + 2 * Y];
+            const float vx_im2 = Vx[p - 2], vx_im1 = Vx[p - 1], vx_ip1 = Vx[p + 1];
+            const float vx_jm1 = Vx[p - Y], vx_jp1 = Vx[p + Y], vx_jp2 = Vx[p + 2 * Y];
+            const float vy_jm2 = Vy[p - 2 * Y], vy_jm1 = Vy[p - Y], vy_jp1 = Vy[p + Y];
+            const float vy_im1 = Vy[p - 1], vy_ip1 = Vy[p + 1], vy_ip2 = Vy[p + 2];
+            const float vz_im1 = Vz[p - 1], vz_ip1 = Vz[p + 1], vz_ip2 = Vz[p + 2];
+            const float vz_jm1 = Vz[p - Y], vz_jp1 = Vz[p + Y], vz_jp2 = Vz[p + 2 * Y];
 
-        // normal components, m_kernel.f90:198-238
-        const float dxVx = (vx_0 - vx_im1) * re40x - (vx_ip1 - vx_im2) * re41x;
-        const float dyVy = (vy_0 - vy_jm1) * re40y - (vy_jp1 - vy_jm2) * re41y;
-        const float dzVz = (vz_0 - vz_m1) * re40z - (vz_p1 - vz_m2) * re41z;
+            // normal components, m_kernel.f90:198-238
+            const float dxVx = (vx_0 - vx_im1) * re40x - (vx_ip1 - vx_im2) * re41x;
+            const float dyVy = (vy_0 - vy_jm1) * re40y - (vy_jp1 - vy_jm2) * re41y;
+            const float dzVz = (vz_0 - vz_m1) * re40z - (vz_p1 - vz_m2) * re41z;
 
-        const float mu2 = 2.0f * mu00;
-        const float lam2mu = ld1<NT>(lam + p) + mu2;
-        const float taup1 = ld1<NT>(taup + p);
-        const float taus1 = ld1<NT>(taus + p);
+            const float mu2 = 2.0f * mu00;
+            const float lam2mu = ld1<NT>(lam + p) + mu2;
+            const float taup1 = ld1<NT>(taup + p);
+            const float taus1 = ld1<NT>(taus + p);
 
-        const float d3v3 = dxVx + dyVy + dzVz;
-        const float dyVy_dzVz = dyVy + dzVz;
-        const float dxVx_dzVz = dxVx + dzVz;
-        const float dxVx_dyVy = dxVx + dyVy;
+            const float d3v3 = dxVx + dyVy + dzVz;
+            const float dyVy_dzVz = dyVy + dzVz;
+            const float dxVx_dzVz = dxVx + dzVz;
+            const float dxVx_dyVy = dxVx + dyVy;
 
-        float Rxx_n = 0.0f, Ryy_n = 0.0f, Rzz_n = 0.0f;
+            float Rxx_n = 0.0f, Ryy_n = 0.0f, Rzz_n = 0.0f;
 #pragma unroll
-        for (int m = 0; m < NM; ++m) {
-            float *rxx = a.R + (0 * NM + m) * rn + r;
-            float *ryy = a.R + (1 * NM + m) * rn + r;
-            float *rzz = a.R + (2 * NM + m) * rn + r;
-            const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-            const float nxx = c1 * ld1<NT>(rxx) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
-            const float nyy = c1 * ld1<NT>(ryy) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
-            const float nzz = c1 * ld1<NT>(rzz) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
-            st1<NT>(rxx, nxx);
-            st1<NT>(ryy, nyy);
-            st1<NT>(rzz, nzz);
-            Rxx_n = Rxx_n + d1 * nxx;
-            Ryy_n = Ryy_n + d1 * nyy;
-            Rzz_n = Rzz_n + d1 * nzz;
-        }
-        const float taup_plus1 = 1.0f + taup1 * (1.0f + d2);
-        const float taus_plus1 = 1.0f + taus1 * (1.0f + d2);
-        st1<NT>(Sxx + p, ld1<NT>(Sxx + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
-        st1<NT>(Syy + p, ld1<NT>(Syy + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
-        st1<NT>(Szz + p, ld1<NT>(Szz + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
+            for (int m = 0; m < NM; ++m) {
+                float *rxx = a.R + (0 * NM + m) * rn + r;
+                float *ryy = a.R + (1 * NM + m) * rn + r;
+                float *rzz = a.R + (2 * NM + m) * rn + r;
+                const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
+                const float nxx = c1 * ld1<NT>(rxx) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
+                const float nyy = c1 * ld1<NT>(ryy) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
+                const float nzz = c1 * ld1<NT>(rzz) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
+                st1<NT>(rxx, nxx);
+                st1<NT>(ryy, nyy);
+                st1<NT>(rzz, nzz);
+                Rxx_n = Rxx_n + d1 * nxx;
+                Ryy_n = Ryy_n + d1 * nyy;
+                Rzz_n = Rzz_n + d1 * nzz;
+            }
+            const float taup_plus1 = 1.0f + taup1 * (1.0f + d2);
+            const float taus_plus1 = 1.0f + taus1 * (1.0f + d2);
+            st1<NT>(Sxx + p, ld1<NT>(Sxx + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
+            st1<NT>(Syy + p, ld1<NT>(Syy + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
+            st1<NT>(Szz + p, ld1<NT>(Szz + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
 
-        // shear components, m_kernel.f90:285-333
-        const float dxVy_dyVx = (vy_ip1 - vy_0) * re40x - (vy_ip2 - vy_im1) * re41x
-                              + (vx_jp1 - vx_0) * re40y - (vx_jp2 - vx_jm1) * re41y;
-        const float dxVz_dzVx = (vz_ip1 - vz_0) * re40x - (vz_ip2 - vz_im1) * re41x
-                              + (vx_p1 - vx_0) * re40z - (vx_p2 - vx_m1) * re41z;
-        const float dyVz_dzVy = (vz_jp1 - vz_0) * re40y - (vz_jp2 - vz_jm1) * re41y
-                              + (vy_p1 - vy_0) * re40z - (vy_p2 - vy_m1) * re41z;
+            // shear components, m_kernel.f90:285-333
+            const float dxVy_dyVx = (vy_ip1 - vy_0) * re40x - (vy_ip2 - vy_im1) * re41x
+                                  + (vx_jp1 - vx_0) * re40y - (vx_jp2 - vx_jm1) * re41y;
+            const float dxVz_dzVx = (vz_ip1 - vz_0) * re40x - (vz_ip2 - vz_im1) * re41x
+                                  + (vx_p1 - vx_0) * re40z - (vx_p2 - vx_m1) * re41z;
+            const float dyVz_dzVy = (vz_jp1 - vz_0) * re40y - (vz_jp2 - vz_jm1) * re41y
+                                  + (vy_p1 - vy_0) * re40z - (vy_p2 - vy_m1) * re41z;
 
-        const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
-        const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
-        const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
+            const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
+            const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
+            const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
 
-        float Ryz_n = 0.0f, Rxz_n = 0.0f, Rxy_n = 0.0f;
+            float Ryz_n = 0.0f, Rxz_n = 0.0f, Rxy_n = 0.0f;
 #pragma unroll
-        for (int m = 0; m < NM; ++m) {
-            float *ryz = a.R + (3 * NM + m) * rn + r;
-            float *rxz = a.R + (4 * NM + m) * rn + r;
-            float *rxy = a.R + (5 * NM + m) * rn + r;
-            const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-            const float nyz = c1 * ld1<NT>(ryz) - c2 * muyz * taus1 * dyVz_dzVy * dt;
-            const float nxz = c1 * ld1<NT>(rxz) - c2 * muxz * taus1 * dxVz_dzVx * dt;
-            const float nxy = c1 * ld1<NT>(rxy) - c2 * muxy * taus1 * dxVy_dyVx * dt;
-            st1<NT>(ryz, nyz);
-            st1<NT>(rxz, nxz);
-            st1<NT>(rxy, nxy);
-            Ryz_n = Ryz_n + d1 * nyz;
-            Rxz_n = Rxz_n + d1 * nxz;
-            Rxy_n = Rxy_n + d1 * nxy;
-        }
-        st1<NT>(Syz + p, ld1<NT>(Syz + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
-        st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
-        st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
+            for (int m = 0; m < NM; ++m) {
+                float *ryz = a.R + (3 * NM + m) * rn + r;
+                float *rxz = a.R + (4 * NM + m) * rn + r;
+                float *rxy = a.R + (5 * NM + m) * rn + r;
+                const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
+                const float nyz = c1 * ld1<NT>(ryz) - c2 * muyz * taus1 * dyVz_dzVy * dt;
+                const float nxz = c1 * ld1<NT>(rxz) - c2 * muxz * taus1 * dxVz_dzVx * dt;
+                const float nxy = c1 * ld1<NT>(rxy) - c2 * muxy * taus1 * dxVy_dyVx * dt;
+                st1<NT>(ryz, nyz);
+                st1<NT>(rxz, nxz);
+                st1<NT>(rxy, nxy);
+                Ryz_n = Ryz_n + d1 * nyz;
+                Rxz_n = Rxz_n + d1 * nxz;
+                Rxy_n = Rxy_n + d1 * nxy;
+            }
+            st1<NT>(Syz + p, ld1<NT>(Syz + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
+            st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
+            st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
 
-        // rotate the z queues
-        vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
-        vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = vy_p2;
-        vz_m2 = vz_m1; vz_m1 = vz_0; vz_0 = vz_p1;
-        mu00 = mu00n; mu10 = mu10n; mu01 = mu01n;
-        mu11 = mu[p + P + 1 + Y];
+            // rotate the z queues
+            vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
+            vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = vy_p2;
+            vz_m2 = vz_m1; vz_m1 = vz_0; vz_0 = vz_p1;
+            mu00 = mu00n; mu10 = mu10n; mu01 = mu01n;
+            mu11 = mu[p + P + 1 + Y];
+        }
+    }
+    if (kb1 >= kb0) {
+        // bottom PML band of the interior columns (box 4)
+        const PmlBox b = a.box[4];
+        const long long astride = (long long)b.nxb * b.nyb;
+        long long q = b.off + (long long)(j - b.j0) * b.nxb + (i - b.i0) + (long long)(kb0 - b.k0) * astride;
+        const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
+        const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
+        for (int k = kb0; k <= kb1; ++k, p += P, q += astride) {
+            const float vx_p2 = Vx[p + 2 * P], vy_p2 = Vy[p + 2 * P], vz_p1 = Vz[p + P];
+            const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
+            pml_stress_cell<NT>(a, p, q, k, vx_0, vx_p1, vy_0, vy_p1, vz_m1, vz_0, mu00, mu00n, mu10, mu10n, mu01,
+                                mu01n, mu11, gxc, gxe, gyc, gye);
+            vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
+            vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = vy_p2;
+            vz_m2 = vz_m1; vz_m1 = vz_0; vz_0 = vz_p1;
+            mu00 = mu00n; mu10 = mu10n; mu01 = mu01n;
+            mu11 = mu[p + P + 1 + Y];
+        }
     }
 }
 
-// kernel__update_vel, m_kernel.f90:75-140
+template <int NM, bool NT>
+__global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
+{
+    stress_int_body<NM, NT>(a);
+}
+
+// kernel__update_vel, m_kernel.f90:75-140, then the bottom PML band
 __global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
 {
     const Grid &g = a.g;
@@ -245,12 +420,13 @@ __global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
     const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
-    const int k0 = g.kb + bz * a.kchunk;
-    const int k1 = min(g.ke, k0 + a.kchunk - 1);
-
-    const long long c2 = g2idx(g, i, j);
-    const int kft = a.m.kfs_top[c2], kfb = a.m.kfs_bot[c2];
-    const int kot = a.m.kob_top[c2], kobb = a.m.kob_bot[c2];
+    const int k0 = 1 + bz * a.kchunk;
+    const int k1 = min(g.nz, k0 + a.kchunk - 1);
+    const int ka1 = (a.mask & MASK_INT) ? min(k1, g.ke) : k0 - 1;
+    const int kb0 = max(k0, g.ke + 1);
+    const int kb1 = (a.has_pml && (a.mask & MASK_PML)) ? k1 : kb0 - 1;
+    if (ka1 < k0 && kb1 < kb0) return;
+    const int ks = ka1 >= k0 ? k0 : kb0;
 
     const long long P = g.plane;
     const int Y = g.pitch;
@@ -266,60 +442,79 @@ __global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
     float *__restrict__ Vz = a.f.Vz;
     const float dt = a.dt;
 
-    long long p = gidx(g, i, j, k0);
+    long long p = gidx(g, i, j, ks);
     float sxz_m2 = Sxz[p - 2 * P], sxz_m1 = Sxz[p - P], sxz_0 = Sxz[p];
     float syz_m2 = Syz[p - 2 * P], syz_m1 = Syz[p - P], syz_0 = Syz[p];
     float szz_m1 = Szz[p - P], szz_0 = Szz[p], szz_p1 = Szz[p + P];
     float rho0 = rho[p];
 
-    for (int k = k0; k <= k1; ++k, p += P) {
-        const float sxz_p1 = Sxz[p + P], syz_p1 = Syz[p + P], szz_p2 = Szz[p + 2 * P];
-        const float rho_kp = rho[p + P];
+    if (ka1 >= k0) {
+        const long long c2 = g2idx(g, i, j);
+        const int kft = a.m.kfs_top[c2], kfb = a.m.kfs_bot[c2];
+        const int kot = a.m.kob_top[c2], kobb = a.m.kob_bot[c2];
+        for (int k = k0; k <= ka1; ++k, p += P) {
+            const float sxz_p1 = Sxz[p + P], syz_p1 = Syz[p + P], szz_p2 = Szz[p + 2 * P];
+            const float rho_kp = rho[p + P];
 
-        const bool in = in_band(k, kft, kfb, kot, kobb);
-        const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
-        const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
-        const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
-        const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
-        const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
-        const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
+            const bool in = in_band(k, kft, kfb, kot, kobb);
+            const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
+            const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
+            const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
+            const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
+            const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
+            const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
 
-        const float sxx_im1 = Sxx[p - 1], sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1], sxx_ip2 = Sxx[p + 2];
-        const float sxy_0 = Sxy[p];
-        const float sxy_jm2 = Sxy[p - 2 * Y], sxy_jm1 = Sxy[p - Y], sxy_jp1 = Sxy[p + Y];
-        const float sxy_im2 = Sxy[p - 2], sxy_im1 = Sxy[p - 1], sxy_ip1 = Sxy[p + 1];
-        const float syy_jm1 = Syy[p - Y], syy_0 = Syy[p], syy_jp1 = Syy[p + Y], syy_jp2 = Syy[p + 2 * Y];
-        const float syz_jm2 = Syz[p - 2 * Y], syz_jm1 = Syz[p - Y], syz_jp1 = Syz[p + Y];
-        const float sxz_im2 = Sxz[p - 2], sxz_im1 = Sxz[p - 1], sxz_ip1 = Sxz[p + 1];
+            const float sxx_im1 = Sxx[p - 1], sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1], sxx_ip2 = Sxx[p + 2];
+            const float sxy_0 = Sxy[p];
+            const float sxy_jm2 = Sxy[p - 2 * Y], sxy_jm1 = Sxy[p - Y], sxy_jp1 = Sxy[p + Y];
+            const float sxy_im2 = Sxy[p - 2], sxy_im1 = Sxy[p - 1], sxy_ip1 = Sxy[p + 1];
+            const float syy_jm1 = Syy[p - Y], syy_0 = Syy[p], syy_jp1 = Syy[p + Y], syy_jp2 = Syy[p + 2 * Y];
+            const float syz_jm2 = Syz[p - 2 * Y], syz_jm1 = Syz[p - Y], syz_jp1 = Syz[p + Y];
+            const float sxz_im2 = Sxz[p - 2], sxz_im1 = Sxz[p - 1], sxz_ip1 = Sxz[p + 1];
 
-        const float d3Sx3 = (sxx_ip1 - sxx_0) * re40x - (sxx_ip2 - sxx_im1) * re41x
-                          + (sxy_0 - sxy_jm1) * re40y - (sxy_jp1 - sxy_jm2) * re41y
-                          + (sxz_0 - sxz_m1) * re40z - (sxz_p1 - sxz_m2) * re41z;
-        const float d3Sy3 = (sxy_0 - sxy_im1) * re40x - (sxy_ip1 - sxy_im2) * re41x
-                          + (syy_jp1 - syy_0) * re40y - (syy_jp2 - syy_jm1) * re41y
-                          + (syz_0 - syz_m1) * re40z - (syz_p1 - syz_m2) * re41z;
-        const float d3Sz3 = (sxz_0 - sxz_im1) * re40x - (sxz_ip1 - sxz_im2) * re41x
-                          + (syz_0 - syz_jm1) * re40y - (syz_jp1 - syz_jm2) * re41y
-                          + (szz_p1 - szz_0) * re40z - (szz_p2 - szz_m1) * re41z;
+            const float d3Sx3 = (sxx_ip1 - sxx_0) * re40x - (sxx_ip2 - sxx_im1) * re41x
+                              + (sxy_0 - sxy_jm1) * re40y - (sxy_jp1 - sxy_jm2) * re41y
+                              + (sxz_0 - sxz_m1) * re40z - (sxz_p1 - sxz_m2) * re41z;
+            const float d3Sy3 = (sxy_0 - sxy_im1) * re40x - (sxy_ip1 - sxy_im2) * re41x
+                              + (syy_jp1 - syy_0) * re40y - (syy_jp2 - syy_jm1) * re41y
+                              + (syz_0 - syz_m1) * re40z - (syz_p1 - syz_m2) * re41z;
+            const float d3Sz3 = (sxz_0 - sxz_im1) * re40x - (sxz_ip1 - sxz_im2) * re41x
+                              + (syz_0 - syz_jm1) * re40y - (syz_jp1 - syz_jm2) * re41y
+                              + (szz_p1 - szz_0) * re40z - (szz_p2 - szz_m1) * re41z;
 
-        Vx[p] = Vx[p] + 2.0f / (rho0 + rho[p + 1]) * d3Sx3 * dt;
-        Vy[p] = Vy[p] + 2.0f / (rho0 + rho[p + Y]) * d3Sy3 * dt;
-        Vz[p] = Vz[p] + 2.0f / (rho0 + rho_kp) * d3Sz3 * dt;
+            st1<true>(Vx + p, ld1<true>(Vx + p) + 2.0f / (rho0 + rho[p + 1]) * d3Sx3 * dt);
+            st1<true>(Vy + p, ld1<true>(Vy + p) + 2.0f / (rho0 + rho[p + Y]) * d3Sy3 * dt);
+            st1<true>(Vz + p, ld1<true>(Vz + p) + 2.0f / (rho0 + rho_kp) * d3Sz3 * dt);
 
-        sxz_m2 = sxz_m1; sxz_m1 = sxz_0; sxz_0 = sxz_p1;
-        syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1;
-        szz_m1 = szz_0; szz_0 = szz_p1; szz_p1 = szz_p2;
-        rho0 = rho_kp;
+            sxz_m2 = sxz_m1; sxz_m1 = sxz_0; sxz_0 = sxz_p1;
+            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1;
+            szz_m1 = szz_0; szz_0 = szz_p1; szz_p1 = szz_p2;
+            rho0 = rho_kp;
+        }
+    }
+    if (kb1 >= kb0) {
+        const PmlBox b = a.box[4];
+        const long long astride = (long long)b.nxb * b.nyb;
+        long long q = b.off + (long long)(j - b.j0) * b.nxb + (i - b.i0) + (long long)(kb0 - b.k0) * astride;
+        const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
+        const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
+        for (int k = kb0; k <= kb1; ++k, p += P, q += astride) {
+            const float sxz_p1 = Sxz[p + P], syz_p1 = Syz[p + P], szz_p2 = Szz[p + 2 * P];
+            const float rho_kp = rho[p + P];
+            pml_vel_cell(a, p, q, k, szz_0, szz_p1, syz_m1, syz_0, sxz_m1, sxz_0, rho0, rho_kp, gxc, gxe, gyc, gye);
+            sxz_m2 = sxz_m1; sxz_m1 = sxz_0; sxz_0 = sxz_p1;
+            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1;
+            szz_m1 = szz_0; szz_0 = szz_p1; szz_p1 = szz_p2;
+            rho0 = rho_kp;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// ADE-CFS PML (m_absorb_p).  Tiles of 64 x BY columns from a host list cover
-// the PML cells: side columns (kbeg_a = 1) over their full depth, interior
-// columns over the bottom na planes (m_global.f90:334-343).  Each lane marches
-// the part of its column that is PML and lies in the tile's k chunk.  The 18
-// auxiliaries are numbered compactly box by box; the auxiliary of plane k in
-// a column is abase + k*astride.
+// Side columns (kbeg_a = 1, m_global.f90:334-343): y slabs as full-row tiles,
+// x slabs of interior rows as tiles na columns wide with several rows per
+// wave (tiles from a host list).  Each lane marches its column through the
+// tile's k chunk.
 // ---------------------------------------------------------------------------
 struct PmlLane {
     int i, j, kb, ke;
@@ -330,9 +525,8 @@ __device__ static inline bool pml_lane(const StepArgs &a, PmlLane &L)
 {
     const Grid &g = a.g;
     const int4 tl = a.tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    // tile: x = first column | (width << 20), y = first row, z..w = k chunk.
-    // width 64: full rows; width na: one x slab, several rows per wave.
-    const int w = tl.x >> 20;
+    const int w = (tl.x >> 20) & 0xFF;
+    const bool xslab = (tl.x >> 28) & 1;
     const int lane = threadIdx.y * blockDim.x + threadIdx.x;
     const int rows = (blockDim.x * blockDim.y) / w;
     if (lane >= rows * w) return false;
@@ -345,12 +539,11 @@ __device__ static inline bool pml_lane(const StepArgs &a, PmlLane &L)
     else if (L.j > g.ny - na) q = 1;
     else if (L.i <= na) q = 2;
     else if (L.i > g.nx - na) q = 3;
-    else q = 4;
-    // each PML cell belongs to exactly one tile: x-slab cells of interior rows
-    // to the slab tiles, every other PML cell to the full-row tiles
-    if ((w == 64) == (q == 2 || q == 3)) return false;
-    const PmlBox b = q == 0 ? a.box[0] : q == 1 ? a.box[1] : q == 2 ? a.box[2] : q == 3 ? a.box[3] : a.box[4];
-    L.kb = max(tl.z, b.k0);
+    else return false;  // interior column: its bottom band belongs to the interior kernels
+    // x-slab cells of interior rows belong to the slab tiles, y-slab cells to full-row tiles
+    if (xslab != (q >= 2)) return false;
+    const PmlBox b = pick_box(a, q);
+    L.kb = tl.z;
     L.ke = min(tl.w, g.nz);
     L.astride = (long long)b.nxb * b.nyb;
     L.abase = b.off + (long long)(L.j - b.j0) * b.nxb + (L.i - b.i0) - (long long)b.k0 * L.astride;
@@ -370,10 +563,6 @@ __global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
     const float *__restrict__ Vy = a.f.Vy;
     const float *__restrict__ Vz = a.f.Vz;
     const float *__restrict__ mu = a.m.mu;
-    const float *__restrict__ lam = a.m.lam;
-    float *__restrict__ ade = a.ade;
-    const float dt = a.dt;
-    const long long n = a.npml;
     const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
     const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
 
@@ -381,57 +570,11 @@ __global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
     long long q = L.abase + (long long)L.kb * L.astride;
     float vx_0 = Vx[p], vy_0 = Vy[p], vz_m1 = Vz[p - P];
     float mu00 = mu[p], mu10 = mu[p + 1], mu01 = mu[p + Y], mu11 = mu[p + 1 + Y];
-
     for (int k = L.kb; k <= L.ke; ++k, p += P, q += L.astride) {
         const float vx_p1 = Vx[p + P], vy_p1 = Vy[p + P], vz_0 = Vz[p];
         const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
-        const float vx_im1 = Vx[p - 1], vy_jm1 = Vy[p - Y];
-        const float vy_ip1 = Vy[p + 1], vz_ip1 = Vz[p + 1];
-        const float vx_jp1 = Vx[p + Y], vz_jp1 = Vz[p + Y];
-        const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
-        const float lam0 = lam[p];
-        float *__restrict__ A = ade + q;
-        const float axVx = A[A_axVx * n], ayVy = A[A_ayVy * n], azVz = A[A_azVz * n];
-        const float ayVx = A[A_ayVx * n], azVx = A[A_azVx * n];
-        const float axVy = A[A_axVy * n], azVy = A[A_azVy * n];
-        const float axVz = A[A_axVz * n], ayVz = A[A_ayVz * n];
-
-        // normal components
-        const float dxVx = (vx_0 - vx_im1) * a.r20x;
-        const float dyVy = (vy_0 - vy_jm1) * a.r20y;
-        const float dzVz = (vz_0 - vz_m1) * a.r20z;
-        const float lam2mu_R = (lam0 + 2.0f * mu00);
-        const float lam_R = lam2mu_R - 2.0f * mu00;
-        const float dxVx_ade = gxc.x * dxVx + gxc.y * axVx;
-        const float dyVy_ade = gyc.x * dyVy + gyc.y * ayVy;
-        const float dzVz_ade = gzc.x * dzVz + gzc.y * azVz;
-        a.f.Sxx[p] = a.f.Sxx[p] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
-        a.f.Syy[p] = a.f.Syy[p] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
-        a.f.Szz[p] = a.f.Szz[p] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
-        A[A_axVx * n] = gxc.z * axVx + gxc.w * dxVx * dt;
-        A[A_ayVy * n] = gyc.z * ayVy + gyc.w * dyVy * dt;
-        A[A_azVz * n] = gzc.z * azVz + gzc.w * dzVz * dt;
-
-        // shear components
-        const float dxVy = (vy_ip1 - vy_0) * a.r20x;
-        const float dxVz = (vz_ip1 - vz_0) * a.r20x;
-        const float dyVx = (vx_jp1 - vx_0) * a.r20y;
-        const float dyVz = (vz_jp1 - vz_0) * a.r20y;
-        const float dzVx = (vx_p1 - vx_0) * a.r20z;
-        const float dzVy = (vy_p1 - vy_0) * a.r20z;
-        const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
-        const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
-        const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
-        a.f.Syz[p] = a.f.Syz[p] + muyz * (gye.x * dyVz + gze.x * dzVy + gye.y * ayVz + gze.y * azVy) * dt;
-        a.f.Sxz[p] = a.f.Sxz[p] + muxz * (gxe.x * dxVz + gze.x * dzVx + gxe.y * axVz + gze.y * azVx) * dt;
-        a.f.Sxy[p] = a.f.Sxy[p] + muxy * (gxe.x * dxVy + gye.x * dyVx + gxe.y * axVy + gye.y * ayVx) * dt;
-        A[A_ayVx * n] = gye.z * ayVx + gye.w * dyVx * dt;
-        A[A_azVx * n] = gze.z * azVx + gze.w * dzVx * dt;
-        A[A_axVy * n] = gxe.z * axVy + gxe.w * dxVy * dt;
-        A[A_azVy * n] = gze.z * azVy + gze.w * dzVy * dt;
-        A[A_axVz * n] = gxe.z * axVz + gxe.w * dxVz * dt;
-        A[A_ayVz * n] = gye.z * ayVz + gye.w * dyVz * dt;
-
+        pml_stress_cell<true>(a, p, q, k, vx_0, vx_p1, vy_0, vy_p1, vz_m1, vz_0, mu00, mu00n, mu10, mu10n, mu01,
+                              mu01n, mu11, gxc, gxe, gyc, gye);
         vx_0 = vx_p1; vy_0 = vy_p1; vz_m1 = vz_0;
         mu00 = mu00n; mu10 = mu10n; mu01 = mu01n;
         mu11 = mu[p + P + 1 + Y];
@@ -446,71 +589,24 @@ __global__ void __launch_bounds__(256) k_vel_pml(StepArgs a)
     const Grid &g = a.g;
     const int i = L.i, j = L.j;
     const long long P = g.plane;
-    const int Y = g.pitch;
-    const float *__restrict__ Sxx = a.f.Sxx;
-    const float *__restrict__ Syy = a.f.Syy;
     const float *__restrict__ Szz = a.f.Szz;
     const float *__restrict__ Syz = a.f.Syz;
     const float *__restrict__ Sxz = a.f.Sxz;
-    const float *__restrict__ Sxy = a.f.Sxy;
     const float *__restrict__ rho = a.m.rho;
-    float *__restrict__ ade = a.ade;
-    const float dt = a.dt;
-    const long long n = a.npml;
     const float4 gxc = a.gxc[i - 1], gxe = a.gxe[i - 1];
     const float4 gyc = a.gyc[j - 1], gye = a.gye[j - 1];
 
     long long p = gidx(g, i, j, L.kb);
     long long q = L.abase + (long long)L.kb * L.astride;
     float szz_0 = Szz[p], syz_m1 = Syz[p - P], sxz_m1 = Sxz[p - P], rho0 = rho[p];
-
     for (int k = L.kb; k <= L.ke; ++k, p += P, q += L.astride) {
         const float szz_p1 = Szz[p + P], syz_0 = Syz[p], sxz_0 = Sxz[p], rho_kp = rho[p + P];
-        const float sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1];
-        const float syy_0 = Syy[p], syy_jp1 = Syy[p + Y];
-        const float syz_jm1 = Syz[p - Y], sxz_im1 = Sxz[p - 1];
-        const float sxy_0 = Sxy[p], sxy_im1 = Sxy[p - 1], sxy_jm1 = Sxy[p - Y];
-        const float rho_ip = rho[p + 1], rho_jp = rho[p + Y];
-        const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
-        float *__restrict__ A = ade + q;
-        const float axSxx = A[A_axSxx * n], aySxy = A[A_aySxy * n], azSxz = A[A_azSxz * n];
-        const float axSxy = A[A_axSxy * n], aySyy = A[A_aySyy * n], azSyz = A[A_azSyz * n];
-        const float axSxz = A[A_axSxz * n], aySyz = A[A_aySyz * n], azSzz = A[A_azSzz * n];
-
-        const float dxSxx = (sxx_ip1 - sxx_0) * a.r20x;
-        const float dySyy = (syy_jp1 - syy_0) * a.r20y;
-        const float dzSzz = (szz_p1 - szz_0) * a.r20z;
-        const float dySyz = (syz_0 - syz_jm1) * a.r20y;
-        const float dzSyz = (syz_0 - syz_m1) * a.r20z;
-        const float dxSxz = (sxz_0 - sxz_im1) * a.r20x;
-        const float dzSxz = (sxz_0 - sxz_m1) * a.r20z;
-        const float dxSxy = (sxy_0 - sxy_im1) * a.r20x;
-        const float dySxy = (sxy_0 - sxy_jm1) * a.r20y;
-        const float bx = 2.0f / (rho0 + rho_ip);
-        const float by = 2.0f / (rho0 + rho_jp);
-        const float bz = 2.0f / (rho0 + rho_kp);
-
-        a.f.Vx[p] = a.f.Vx[p] + bx * (gxe.x * dxSxx + gyc.x * dySxy + gzc.x * dzSxz
-                                      + gxe.y * axSxx + gyc.y * aySxy + gzc.y * azSxz) * dt;
-        a.f.Vy[p] = a.f.Vy[p] + by * (gxc.x * dxSxy + gye.x * dySyy + gzc.x * dzSyz
-                                      + gxc.y * axSxy + gye.y * aySyy + gzc.y * azSyz) * dt;
-        a.f.Vz[p] = a.f.Vz[p] + bz * (gxc.x * dxSxz + gyc.x * dySyz + gze.x * dzSzz
-                                      + gxc.y * axSxz + gyc.y * aySyz + gze.y * azSzz) * dt;
-        A[A_axSxx * n] = gxe.z * axSxx + gxe.w * dxSxx * dt;
-        A[A_aySxy * n] = gyc.z * aySxy + gyc.w * dySxy * dt;
-        A[A_azSxz * n] = gzc.z * azSxz + gzc.w * dzSxz * dt;
-        A[A_axSxy * n] = gxc.z * axSxy + gxc.w * dxSxy * dt;
-        A[A_aySyy * n] = gye.z * aySyy + gye.w * dySyy * dt;
-        A[A_azSyz * n] = gzc.z * azSyz + gzc.w * dzSyz * dt;
-        A[A_axSxz * n] = gxc.z * axSxz + gxc.w * dxSxz * dt;
-        A[A_aySyz * n] = gyc.z * aySyz + gyc.w * dySyz * dt;
-        A[A_azSzz * n] = gze.z * azSzz + gze.w * dzSzz * dt;
-
+        pml_vel_cell(a, p, q, k, szz_0, szz_p1, syz_m1, syz_0, sxz_m1, sxz_0, rho0, rho_kp, gxc, gxe, gyc, gye);
         szz_0 = szz_p1; syz_m1 = syz_0; sxz_m1 = sxz_0; rho0 = rho_kp;
     }
 }
 
-static void step_args(swpc_ctx *c, StepArgs &a)
+static void step_args(swpc_ctx *c, StepArgs &a, int mask)
 {
     a.g = c->g;
     a.f.Vx = c->fields[SWPC_VX]; a.f.Vy = c->fields[SWPC_VY]; a.f.Vz = c->fields[SWPC_VZ];
@@ -532,27 +628,34 @@ static void step_args(swpc_ctx *c, StepArgs &a)
     a.r20z = 1.0f / (float)c->desc.dz;
     a.tiles = (const int4 *)c->d_pml_tiles;
     a.ntiles = c->npml_tiles;
+    a.has_pml = (c->desc.abc == SWPC_ABC_PML && c->npml > 0) ? 1 : 0;
+    a.mask = mask;
     a.by = c->by;
     a.kchunk = c->kchunk;
     const Grid &g = c->g;
     a.bx0 = (g.ib - 1) / BX;
     a.nbx = (g.ie - 1) / BX - a.bx0 + 1;
     a.nby = (g.nyk + a.by - 1) / a.by;
-    a.nbz = (g.nzk + a.kchunk - 1) / a.kchunk;
+    a.nbz = (g.nz + a.kchunk - 1) / a.kchunk;
 }
 
-hipError_t launch_stress(swpc_ctx *c, int mask)
+hipError_t launch_stress(swpc_ctx *c, int mask, hipStream_t pml_stream)
 {
     StepArgs a;
-    step_args(c, a);
+    step_args(c, a, mask);
     dim3 block(BX, a.by);
-    if ((mask & MASK_INT) && c->g.nxk > 0 && c->g.nyk > 0 && c->g.nzk > 0) {
+    if ((mask & MASK_PML) && a.has_pml && c->npml_tiles > 0) {
+        hipLaunchKernelGGL(k_stress_pml, dim3(c->npml_tiles), block, 0, pml_stream ? pml_stream : c->stream, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (c->g.nxk > 0 && c->g.nyk > 0 && ((mask & MASK_INT) || a.has_pml)) {
         dim3 grid(a.nbx * a.nby * a.nbz);
         const bool nt = (c->variant & 1) != 0;
-#define SWPC_STRESS_CASE(NMV)                                                                         \
-    case NMV:                                                                                         \
-        if (nt) hipLaunchKernelGGL((k_stress_int<NMV, true>), grid, block, 0, c->stream, a);        \
-        else hipLaunchKernelGGL((k_stress_int<NMV, false>), grid, block, 0, c->stream, a);          \
+#define SWPC_STRESS_CASE(NMV)                                                                       \
+    case NMV:                                                                                       \
+        if (nt) hipLaunchKernelGGL((k_stress_int<NMV, true>), grid, block, 0, c->stream, a);      \
+        else hipLaunchKernelGGL((k_stress_int<NMV, false>), grid, block, 0, c->stream, a);        \
         break;
         switch (c->g.nm) {
             SWPC_STRESS_CASE(0) SWPC_STRESS_CASE(1) SWPC_STRESS_CASE(2) SWPC_STRESS_CASE(3) SWPC_STRESS_CASE(4)
@@ -560,37 +663,32 @@ hipError_t launch_stress(swpc_ctx *c, int mask)
         default: return hipErrorInvalidValue;
         }
 #undef SWPC_STRESS_CASE
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if ((mask & MASK_PML) && c->npml > 0 && c->npml_tiles > 0) {
-        hipLaunchKernelGGL(k_stress_pml, dim3(c->npml_tiles), block, 0, c->stream, a);
         return hipGetLastError();
     }
     return hipSuccess;
 }
 
-hipError_t launch_vel(swpc_ctx *c, int mask)
+hipError_t launch_vel(swpc_ctx *c, int mask, hipStream_t pml_stream)
 {
     StepArgs a;
-    step_args(c, a);
+    step_args(c, a, mask);
     dim3 block(BX, a.by);
-    if ((mask & MASK_INT) && c->g.nxk > 0 && c->g.nyk > 0 && c->g.nzk > 0) {
-        hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), block, 0, c->stream, a);
+    if ((mask & MASK_PML) && a.has_pml && c->npml_tiles > 0) {
+        hipLaunchKernelGGL(k_vel_pml, dim3(c->npml_tiles), block, 0, pml_stream ? pml_stream : c->stream, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if ((mask & MASK_PML) && c->npml > 0 && c->npml_tiles > 0) {
-        hipLaunchKernelGGL(k_vel_pml, dim3(c->npml_tiles), block, 0, c->stream, a);
+    if (c->g.nxk > 0 && c->g.nyk > 0 && ((mask & MASK_INT) || a.has_pml)) {
+        hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), block, 0, c->stream, a);
         return hipGetLastError();
     }
     return hipSuccess;
 }
 
-hipError_t launch_kernel_stress(swpc_ctx *c) { return launch_stress(c, MASK_INT); }
-hipError_t launch_pml_stress(swpc_ctx *c) { return launch_stress(c, MASK_PML); }
-hipError_t launch_kernel_vel(swpc_ctx *c) { return launch_vel(c, MASK_INT); }
-hipError_t launch_pml_vel(swpc_ctx *c) { return launch_vel(c, MASK_PML); }
+hipError_t launch_kernel_stress(swpc_ctx *c) { return launch_stress(c, MASK_INT, nullptr); }
+hipError_t launch_pml_stress(swpc_ctx *c) { return launch_stress(c, MASK_PML, nullptr); }
+hipError_t launch_kernel_vel(swpc_ctx *c) { return launch_vel(c, MASK_INT, nullptr); }
+hipError_t launch_pml_vel(swpc_ctx *c) { return launch_vel(c, MASK_PML, nullptr); }
 
 // ---------------------------------------------------------------------------
 // Cerjan sponge (m_absorb_c.f90:113-198).  Profiles carry the 3-cell halo:

@@ -91,6 +91,8 @@ struct swpc_ctx {
     Grid g;
     int device;
     hipStream_t stream;
+    hipStream_t stream2;            // PML kernels alongside the interior ones
+    hipEvent_t ev_fork, ev_join;
     char err[512];
 
     float *fields[SWPC_NFIELD];
@@ -105,7 +107,7 @@ struct swpc_ctx {
     long long npml;
     float *ade;
     float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
-    int *d_pml_tiles;  // int4 (i0, j0, k0, k1) per PML tile
+    int *d_pml_tiles;  // int4 (i0 | width << 20 | xslab << 28, j0, k0, k1) per side-column tile
     int npml_tiles;
     // Cerjan
     float *cgx_c, *cgx_b, *cgy_c, *cgy_b, *cgz_c, *cgz_b;
@@ -140,13 +142,16 @@ struct swpc_ctx {
     float *d_vmax;
 
     int kchunk, by;
-    int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel
+    int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel,
+                       // bit 1: side-column PML kernels on a second stream
     long long bytes;
 };
 
 // ---- launchers (stencil.hip / aux.hip) ----
-hipError_t launch_stress(swpc_ctx *c, int mask);  // mask: 1 interior, 2 PML, 3 both
-hipError_t launch_vel(swpc_ctx *c, int mask);
+// mask: 1 interior, 2 PML, 3 both; the PML kernel goes to pml_stream when
+// given (it updates cells disjoint from the interior kernel's)
+hipError_t launch_stress(swpc_ctx *c, int mask, hipStream_t pml_stream);
+hipError_t launch_vel(swpc_ctx *c, int mask, hipStream_t pml_stream);
 hipError_t launch_kernel_stress(swpc_ctx *c);
 hipError_t launch_kernel_vel(swpc_ctx *c);
 hipError_t launch_pml_stress(swpc_ctx *c);

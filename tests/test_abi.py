@@ -1,0 +1,107 @@
+"""The C-ABI boundary without a GPU: the shared libraries load, export every
+entry point their interface declares, the Python binding types each of them,
+and argument validation happens on the host before any device call.
+
+No kernel runs here; the parity tests in test_gpu_parity.py call the same
+entry points on a GPU.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from openswpc_amd import gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "swpc_gpu.h")
+HOST_SRC = os.path.join(ROOT, "openswpc_amd", "host", "swpc_host.f90")
+HOST_LIB = os.path.join(ROOT, "openswpc_amd", "lib", "libswpc_host.so")
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(swpc_[a-z0-9_]+)\s*\(", text)))
+
+
+def exported(path):
+    lib = C.CDLL(path)
+    return lambda name: hasattr(lib, name)
+
+
+def test_header_declares_the_time_loop():
+    names = declared_functions(HEADER)
+    for want in ("swpc_create", "swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_source_stressglut",
+                 "swpc_kernel_update_vel", "swpc_absorb_update_vel", "swpc_source_bodyforce", "swpc_wav_store",
+                 "swpc_step", "swpc_kernel_vmax", "swpc_get_seismograms"):
+        assert want in names
+
+
+def test_gpu_library_exports_every_declared_symbol():
+    has = exported(gpu.LIB_PATH)
+    missing = [n for n in declared_functions(HEADER) if not has(n)]
+    assert not missing, f"declared in swpc_gpu.h but not exported: {missing}"
+
+
+def test_binding_types_every_declared_symbol():
+    L = gpu.load_library()
+    untyped = [n for n in declared_functions(HEADER) if getattr(L, n).argtypes is None]
+    assert not untyped, f"ctypes binding leaves these untyped: {untyped}"
+
+
+def test_host_library_exports_every_bind_c_routine():
+    names = re.findall(r"bind\(C,\s*name='([a-z0-9_]+)'\)", open(HOST_SRC).read())
+    assert len(names) >= 13
+    has = exported(HOST_LIB)
+    assert [n for n in names if not has(n)] == []
+
+
+def _desc(**kw):
+    d = dict(nx=40, ny=36, nz=32, na=8, nm=3, abc=0, dx=0.5, dy=0.5, dz=0.5, dt=0.01, device=0)
+    d.update(kw)
+    return gpu._Desc(**d)
+
+
+@pytest.mark.parametrize("bad", [dict(nx=3), dict(nz=2), dict(na=-1), dict(nm=9), dict(nm=-1), dict(dt=0.0),
+                                 dict(dx=-0.5), dict(abc=7), dict(na=20), dict(nz=8, na=8)])
+def test_create_rejects_bad_descriptors_on_host(bad):
+    """swpc_create validates before touching the device (SWPC_EINVAL = -1)."""
+    L = gpu.load_library()
+    h = C.c_void_p()
+    assert L.swpc_create(C.byref(_desc(**bad)), C.byref(h)) == -1
+    assert h.value is None
+
+
+def test_null_context_is_rejected():
+    L = gpu.load_library()
+    for name in ("swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_kernel_update_vel",
+                 "swpc_absorb_update_vel", "swpc_update_stress", "swpc_update_vel", "swpc_synchronize"):
+        assert getattr(L, name)(None) == -1, name
+    for name in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store"):
+        assert getattr(L, name)(None, 1) == -1, name
+    assert L.swpc_step(None, 1, 1, 0, 1) == -1
+    assert L.swpc_kernel_vmax(None, np.zeros(3, np.float32)) == -1
+    assert L.swpc_destroy(None) == 0
+    assert L.swpc_device_bytes(None) == 0
+    assert L.swpc_stream(None) is None
+
+
+def test_no_cpu_fallback_without_gpu():
+    """Without a device the product raises instead of computing elsewhere."""
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(gpu.SwpcError):
+        gpu.SwpcGPU(40, 36, 32, 8, 3)
+
+
+def test_missing_library_raises(tmp_path):
+    saved = gpu._lib
+    try:
+        gpu._lib = None
+        with pytest.raises(gpu.SwpcError):
+            gpu.load_library(str(tmp_path / "libswpc_gpu.so"))
+    finally:
+        gpu._lib = saved

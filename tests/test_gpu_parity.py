@@ -116,15 +116,16 @@ def test_vmax_matches(orc):
     assert np.array_equal(v_g, v_o), (v_g, v_o)
 
 
-@pytest.mark.parametrize("kchunk,by,variant", [(7, 1, 0), (16, 2, 1), (200, 4, 1), (64, 4, 0)])
-def test_launch_shapes_same_result(orc, kchunk, by, variant):
+@pytest.mark.parametrize("kchunk,by,variant,graph", [(7, 1, 0, False), (16, 2, 1, True), (200, 4, 3, True),
+                                                     (64, 4, 3, False), (32, 2, 2, True), (64, 4, 2, False)])
+def test_launch_shapes_same_result(orc, kchunk, by, variant, graph):
     cfg = small_config(nx=44, ny=40, nz=36)
     m = O.build_model(cfg)
     run = O.OracleRun(m, 32)
     run.run(20)
     g = gpu_from_model(m, kchunk=kchunk, by=by, variant=variant)
-    g.step(1, 20)
-    _check(g, run, f"kchunk={kchunk} by={by} variant={variant}")
+    g.step(1, 20, use_graph=graph, block=6)
+    _check(g, run, f"kchunk={kchunk} by={by} variant={variant} graph={graph}")
 
 
 @pytest.mark.parametrize("na,by", [(7, 4), (5, 2), (13, 1)])
