@@ -139,6 +139,26 @@ int swpc_update_vel(swpc_ctx *ctx);
    HIP graph of `block` steps (falls back to direct launches for the tail). */
 int swpc_step(swpc_ctx *ctx, int it0, int nsteps, int use_graph, int block);
 
+/* Launch slots timed by swpc_step_timed, in step order. */
+enum swpc_timer {
+    SWPC_T_WAV = 0,       /* wav__store                                          */
+    SWPC_T_STRESS_SIDE,   /* absorb__update_stress, side columns (x and y slabs)  */
+    SWPC_T_STRESS,        /* kernel__update_stress + its bottom PML band         */
+    SWPC_T_STRESS_ABC,    /* absorb__update_stress with the Cerjan sponge        */
+    SWPC_T_GLUT,          /* source__stressglut                                  */
+    SWPC_T_VEL_SIDE,      /* absorb__update_vel, side columns                    */
+    SWPC_T_VEL,           /* kernel__update_vel + its bottom PML band            */
+    SWPC_T_VEL_ABC,       /* absorb__update_vel with the Cerjan sponge           */
+    SWPC_T_BODYFORCE,     /* source__bodyforce                                   */
+    SWPC_NTIMER
+};
+
+/* swpc_step with direct launches and HIP events between them: ms[t] receives
+   the milliseconds of slot t summed over the nsteps steps (the per-routine
+   breakdown of the reference's stopwatch_mode, src/shared/m_pwatch.f90:35-195,
+   src/swpc_3d/main.f90:66).  Synchronises after every step. */
+int swpc_step_timed(swpc_ctx *ctx, int it0, int nsteps, float *ms);
+
 /* kernel__vmax (src/swpc_3d/m_kernel.f90:350-374): max |V| one cell below
    the ocean bottom, 5 cells inside the absorber. Synchronises. */
 int swpc_kernel_vmax(swpc_ctx *ctx, float out[3]);

@@ -612,38 +612,49 @@ extern "C" int swpc_wav_store(swpc_ctx *c, int it)
 
 // one step in the order of src/swpc_3d/main.f90:122-138.  d_it != nullptr:
 // kernels read the step number from device memory and the step ends by
-// advancing it (graph-capturable).
-static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it)
+// advancing it (graph-capturable).  marks != nullptr: an event is recorded
+// before the step and after each of the SWPC_NTIMER launch slots (timing;
+// no second stream then).
+static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it, hipEvent_t *marks = nullptr)
 {
     // kernel__update_stress and absorb__update_stress touch disjoint cells
-    // and only read V: with the PML both run in one launch; the Cerjan taper
-    // is a separate sweep over the updated stress, as in the reference.
+    // and only read V: with the PML the interior columns (interior planes and
+    // bottom band) are one launch and the side columns another; the Cerjan
+    // taper is a separate sweep over the updated stress, as in the reference.
     hipError_t e;
     const bool pml = c->desc.abc == SWPC_ABC_PML;
-    const bool fork = pml && (c->variant & 2) && c->npml_tiles > 0;
-    if ((e = launch_wav_store(c, it, d_it)) != hipSuccess) return e;
-    if (fork) {
-        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
-    }
-    if ((e = launch_stress(c, pml ? 3 : 1, fork ? c->stream2 : nullptr)) != hipSuccess) return e;
-    if (fork) {
-        if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
-    }
+    const bool fork = !marks && pml && (c->variant & 2) && c->npml_tiles > 0;
+    hipStream_t side = fork ? c->stream2 : c->stream;
+    int slot = 0;
+    auto mark = [&]() -> hipError_t { return marks ? hipEventRecord(marks[slot++], c->stream) : hipSuccess; };
+    auto fork_to_side = [&]() -> hipError_t {
+        if (!fork) return hipSuccess;
+        hipError_t r = hipEventRecord(c->ev_fork, c->stream);
+        return r != hipSuccess ? r : hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+    };
+    auto join_side = [&]() -> hipError_t {
+        if (!fork) return hipSuccess;
+        hipError_t r = hipEventRecord(c->ev_join, c->stream2);
+        return r != hipSuccess ? r : hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    };
+    if ((e = mark()) != hipSuccess) return e;
+    if ((e = launch_wav_store(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
+    if ((e = fork_to_side()) != hipSuccess) return e;
+    if (pml && (e = launch_stress_side(c, side)) != hipSuccess) return e;
+    if ((e = mark()) != hipSuccess) return e;
+    if ((e = launch_stress_cols(c, pml ? 3 : 1)) != hipSuccess || (e = join_side()) != hipSuccess) return e;
+    if ((e = mark()) != hipSuccess) return e;
     if (!pml && (e = launch_cerjan_stress(c)) != hipSuccess) return e;
-    if ((e = launch_stressglut(c, it, d_it)) != hipSuccess) return e;
-    if (fork) {
-        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
-    }
-    if ((e = launch_vel(c, pml ? 3 : 1, fork ? c->stream2 : nullptr)) != hipSuccess) return e;
-    if (fork) {
-        if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
-    }
+    if ((e = mark()) != hipSuccess) return e;
+    if ((e = launch_stressglut(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
+    if ((e = fork_to_side()) != hipSuccess) return e;
+    if (pml && (e = launch_vel_side(c, side)) != hipSuccess) return e;
+    if ((e = mark()) != hipSuccess) return e;
+    if ((e = launch_vel_cols(c, pml ? 3 : 1)) != hipSuccess || (e = join_side()) != hipSuccess) return e;
+    if ((e = mark()) != hipSuccess) return e;
     if (!pml && (e = launch_cerjan_vel(c)) != hipSuccess) return e;
-    if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess) return e;
+    if ((e = mark()) != hipSuccess) return e;
+    if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if (d_it) e = launch_step_advance(c);
     return e;
 }
@@ -684,6 +695,32 @@ extern "C" int swpc_step(swpc_ctx *c, int it0, int nsteps, int use_graph, int bl
         for (; done + block <= nsteps; done += block) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     }
     for (; done < nsteps; ++done) HIPCHK(c, enqueue_step(c, it0 + done, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_step_timed(swpc_ctx *c, int it0, int nsteps, float *ms)
+{
+    if (!c || !ms) return SWPC_EINVAL;
+    if (nsteps < 0) return fail(c, SWPC_EINVAL, "swpc_step_timed: negative step count");
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int t = 0; t < SWPC_NTIMER; ++t) ms[t] = 0.0f;
+    hipEvent_t ev[SWPC_NTIMER + 1] = {};
+    hipError_t e = hipSuccess;
+    for (int t = 0; t <= SWPC_NTIMER && e == hipSuccess; ++t) e = hipEventCreate(&ev[t]);
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+        e = enqueue_step(c, it0 + s, nullptr, ev);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[SWPC_NTIMER]);
+        for (int t = 0; t < SWPC_NTIMER && e == hipSuccess; ++t) {
+            float x = 0.0f;
+            e = hipEventElapsedTime(&x, ev[t], ev[t + 1]);
+            ms[t] += x;
+        }
+    }
+    for (int t = 0; t <= SWPC_NTIMER; ++t)
+        if (ev[t]) (void)hipEventDestroy(ev[t]);
+    if (e != hipSuccess) return fail(c, SWPC_EHIP, "swpc_step_timed: %s", hipGetErrorString(e));
     return SWPC_OK;
 }
 

@@ -639,50 +639,68 @@ static void step_args(swpc_ctx *c, StepArgs &a, int mask)
     a.nbz = (g.nz + a.kchunk - 1) / a.kchunk;
 }
 
-hipError_t launch_stress(swpc_ctx *c, int mask, hipStream_t pml_stream)
+// side-column PML tiles of one phase (y slabs, x slabs)
+hipError_t launch_stress_side(swpc_ctx *c, hipStream_t s)
+{
+    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0 || c->npml_tiles <= 0) return hipSuccess;
+    StepArgs a;
+    step_args(c, a, MASK_PML);
+    hipLaunchKernelGGL(k_stress_pml, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vel_side(swpc_ctx *c, hipStream_t s)
+{
+    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0 || c->npml_tiles <= 0) return hipSuccess;
+    StepArgs a;
+    step_args(c, a, MASK_PML);
+    hipLaunchKernelGGL(k_vel_pml, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
+    return hipGetLastError();
+}
+
+// interior columns: interior planes (mask & 1) and their bottom PML band (mask & 2)
+hipError_t launch_stress_cols(swpc_ctx *c, int mask)
 {
     StepArgs a;
     step_args(c, a, mask);
-    dim3 block(BX, a.by);
-    if ((mask & MASK_PML) && a.has_pml && c->npml_tiles > 0) {
-        hipLaunchKernelGGL(k_stress_pml, dim3(c->npml_tiles), block, 0, pml_stream ? pml_stream : c->stream, a);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (c->g.nxk > 0 && c->g.nyk > 0 && ((mask & MASK_INT) || a.has_pml)) {
-        dim3 grid(a.nbx * a.nby * a.nbz);
-        const bool nt = (c->variant & 1) != 0;
+    if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
+    dim3 block(BX, a.by), grid(a.nbx * a.nby * a.nbz);
+    const bool nt = (c->variant & 1) != 0;
 #define SWPC_STRESS_CASE(NMV)                                                                       \
     case NMV:                                                                                       \
         if (nt) hipLaunchKernelGGL((k_stress_int<NMV, true>), grid, block, 0, c->stream, a);      \
         else hipLaunchKernelGGL((k_stress_int<NMV, false>), grid, block, 0, c->stream, a);        \
         break;
-        switch (c->g.nm) {
-            SWPC_STRESS_CASE(0) SWPC_STRESS_CASE(1) SWPC_STRESS_CASE(2) SWPC_STRESS_CASE(3) SWPC_STRESS_CASE(4)
-            SWPC_STRESS_CASE(5) SWPC_STRESS_CASE(6) SWPC_STRESS_CASE(7) SWPC_STRESS_CASE(8)
-        default: return hipErrorInvalidValue;
-        }
-#undef SWPC_STRESS_CASE
-        return hipGetLastError();
+    switch (c->g.nm) {
+        SWPC_STRESS_CASE(0) SWPC_STRESS_CASE(1) SWPC_STRESS_CASE(2) SWPC_STRESS_CASE(3) SWPC_STRESS_CASE(4)
+        SWPC_STRESS_CASE(5) SWPC_STRESS_CASE(6) SWPC_STRESS_CASE(7) SWPC_STRESS_CASE(8)
+    default: return hipErrorInvalidValue;
     }
-    return hipSuccess;
+#undef SWPC_STRESS_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_vel_cols(swpc_ctx *c, int mask)
+{
+    StepArgs a;
+    step_args(c, a, mask);
+    if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
+    hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), dim3(BX, a.by), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress(swpc_ctx *c, int mask, hipStream_t pml_stream)
+{
+    hipError_t e;
+    if ((mask & MASK_PML) && (e = launch_stress_side(c, pml_stream)) != hipSuccess) return e;
+    return launch_stress_cols(c, mask);
 }
 
 hipError_t launch_vel(swpc_ctx *c, int mask, hipStream_t pml_stream)
 {
-    StepArgs a;
-    step_args(c, a, mask);
-    dim3 block(BX, a.by);
-    if ((mask & MASK_PML) && a.has_pml && c->npml_tiles > 0) {
-        hipLaunchKernelGGL(k_vel_pml, dim3(c->npml_tiles), block, 0, pml_stream ? pml_stream : c->stream, a);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (c->g.nxk > 0 && c->g.nyk > 0 && ((mask & MASK_INT) || a.has_pml)) {
-        hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), block, 0, c->stream, a);
-        return hipGetLastError();
-    }
-    return hipSuccess;
+    hipError_t e;
+    if ((mask & MASK_PML) && (e = launch_vel_side(c, pml_stream)) != hipSuccess) return e;
+    return launch_vel_cols(c, mask);
 }
 
 hipError_t launch_kernel_stress(swpc_ctx *c) { return launch_stress(c, MASK_INT, nullptr); }

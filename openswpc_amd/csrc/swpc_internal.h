@@ -152,6 +152,10 @@ struct swpc_ctx {
 // given (it updates cells disjoint from the interior kernel's)
 hipError_t launch_stress(swpc_ctx *c, int mask, hipStream_t pml_stream);
 hipError_t launch_vel(swpc_ctx *c, int mask, hipStream_t pml_stream);
+hipError_t launch_stress_side(swpc_ctx *c, hipStream_t s);
+hipError_t launch_vel_side(swpc_ctx *c, hipStream_t s);
+hipError_t launch_stress_cols(swpc_ctx *c, int mask);
+hipError_t launch_vel_cols(swpc_ctx *c, int mask);
 hipError_t launch_kernel_stress(swpc_ctx *c);
 hipError_t launch_kernel_vel(swpc_ctx *c);
 hipError_t launch_pml_stress(swpc_ctx *c);

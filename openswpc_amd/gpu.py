@@ -31,6 +31,7 @@ FIELDS = {"Vx": 0, "Vy": 1, "Vz": 2, "Sxx": 3, "Syy": 4, "Szz": 5, "Syz": 6, "Sx
 WAV = {"vel": 1, "disp": 2, "stress": 4, "strain": 8}
 NCMP = {"vel": 3, "disp": 3, "stress": 6, "strain": 6}
 ABC = {"pml": 0, "cerjan": 1}
+TIMERS = ("wav", "stress_side", "stress", "stress_abc", "glut", "vel_side", "vel", "vel_abc", "bodyforce")
 
 
 class SwpcError(RuntimeError):
@@ -73,6 +74,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for nm in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store"):
         getattr(L, nm).argtypes = [vp, i]
     L.swpc_step.argtypes = [vp, i, i, i, i]
+    L.swpc_step_timed.argtypes = [vp, i, i, fp]
     L.swpc_kernel_vmax.argtypes = [vp, fp]
     L.swpc_get_field.argtypes = [vp, i, fp]
     L.swpc_set_field.argtypes = [vp, i, fp]
@@ -189,6 +191,13 @@ class SwpcGPU:
 
     def step(self, it0, nsteps, use_graph=False, block=100):
         self._chk(self.L.swpc_step(self.h, int(it0), int(nsteps), 1 if use_graph else 0, int(block)))
+
+    def step_timed(self, it0, nsteps):
+        """direct-launch steps with HIP events between launches; returns the
+        milliseconds of each launch slot (TIMERS) summed over the steps"""
+        ms = np.zeros(len(TIMERS), np.float32)
+        self._chk(self.L.swpc_step_timed(self.h, int(it0), int(nsteps), ms))
+        return dict(zip(TIMERS, (float(x) for x in ms)))
 
     def synchronize(self):
         self._chk(self.L.swpc_synchronize(self.h))

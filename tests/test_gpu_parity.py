@@ -68,6 +68,23 @@ def test_graph_replay_matches_direct(orc):
     assert ulp_diff(g.seismograms("vel"), run.wav["vel"]) == 0
 
 
+@pytest.mark.parametrize("abc", ["pml", "cerjan"])
+def test_step_timed_matches_oracle(orc, abc):
+    """swpc_step_timed runs the same launches as swpc_step, with events between them."""
+    cfg = small_config(abc=abc)
+    m = O.build_model(cfg)
+    run = O.OracleRun(m, 32)
+    run.run(9)
+    g = gpu_from_model(m)
+    ms = g.step_timed(1, 9)
+    _check(g, run, f"timed steps {abc}")
+    assert ulp_diff(g.seismograms("vel"), run.wav["vel"]) == 0
+    assert all(v >= 0.0 for v in ms.values())
+    assert ms["stress"] > 0.0 and ms["vel"] > 0.0
+    # an empty slot still shows the few microseconds between two events
+    assert ms["stress_side" if abc == "pml" else "stress_abc"] > 0.0
+
+
 def test_random_fields_one_step_each_routine(orc):
     """Random V/S everywhere (band, interior, PML) exercise every stencil term."""
     cfg = small_config(nm=3, sources=[], stations=[])
