@@ -45,6 +45,63 @@ __device__ static inline float hmean4(float a, float b, float c, float d, float 
 
 #define EPSL 1.1920928955078125e-07f  // epsilon(1.0)
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename T>
+__device__ static inline T ldv(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ static inline void stv(T *p, T v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// The 6*NM memory variables of one cell, value v = c*NM + m for component
+// c = xx yy zz yz xz xy and mechanism m, packed as 16-byte vectors: value v
+// lives in vector v/4 of stream v/4 (streams of rn float4 each), the last
+// two values (6*NM mod 4 = 2) in a float2 stream after them.  One cell's
+// values are loaded and stored together, so a wave moves 1 KB per access and
+// the kernel keeps 6*NM/4 streams open instead of 6*NM.
+template <int NM>
+struct RCell {
+    static constexpr int NV = 6 * NM, N4 = NV / 4, N2 = (NV % 4) / 2;
+    float v[NV > 0 ? NV : 1];
+
+    template <bool NT>
+    __device__ __forceinline__ void load(const float *R, long long rn, long long r)
+    {
+#pragma unroll
+        for (int s = 0; s < N4; ++s) {
+            const f4v t = ldv<NT>((const f4v *)R + s * rn + r);
+            v[4 * s] = t.x; v[4 * s + 1] = t.y; v[4 * s + 2] = t.z; v[4 * s + 3] = t.w;
+        }
+        if constexpr (N2 == 1) {
+            const f2v t = ldv<NT>((const f2v *)(R + 4 * N4 * rn) + r);
+            v[4 * N4] = t.x; v[4 * N4 + 1] = t.y;
+        }
+    }
+    template <bool NT>
+    __device__ __forceinline__ void store(float *R, long long rn, long long r) const
+    {
+#pragma unroll
+        for (int s = 0; s < N4; ++s) {
+            f4v t;
+            t.x = v[4 * s]; t.y = v[4 * s + 1]; t.z = v[4 * s + 2]; t.w = v[4 * s + 3];
+            stv<NT>((f4v *)R + s * rn + r, t);
+        }
+        if constexpr (N2 == 1) {
+            f2v t;
+            t.x = v[4 * N4]; t.y = v[4 * N4 + 1];
+            stv<NT>((f2v *)(R + 4 * N4 * rn) + r, t);
+        }
+    }
+};
+
 // Streams read or written once per step (memory variables, stress, the
 // per-cell medium) can bypass cache retention so the L2 keeps the V and mu
 // rows that neighbouring tiles re-read.
@@ -317,19 +374,18 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a)
             const float dxVx_dzVz = dxVx + dzVz;
             const float dxVx_dyVy = dxVx + dyVy;
 
+            RCell<NM> R;
+            R.template load<NT>(a.R, rn, r);
             float Rxx_n = 0.0f, Ryy_n = 0.0f, Rzz_n = 0.0f;
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
-                float *rxx = a.R + (0 * NM + m) * rn + r;
-                float *ryy = a.R + (1 * NM + m) * rn + r;
-                float *rzz = a.R + (2 * NM + m) * rn + r;
                 const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-                const float nxx = c1 * ld1<NT>(rxx) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
-                const float nyy = c1 * ld1<NT>(ryy) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
-                const float nzz = c1 * ld1<NT>(rzz) - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
-                st1<NT>(rxx, nxx);
-                st1<NT>(ryy, nyy);
-                st1<NT>(rzz, nzz);
+                const float nxx = c1 * R.v[0 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
+                const float nyy = c1 * R.v[1 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
+                const float nzz = c1 * R.v[2 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
+                R.v[0 * NM + m] = nxx;
+                R.v[1 * NM + m] = nyy;
+                R.v[2 * NM + m] = nzz;
                 Rxx_n = Rxx_n + d1 * nxx;
                 Ryy_n = Ryy_n + d1 * nyy;
                 Rzz_n = Rzz_n + d1 * nzz;
@@ -355,20 +411,18 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a)
             float Ryz_n = 0.0f, Rxz_n = 0.0f, Rxy_n = 0.0f;
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
-                float *ryz = a.R + (3 * NM + m) * rn + r;
-                float *rxz = a.R + (4 * NM + m) * rn + r;
-                float *rxy = a.R + (5 * NM + m) * rn + r;
                 const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-                const float nyz = c1 * ld1<NT>(ryz) - c2 * muyz * taus1 * dyVz_dzVy * dt;
-                const float nxz = c1 * ld1<NT>(rxz) - c2 * muxz * taus1 * dxVz_dzVx * dt;
-                const float nxy = c1 * ld1<NT>(rxy) - c2 * muxy * taus1 * dxVy_dyVx * dt;
-                st1<NT>(ryz, nyz);
-                st1<NT>(rxz, nxz);
-                st1<NT>(rxy, nxy);
+                const float nyz = c1 * R.v[3 * NM + m] - c2 * muyz * taus1 * dyVz_dzVy * dt;
+                const float nxz = c1 * R.v[4 * NM + m] - c2 * muxz * taus1 * dxVz_dzVx * dt;
+                const float nxy = c1 * R.v[5 * NM + m] - c2 * muxy * taus1 * dxVy_dyVx * dt;
+                R.v[3 * NM + m] = nyz;
+                R.v[4 * NM + m] = nxz;
+                R.v[5 * NM + m] = nxy;
                 Ryz_n = Ryz_n + d1 * nyz;
                 Rxz_n = Rxz_n + d1 * nxz;
                 Rxy_n = Rxy_n + d1 * nxy;
             }
+            R.template store<NT>(a.R, rn, r);
             st1<NT>(Syz + p, ld1<NT>(Syz + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
             st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
             st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);

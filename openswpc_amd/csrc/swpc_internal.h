@@ -4,7 +4,8 @@
 //   3D halo arrays (fields V3/S6, medium rho/lam/mu/taup/taus):
 //     idx(i,j,k) = (k+2)*plane + (j+2)*pitch + (i-1+ox),  i,j,k in -2..n+3
 //     ox = 32 puts i = 1 on a 128-byte line; pitch is a multiple of 32 floats.
-//   Memory variables: 6*NM arrays over the kernel rows j in [jb,je], planes
+//   Memory variables: 6*NM values per cell packed into float4 streams (and a
+//     float2 tail) over the kernel rows j in [jb,je], planes
 //     k in [kb,ke], with full-width rows: ridx(i,j,k) = ((k-kb)*nyk + (j-jb))*rpitch + (i-1).
 //   PML auxiliaries: 18 arrays over the PML cells only, indexed by a compact
 //     cell number running over five boxes (two y slabs, two x slabs, bottom).
