@@ -95,6 +95,46 @@ static int build_pml_tiles(swpc_ctx *c)
     return SWPC_OK;
 }
 
+// k chunks of the interior-column kernels.  Blocks of one chunk form a
+// contiguous run of logical tiles, so each XCD takes whole chunks; a plane
+// of the bottom PML band costs a different time than an interior plane (more
+// in the velocity phase, about the same in the stress phase), so chunk
+// boundaries equalise estimated time per chunk rather than planes.
+static int build_zchunks(swpc_ctx *c)
+{
+    const Grid &g = c->g;
+    const int nz = g.nz, nm = g.nm;
+    const int nbz = std::max(1, std::min(nz, (nz + c->kchunk - 1) / c->kchunk));
+    const bool pml = c->desc.abc == SWPC_ABC_PML && c->npml > 0;
+    // relative time per cell: algorithmic bytes over the streaming rate the
+    // two loops reach (interior ~4.7 TB/s, PML band ~3.0 TB/s on MI355X)
+    const double w_int[2] = {4.0 * (3 + 4 + 12 + 12 * nm), 4.0 * (6 + 1 + 6)};   // stress, velocity
+    const double w_pml[2] = {4.0 * (3 + 2 + 12 + 18) * 1.55, 4.0 * (6 + 1 + 6 + 18) * 1.55};
+    std::vector<int> zc(2 * (nbz + 1));
+    for (int ph = 0; ph < 2; ++ph) {
+        std::vector<double> W(nz + 1, 0.0);
+        for (int k = 1; k <= nz; ++k) W[k] = W[k - 1] + ((pml && k > g.ke) ? w_pml[ph] : w_int[ph]);
+        int *z = zc.data() + ph * (nbz + 1);
+        z[0] = 1;
+        int k = 0;
+        for (int b = 1; b < nbz; ++b) {
+            const double target = W[nz] * b / nbz;
+            while (k < nz && W[k] < target) ++k;
+            // chunk b starts after plane k; keep every chunk at least one plane deep
+            z[b] = std::min(std::max(k + 1, z[b - 1] + 1), nz - (nbz - b) + 1);
+        }
+        z[nbz] = nz + 1;
+    }
+    dfree(c->d_zchunks);
+    c->d_zchunks = nullptr;
+    c->nbz = nbz;
+    if (dalloc(c, (void **)&c->d_zchunks, zc.size() * sizeof(int)) != hipSuccess)
+        return fail(c, SWPC_ENOMEM, "k-chunk table allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_zchunks, zc.data(), zc.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
 extern "C" const char *swpc_last_error(const swpc_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
 extern "C" int64_t swpc_device_bytes(const swpc_ctx *ctx) { return ctx ? ctx->bytes : 0; }
@@ -115,7 +155,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     memset(c, 0, sizeof(*c));
     c->desc = *d;
     c->device = d->device;
-    c->kchunk = 64;
+    c->kchunk = 32;
     c->by = 4;
     c->variant = 1;
     if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
@@ -177,7 +217,10 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
             {1, ny - na + 1, 1, nx, na, nz, 0},
             {1, na + 1, 1, na, ny - 2 * na, nz, 0},
             {nx - na + 1, na + 1, 1, na, ny - 2 * na, nz, 0},
-            {na + 1, na + 1, nz - na + 1, nx - 2 * na, ny - 2 * na, na, 0},
+            // bottom band under the interior columns: rows padded to the
+            // full aligned width (i from 1) so a wave's auxiliaries start
+            // on a 128-byte line like its field rows
+            {1, na + 1, nz - na + 1, g.rpitch, ny - 2 * na, na, 0},
         };
         long long off = 0;
         for (int q = 0; q < 5; ++q) {
@@ -224,6 +267,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = SWPC_EHIP; goto bad; }
     if ((rc = build_pml_tiles(c)) != SWPC_OK) goto bad;
+    if ((rc = build_zchunks(c)) != SWPC_OK) goto bad;
     *out = c;
     return SWPC_OK;
 bad:
@@ -240,7 +284,7 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     for (int q = 0; q < SWPC_NFIELD; ++q) dfree(c->fields[q]);
     dfree(c->rho); dfree(c->lam); dfree(c->mu); dfree(c->taup); dfree(c->taus);
     dfree(c->kfs_top); dfree(c->kfs_bot); dfree(c->kob_top); dfree(c->kob_bot); dfree(c->kob);
-    dfree(c->R); dfree(c->ade); dfree(c->d_pml_tiles);
+    dfree(c->R); dfree(c->ade); dfree(c->d_pml_tiles); dfree(c->d_zchunks);
     dfree(c->gxc); dfree(c->gxe); dfree(c->gyc); dfree(c->gye); dfree(c->gzc); dfree(c->gze);
     dfree(c->cgx_c); dfree(c->cgx_b); dfree(c->cgy_c); dfree(c->cgy_b); dfree(c->cgz_c); dfree(c->cgz_b);
     dfree(c->d_targets); dfree(c->d_contrib); dfree(c->d_comp); dfree(c->d_mo); dfree(c->d_stf);
@@ -787,5 +831,6 @@ extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by, int variant)
     if (by > 0) c->by = by;
     if (variant >= 0) c->variant = variant;
     reset_graph(c);
-    return build_pml_tiles(c);
+    int rc = build_pml_tiles(c);
+    return rc != SWPC_OK ? rc : build_zchunks(c);
 }

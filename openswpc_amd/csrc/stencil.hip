@@ -136,7 +136,8 @@ struct StepArgs {
     int ntiles;
     int has_pml, mask;
     // interior launch shape
-    int kchunk, bx0, nbx, nby, nbz, by;
+    const int *zc;       // first plane of each k chunk (nbz + 1 entries, last = nz + 1)
+    int bx0, nbx, nby, nbz, by;
 };
 
 // ---------------------------------------------------------------------------
@@ -302,8 +303,8 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a)
     const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
-    const int k0 = 1 + bz * a.kchunk;
-    const int k1 = min(g.nz, k0 + a.kchunk - 1);
+    const int k0 = a.zc[bz];
+    const int k1 = a.zc[bz + 1] - 1;
     const int ka1 = (a.mask & MASK_INT) ? min(k1, g.ke) : k0 - 1;      // interior planes k0..ka1
     const int kb0 = max(k0, g.ke + 1);
     const int kb1 = (a.has_pml && (a.mask & MASK_PML)) ? k1 : kb0 - 1;  // bottom PML planes kb0..kb1
@@ -474,8 +475,8 @@ __global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
     const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
-    const int k0 = 1 + bz * a.kchunk;
-    const int k1 = min(g.nz, k0 + a.kchunk - 1);
+    const int k0 = a.zc[bz];
+    const int k1 = a.zc[bz + 1] - 1;
     const int ka1 = (a.mask & MASK_INT) ? min(k1, g.ke) : k0 - 1;
     const int kb0 = max(k0, g.ke + 1);
     const int kb1 = (a.has_pml && (a.mask & MASK_PML)) ? k1 : kb0 - 1;
@@ -685,12 +686,12 @@ static void step_args(swpc_ctx *c, StepArgs &a, int mask)
     a.has_pml = (c->desc.abc == SWPC_ABC_PML && c->npml > 0) ? 1 : 0;
     a.mask = mask;
     a.by = c->by;
-    a.kchunk = c->kchunk;
+    a.zc = c->d_zchunks;
     const Grid &g = c->g;
     a.bx0 = (g.ib - 1) / BX;
     a.nbx = (g.ie - 1) / BX - a.bx0 + 1;
     a.nby = (g.nyk + a.by - 1) / a.by;
-    a.nbz = (g.nz + a.kchunk - 1) / a.kchunk;
+    a.nbz = c->nbz;
 }
 
 // side-column PML tiles of one phase (y slabs, x slabs)
@@ -738,6 +739,7 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
 {
     StepArgs a;
     step_args(c, a, mask);
+    a.zc = c->d_zchunks + c->nbz + 1;
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
     hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), dim3(BX, a.by), 0, c->stream, a);
     return hipGetLastError();

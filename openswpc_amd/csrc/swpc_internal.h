@@ -8,7 +8,8 @@
 //     float2 tail) over the kernel rows j in [jb,je], planes
 //     k in [kb,ke], with full-width rows: ridx(i,j,k) = ((k-kb)*nyk + (j-jb))*rpitch + (i-1).
 //   PML auxiliaries: 18 arrays over the PML cells only, indexed by a compact
-//     cell number running over five boxes (two y slabs, two x slabs, bottom).
+//     cell number running over five boxes (two y slabs, two x slabs, bottom);
+//     the bottom box keeps full aligned rows (i from 1, rpitch wide).
 //   (i,j) maps: (j+2)*pitch + (i-1+ox).
 #pragma once
 
@@ -143,6 +144,8 @@ struct swpc_ctx {
     float *d_vmax;
 
     int kchunk, by;
+    int nbz;           // k chunks of the interior-column kernels
+    int *d_zchunks;    // first plane of each chunk, stress phase then velocity phase (2 x (nbz+1))
     int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel,
                        // bit 1: side-column PML kernels on a second stream
     long long bytes;
