@@ -1,0 +1,190 @@
+!! m_swpc_gpu -- iso_c_binding interfaces to libswpc_gpu (include/swpc_gpu.h)
+!!
+!! A Fortran caller (the reference's own main loop, or swpc_fdrive.f90 here)
+!! uses this module to drive the MI355X time loop.  Arrays are passed in the
+!! reference's layout: 3D (k,i,j) with bounds (-2:nz+3, -2:nx+3, -2:ny+3),
+!! 2D maps (-2:nx+3, -2:ny+3), real(SP) = c_float.  Every function returns 0
+!! on success and a negative code otherwise; swpc_last_error_f gives the text.
+module m_swpc_gpu
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: SWPC_ABC_PML = 0, SWPC_ABC_CERJAN = 1
+    integer(c_int), parameter :: SWPC_VX = 0, SWPC_VY = 1, SWPC_VZ = 2, SWPC_SXX = 3, SWPC_SYY = 4, &
+                                 SWPC_SZZ = 5, SWPC_SYZ = 6, SWPC_SXZ = 7, SWPC_SXY = 8
+    integer(c_int), parameter :: SWPC_WAV_V = 1, SWPC_WAV_U = 2, SWPC_WAV_STRESS = 4, SWPC_WAV_STRAIN = 8
+    integer(c_int), parameter :: SWPC_NTIMER = 9
+
+    type, bind(C) :: swpc_desc
+        integer(c_int) :: nx, ny, nz, na, nm, abc
+        real(c_double) :: dx, dy, dz
+        real(c_float) :: dt
+        integer(c_int) :: device
+    end type swpc_desc
+
+    interface
+        integer(c_int) function swpc_create(desc, ctx) bind(C, name='swpc_create')
+            import :: c_int, c_ptr, swpc_desc
+            type(swpc_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function swpc_destroy(ctx) bind(C, name='swpc_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        type(c_ptr) function swpc_last_error(ctx) bind(C, name='swpc_last_error')
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_set_medium(ctx, rho, lam, mu, taup, taus, kfs_top, kfs_bot, kob_top, kob_bot, &
+                                                kob) bind(C, name='swpc_set_medium')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: rho(*), lam(*), mu(*), taup(*), taus(*)
+            integer(c_int), intent(in) :: kfs_top(*), kfs_bot(*), kob_top(*), kob_bot(*), kob(*)
+        end function
+        integer(c_int) function swpc_set_visco(ctx, c1, c2, d1, d2) bind(C, name='swpc_set_visco')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: c1(*), c2(*), d1(*)
+            real(c_float), value :: d2
+        end function
+        integer(c_int) function swpc_set_pml(ctx, gxc, gxe, gyc, gye, gzc, gze) bind(C, name='swpc_set_pml')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: gxc(*), gxe(*), gyc(*), gye(*), gzc(*), gze(*)
+        end function
+        integer(c_int) function swpc_set_cerjan(ctx, gx_c, gx_b, gy_c, gy_b, gz_c, gz_b) bind(C, name='swpc_set_cerjan')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: gx_c(*), gx_b(*), gy_c(*), gy_b(*), gz_c(*), gz_b(*)
+        end function
+        integer(c_int) function swpc_set_sources(ctx, nsrc, bf_mode, isrc, jsrc, ksrc, mo, comp, M0, nt, stf) &
+            bind(C, name='swpc_set_sources')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, bf_mode, nt
+            integer(c_int), intent(in) :: isrc(*), jsrc(*), ksrc(*)
+            real(c_float), intent(in) :: mo(*), comp(*), stf(*)
+            real(c_float), value :: M0
+        end function
+        integer(c_int) function swpc_set_receivers(ctx, nst, ist, jst, kst, what, ntdec_w, ntw, M0, UC) &
+            bind(C, name='swpc_set_receivers')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nst, what, ntdec_w, ntw
+            integer(c_int), intent(in) :: ist(*), jst(*), kst(*)
+            real(c_float), value :: M0, UC
+        end function
+        integer(c_int) function swpc_kernel_update_stress(ctx) bind(C, name='swpc_kernel_update_stress')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_absorb_update_stress(ctx) bind(C, name='swpc_absorb_update_stress')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_source_stressglut(ctx, it) bind(C, name='swpc_source_stressglut')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+        integer(c_int) function swpc_kernel_update_vel(ctx) bind(C, name='swpc_kernel_update_vel')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_absorb_update_vel(ctx) bind(C, name='swpc_absorb_update_vel')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_source_bodyforce(ctx, it) bind(C, name='swpc_source_bodyforce')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+        integer(c_int) function swpc_wav_store(ctx, it) bind(C, name='swpc_wav_store')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+        integer(c_int) function swpc_update_stress(ctx) bind(C, name='swpc_update_stress')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_update_vel(ctx) bind(C, name='swpc_update_vel')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_step(ctx, it0, nsteps, use_graph, block) bind(C, name='swpc_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it0, nsteps, use_graph, block
+        end function
+        integer(c_int) function swpc_step_timed(ctx, it0, nsteps, ms) bind(C, name='swpc_step_timed')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it0, nsteps
+            real(c_float), intent(out) :: ms(*)
+        end function
+        integer(c_int) function swpc_kernel_vmax(ctx, out) bind(C, name='swpc_kernel_vmax')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: out(3)
+        end function
+        integer(c_int) function swpc_get_field(ctx, field, out) bind(C, name='swpc_get_field')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: field
+            real(c_float), intent(out) :: out(*)
+        end function
+        integer(c_int) function swpc_set_field(ctx, field, in) bind(C, name='swpc_set_field')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: field
+            real(c_float), intent(in) :: in(*)
+        end function
+        integer(c_int) function swpc_get_seismograms(ctx, kind, out) bind(C, name='swpc_get_seismograms')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: kind
+            real(c_float), intent(out) :: out(*)
+        end function
+        integer(c_int) function swpc_synchronize(ctx) bind(C, name='swpc_synchronize')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int64_t) function swpc_device_bytes(ctx) bind(C, name='swpc_device_bytes')
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_set_tuning(ctx, kchunk, by, variant) bind(C, name='swpc_set_tuning')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: kchunk, by, variant
+        end function
+        type(c_ptr) function swpc_stream(ctx) bind(C, name='swpc_stream')
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end function
+    end interface
+
+contains
+
+    !! swpc_last_error as a Fortran string
+    function swpc_last_error_f(ctx) result(msg)
+        type(c_ptr), intent(in) :: ctx
+        character(len=512) :: msg
+        character(kind=c_char), pointer :: p(:)
+        type(c_ptr) :: cp
+        integer :: n
+        msg = ''
+        cp = swpc_last_error(ctx)
+        if (.not. c_associated(cp)) return
+        call c_f_pointer(cp, p, [512])
+        do n = 1, 512
+            if (p(n) == c_null_char) exit
+            msg(n:n) = p(n)
+        end do
+    end function swpc_last_error_f
+
+end module m_swpc_gpu
