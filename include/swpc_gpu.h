@@ -139,9 +139,53 @@ int swpc_update_vel(swpc_ctx *ctx);
    HIP graph of `block` steps (falls back to direct launches for the tail). */
 int swpc_step(swpc_ctx *ctx, int it0, int nsteps, int use_graph, int block);
 
+/* ---- wavefield sections (snap__setup / snap__write,
+   src/swpc_3d/m_snap.f90:95-322, 919-2189) ---- */
+enum swpc_snap_plane { SWPC_SNP_XY = 0, SWPC_SNP_YZ, SWPC_SNP_XZ, SWPC_SNP_FS, SWPC_SNP_OB };
+enum swpc_snap_qty { SWPC_SNP_PS = 0, SWPC_SNP_V, SWPC_SNP_U };
+#define SWPC_SNP_NSECTION 15
+/* section id of plane p and quantity q: xy_ps = 0, xy_v = 1, ... ob_u = 14 */
+#define SWPC_SNP_SECTION(p, q) ((p) * 3 + (q))
+
+typedef struct swpc_snap_desc {
+    int sw[SWPC_SNP_NSECTION]; /* nonzero: section on (xy_ps%sw .. ob_u%sw, m_snap.f90:105-119) */
+    int k0_xy, i0_yz, j0_xz;   /* section planes as grid indices (z2k(z0_xy) etc., m_snap.f90:158-160) */
+    int idec, jdec, kdec;      /* spatial decimation: section point n samples grid n*dec - dec/2 */
+    int ntdec_s;               /* a frame at every step it with mod(it-1, ntdec_s) == 0 */
+    int nframe;                /* frames kept on the device per section: frame f = (it-1)/ntdec_s */
+    float M0, UC;              /* output scales, as for the receivers */
+} swpc_snap_desc;
+
+/* Enable sections.  kfs is the free-surface map (i,j) with the halo
+   (medium__setup, m_medium.f90:339-394), used by the fs sections.  The
+   sections are (nxs, nys) for xy/fs/ob, (nys, nzs) for yz and (nxs, nzs)
+   for xz, with nxs = (nx + idec/2)/idec etc. (m_snap.f90:130-132).  Frames,
+   u accumulators and maximum maps start at zero.  Replaces snap__setup. */
+int swpc_set_snapshots(swpc_ctx *ctx, const swpc_snap_desc *desc, const int32_t *kfs);
+
+/* snap__write (src/swpc_3d/m_snap.f90:919-947): takes the frames of step it
+   (ps and v at output steps; u accumulation and the fs/ob maxima every
+   step).  swpc_step calls it after wav__store, as main.f90:123-124 does.
+   Fails if a frame index would reach nframe. */
+int swpc_snap_write(swpc_ctx *ctx, int it);
+
+/* Frame f of a section as (ns1, ns2, ncomp) in Fortran order, ncomp = 4
+   for ps (div, rot_x, rot_y, rot_z) and 3 for v and u: the buffer
+   the reference writes per frame (m_snap.f90:1054-1057). */
+int swpc_get_snapshot(swpc_ctx *ctx, int section, int frame, float *out);
+
+/* Maximum-amplitude map (nxs, nys, 3) = (max |z|, max horizontal, their
+   norm) of an fs or ob, v or u section (max_fs_v etc., m_snap.f90:85,
+   written by output__put_maxval :2295-2348). */
+int swpc_get_snap_max(swpc_ctx *ctx, int section, float *out);
+
+/* dims = (ns1, ns2, ncomp) of a section. */
+int swpc_snap_dims(const swpc_ctx *ctx, int section, int dims[3]);
+
 /* Launch slots timed by swpc_step_timed, in step order. */
 enum swpc_timer {
     SWPC_T_WAV = 0,       /* wav__store                                          */
+    SWPC_T_SNAP,          /* snap__write                                         */
     SWPC_T_STRESS_SIDE,   /* absorb__update_stress, side columns (x and y slabs)  */
     SWPC_T_STRESS,        /* kernel__update_stress + its bottom PML band         */
     SWPC_T_STRESS_ABC,    /* absorb__update_stress with the Cerjan sponge        */

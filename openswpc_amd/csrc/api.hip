@@ -275,6 +275,23 @@ bad:
     return rc;
 }
 
+static void free_snapshots(swpc_ctx *c)
+{
+    for (int s = 0; s < SWPC_SNP_NSECTION; ++s) {
+        dfree(c->d_snap_frame[s]);
+        dfree(c->d_snap_max[s]);
+        c->d_snap_frame[s] = c->d_snap_max[s] = nullptr;
+    }
+    for (int p = 0; p < 5; ++p) {
+        dfree(c->d_snap_u[p]);
+        c->d_snap_u[p] = nullptr;
+    }
+    dfree(c->d_kfs);
+    c->d_kfs = nullptr;
+    c->snap_ntask = 0;
+    c->snap_nblk = 0;
+}
+
 extern "C" int swpc_destroy(swpc_ctx *c)
 {
     if (!c) return SWPC_OK;
@@ -291,6 +308,7 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     dfree(c->d_ist); dfree(c->d_jst); dfree(c->d_kst); dfree(c->d_uacc); dfree(c->d_eacc);
     for (int q = 0; q < 4; ++q) dfree(c->d_wav[q]);
     dfree(c->staging); dfree(c->d_vmax); dfree(c->d_it);
+    free_snapshots(c);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -567,6 +585,126 @@ static int ready(swpc_ctx *c)
     return SWPC_OK;
 }
 
+extern "C" int swpc_set_snapshots(swpc_ctx *c, const swpc_snap_desc *d, const int32_t *kfs)
+{
+    if (!c) return SWPC_EINVAL;
+    if (!d || !kfs) return fail(c, SWPC_EINVAL, "swpc_set_snapshots: null argument");
+    const Grid &g = c->g;
+    if (d->idec < 1 || d->jdec < 1 || d->kdec < 1 || d->ntdec_s < 1 || d->nframe < 1)
+        return fail(c, SWPC_EINVAL, "swpc_set_snapshots: decimations and nframe must be >= 1");
+    if (d->k0_xy < 1 || d->k0_xy > g.nz || d->i0_yz < 1 || d->i0_yz > g.nx || d->j0_xz < 1 || d->j0_xz > g.ny)
+        return fail(c, SWPC_EINVAL, "swpc_set_snapshots: section plane outside the grid");
+    for (int j = 1; j <= g.ny; ++j)
+        for (int i = 1; i <= g.nx; ++i) {
+            const int k = kfs[(size_t)(j + 2) * (g.nx + 6) + (i + 2)];
+            if (k < 0 || k > g.nz) return fail(c, SWPC_EINVAL, "swpc_set_snapshots: kfs outside 0..nz");
+        }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    reset_graph(c);
+    free_snapshots(c);
+    c->snap = *d;
+    const int nxs = (g.nx + d->idec / 2) / d->idec, nys = (g.ny + d->jdec / 2) / d->jdec;
+    const int nzs = (g.nz + d->kdec / 2) / d->kdec;
+    int nblk = 0, ntask = 0;
+    for (int s = 0; s < SWPC_SNP_NSECTION; ++s) {
+        const int plane = s / 3, q = s % 3;
+        const int ns1 = plane == SWPC_SNP_YZ ? nys : nxs;
+        const int ns2 = plane == SWPC_SNP_XY || plane >= SWPC_SNP_FS ? nys : nzs;
+        const int ncomp = q == SWPC_SNP_PS ? 4 : 3;
+        c->snap_dims[s][0] = ns1;
+        c->snap_dims[s][1] = ns2;
+        c->snap_dims[s][2] = ncomp;
+        if (!d->sw[s]) continue;
+        const size_t npt = (size_t)ns1 * ns2;
+        SnapTask &T = c->snap_task[ntask];
+        T.plane = plane;
+        T.q = q;
+        T.ns1 = ns1;
+        T.ns2 = ns2;
+        T.blk0 = nblk;
+        T.uacc = T.maxmap = nullptr;
+        if (dalloc(c, (void **)&c->d_snap_frame[s], npt * ncomp * (size_t)d->nframe * sizeof(float)) != hipSuccess)
+            return fail(c, SWPC_ENOMEM, "snapshot frame store allocation failed (%zu MB)",
+                        npt * ncomp * (size_t)d->nframe * 4 >> 20);
+        T.frame = c->d_snap_frame[s];
+        if (q == SWPC_SNP_U) {
+            if (dalloc(c, (void **)&c->d_snap_u[plane], npt * 3 * sizeof(float)) != hipSuccess)
+                return fail(c, SWPC_ENOMEM, "snapshot accumulator allocation failed");
+            T.uacc = c->d_snap_u[plane];
+        }
+        if (q != SWPC_SNP_PS && plane >= SWPC_SNP_FS) {
+            if (dalloc(c, (void **)&c->d_snap_max[s], npt * 3 * sizeof(float)) != hipSuccess)
+                return fail(c, SWPC_ENOMEM, "snapshot maximum map allocation failed");
+            T.maxmap = c->d_snap_max[s];
+        }
+        nblk += (int)((npt + 255) / 256);
+        ++ntask;
+    }
+    if (dalloc(c, (void **)&c->d_kfs, (size_t)g.plane * sizeof(int)) != hipSuccess)
+        return fail(c, SWPC_ENOMEM, "kfs map allocation failed");
+    int rc = upload_map(c, kfs, c->d_kfs);
+    if (rc) return rc;
+    c->snap_ntask = ntask;
+    c->snap_nblk = nblk;
+    return SWPC_OK;
+}
+
+// frames that steps it0 .. it0+nsteps-1 would take must fit the frame store
+static int check_snap_frames(swpc_ctx *c, int it0, int nsteps)
+{
+    if (c->snap_ntask <= 0 || nsteps <= 0) return SWPC_OK;
+    const int last = it0 + nsteps - 1;
+    if (last < 1) return SWPC_OK;
+    const int f = (last - 1) / c->snap.ntdec_s;
+    if (f >= c->snap.nframe)
+        return fail(c, SWPC_EINVAL, "snapshot frame store holds %d frames; step %d would take frame %d",
+                    c->snap.nframe, last, f);
+    return SWPC_OK;
+}
+
+extern "C" int swpc_snap_write(swpc_ctx *c, int it)
+{
+    if (!c) return SWPC_EINVAL;
+    int rc = check_snap_frames(c, it, 1);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_snap(c, it, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_snap_dims(const swpc_ctx *c, int section, int dims[3])
+{
+    if (!c || !dims || section < 0 || section >= SWPC_SNP_NSECTION) return SWPC_EINVAL;
+    for (int q = 0; q < 3; ++q) dims[q] = c->snap_dims[section][q];
+    return c->d_snap_frame[section] ? SWPC_OK : SWPC_ESTATE;
+}
+
+extern "C" int swpc_get_snapshot(swpc_ctx *c, int section, int frame, float *out)
+{
+    if (!c || !out || section < 0 || section >= SWPC_SNP_NSECTION) return SWPC_EINVAL;
+    if (!c->d_snap_frame[section]) return fail(c, SWPC_ESTATE, "swpc_get_snapshot: section %d not enabled", section);
+    if (frame < 0 || frame >= c->snap.nframe) return fail(c, SWPC_EINVAL, "swpc_get_snapshot: frame out of range");
+    const size_t n = (size_t)c->snap_dims[section][0] * c->snap_dims[section][1] * c->snap_dims[section][2];
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_snap_frame[section] + n * frame, n * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_get_snap_max(swpc_ctx *c, int section, float *out)
+{
+    if (!c || !out || section < 0 || section >= SWPC_SNP_NSECTION) return SWPC_EINVAL;
+    if (!c->d_snap_max[section])
+        return fail(c, SWPC_ESTATE, "swpc_get_snap_max: no maximum map for section %d (fs/ob v or u only)", section);
+    const size_t n = (size_t)c->snap_dims[section][0] * c->snap_dims[section][1] * 3;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_snap_max[section], n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
 extern "C" int swpc_kernel_update_stress(swpc_ctx *c)
 {
     if (!c) return SWPC_EINVAL;
@@ -683,6 +821,7 @@ static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it, hipEvent_t 
     };
     if ((e = mark()) != hipSuccess) return e;
     if ((e = launch_wav_store(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
+    if ((e = launch_snap(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if ((e = fork_to_side()) != hipSuccess) return e;
     if (pml && (e = launch_stress_side(c, side)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
@@ -714,6 +853,7 @@ extern "C" int swpc_step(swpc_ctx *c, int it0, int nsteps, int use_graph, int bl
     if (nsteps < 0) return fail(c, SWPC_EINVAL, "swpc_step: negative step count");
     int rc = ready(c);
     if (rc) return rc;
+    if ((rc = check_snap_frames(c, it0, nsteps))) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     int done = 0;
     if (use_graph && block > 0 && nsteps >= block) {
@@ -748,6 +888,7 @@ extern "C" int swpc_step_timed(swpc_ctx *c, int it0, int nsteps, float *ms)
     if (nsteps < 0) return fail(c, SWPC_EINVAL, "swpc_step_timed: negative step count");
     int rc = ready(c);
     if (rc) return rc;
+    if ((rc = check_snap_frames(c, it0, nsteps))) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     for (int t = 0; t < SWPC_NTIMER; ++t) ms[t] = 0.0f;
     hipEvent_t ev[SWPC_NTIMER + 1] = {};

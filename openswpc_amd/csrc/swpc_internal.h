@@ -88,6 +88,15 @@ struct SrcContrib {
     long long rho_a, rho_b;  // BF: the two densities of the staggered buoyancy
 };
 
+// one enabled section of snap__write
+struct SnapTask {
+    int plane, q, ns1, ns2;
+    int blk0;            // first block of the task in the snapshot launch
+    float *frame;        // (ns1, ns2, ncomp, nframe)
+    float *uacc;         // (ns1, ns2, 3) for u sections
+    float *maxmap;       // (ns1, ns2, 3) for fs/ob v and u sections
+};
+
 struct swpc_ctx {
     swpc_desc desc;
     Grid g;
@@ -133,6 +142,16 @@ struct swpc_ctx {
     float *d_uacc, *d_eacc;
     float *d_wav[4];   // V, U, stress, strain: (ntw, ncmp, nst)
 
+    // sections (snapshots)
+    swpc_snap_desc snap;
+    int snap_ntask, snap_nblk;
+    SnapTask snap_task[SWPC_SNP_NSECTION];
+    int snap_dims[SWPC_SNP_NSECTION][3];
+    float *d_snap_frame[SWPC_SNP_NSECTION];
+    float *d_snap_u[5];
+    float *d_snap_max[SWPC_SNP_NSECTION];
+    int *d_kfs;
+
     // graphs and device step counter
     int *d_it;
     hipGraphExec_t graph_exec;
@@ -170,6 +189,7 @@ hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it);
 hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it);
 hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it);
 hipError_t launch_step_advance(swpc_ctx *c);
+hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it);
 hipError_t launch_vmax(swpc_ctx *c);
 hipError_t launch_ref_to_dev(swpc_ctx *c, const float *src_ref, float *dst, hipStream_t s);
 hipError_t launch_dev_to_ref(swpc_ctx *c, const float *src, float *dst_ref, hipStream_t s);

@@ -31,7 +31,21 @@ FIELDS = {"Vx": 0, "Vy": 1, "Vz": 2, "Sxx": 3, "Syy": 4, "Szz": 5, "Syz": 6, "Sx
 WAV = {"vel": 1, "disp": 2, "stress": 4, "strain": 8}
 NCMP = {"vel": 3, "disp": 3, "stress": 6, "strain": 6}
 ABC = {"pml": 0, "cerjan": 1}
-TIMERS = ("wav", "stress_side", "stress", "stress_abc", "glut", "vel_side", "vel", "vel_abc", "bodyforce")
+TIMERS = ("wav", "snap", "stress_side", "stress", "stress_abc", "glut", "vel_side", "vel", "vel_abc", "bodyforce")
+SNAP_PLANES = ("xy", "yz", "xz", "fs", "ob")
+SNAP_QS = ("ps", "v", "u")
+
+
+def snap_section(name: str) -> int:
+    """section id of 'xz_ps', 'fs_v', ... (plane * 3 + quantity)"""
+    pl, q = name.split("_")
+    return SNAP_PLANES.index(pl) * 3 + SNAP_QS.index(q)
+
+
+class _SnapDesc(C.Structure):
+    _fields_ = [("sw", C.c_int * 15), ("k0_xy", C.c_int), ("i0_yz", C.c_int), ("j0_xz", C.c_int),
+                ("idec", C.c_int), ("jdec", C.c_int), ("kdec", C.c_int), ("ntdec_s", C.c_int),
+                ("nframe", C.c_int), ("M0", C.c_float), ("UC", C.c_float)]
 
 
 class SwpcError(RuntimeError):
@@ -75,6 +89,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         getattr(L, nm).argtypes = [vp, i]
     L.swpc_step.argtypes = [vp, i, i, i, i]
     L.swpc_step_timed.argtypes = [vp, i, i, fp]
+    L.swpc_set_snapshots.argtypes = [vp, C.POINTER(_SnapDesc), ip]
+    L.swpc_snap_write.argtypes = [vp, i]
+    L.swpc_get_snapshot.argtypes = [vp, i, i, fp]
+    L.swpc_get_snap_max.argtypes = [vp, i, fp]
+    L.swpc_snap_dims.argtypes = [vp, i, ip]
     L.swpc_kernel_vmax.argtypes = [vp, fp]
     L.swpc_get_field.argtypes = [vp, i, fp]
     L.swpc_set_field.argtypes = [vp, i, fp]
@@ -198,6 +217,40 @@ class SwpcGPU:
         ms = np.zeros(len(TIMERS), np.float32)
         self._chk(self.L.swpc_step_timed(self.h, int(it0), int(nsteps), ms))
         return dict(zip(TIMERS, (float(x) for x in ms)))
+
+    # ---- sections (snap__setup / snap__write) ----
+    def set_snapshots(self, sections, kfs, k0_xy=1, i0_yz=1, j0_xz=1, idec=1, jdec=1, kdec=1, ntdec_s=1, nframe=1,
+                      M0=1.0, UC=1.0):
+        """sections: names like 'xz_ps', 'fs_v', 'ob_u'; kfs: (ny+6, nx+6) free-surface map"""
+        d = _SnapDesc()
+        for name in sections:
+            d.sw[snap_section(name)] = 1
+        d.k0_xy, d.i0_yz, d.j0_xz = k0_xy, i0_yz, j0_xz
+        d.idec, d.jdec, d.kdec, d.ntdec_s, d.nframe = idec, jdec, kdec, ntdec_s, nframe
+        d.M0, d.UC = M0, UC
+        self._chk(self.L.swpc_set_snapshots(self.h, C.byref(d), _i32(kfs)))
+        self.snap_nframe = nframe
+
+    def snap_write(self, it):
+        self._chk(self.L.swpc_snap_write(self.h, int(it)))
+
+    def snap_dims(self, name):
+        dims = np.zeros(3, np.int32)
+        self._chk(self.L.swpc_snap_dims(self.h, snap_section(name), dims))
+        return tuple(int(x) for x in dims)
+
+    def snapshot(self, name, frame):
+        """frame of a section as numpy (ncomp, ns2, ns1) = Fortran (ns1, ns2, ncomp)"""
+        ns1, ns2, nc = self.snap_dims(name)
+        out = np.empty((nc, ns2, ns1), np.float32)
+        self._chk(self.L.swpc_get_snapshot(self.h, snap_section(name), int(frame), out))
+        return out
+
+    def snap_max(self, name):
+        ns1, ns2, _ = self.snap_dims(name)
+        out = np.empty((3, ns2, ns1), np.float32)
+        self._chk(self.L.swpc_get_snap_max(self.h, snap_section(name), out))
+        return out
 
     def synchronize(self):
         self._chk(self.L.swpc_synchronize(self.h))

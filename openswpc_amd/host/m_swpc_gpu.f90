@@ -13,7 +13,9 @@ module m_swpc_gpu
     integer(c_int), parameter :: SWPC_VX = 0, SWPC_VY = 1, SWPC_VZ = 2, SWPC_SXX = 3, SWPC_SYY = 4, &
                                  SWPC_SZZ = 5, SWPC_SYZ = 6, SWPC_SXZ = 7, SWPC_SXY = 8
     integer(c_int), parameter :: SWPC_WAV_V = 1, SWPC_WAV_U = 2, SWPC_WAV_STRESS = 4, SWPC_WAV_STRAIN = 8
-    integer(c_int), parameter :: SWPC_NTIMER = 9
+    integer(c_int), parameter :: SWPC_NTIMER = 10
+    integer(c_int), parameter :: SWPC_SNP_XY = 0, SWPC_SNP_YZ = 1, SWPC_SNP_XZ = 2, SWPC_SNP_FS = 3, SWPC_SNP_OB = 4
+    integer(c_int), parameter :: SWPC_SNP_PS = 0, SWPC_SNP_V = 1, SWPC_SNP_U = 2, SWPC_SNP_NSECTION = 15
 
     type, bind(C) :: swpc_desc
         integer(c_int) :: nx, ny, nz, na, nm, abc
@@ -21,6 +23,15 @@ module m_swpc_gpu
         real(c_float) :: dt
         integer(c_int) :: device
     end type swpc_desc
+
+    !! section id = plane * 3 + quantity
+    type, bind(C) :: swpc_snap_desc
+        integer(c_int) :: sw(SWPC_SNP_NSECTION)
+        integer(c_int) :: k0_xy, i0_yz, j0_xz
+        integer(c_int) :: idec, jdec, kdec
+        integer(c_int) :: ntdec_s, nframe
+        real(c_float) :: M0, UC
+    end type swpc_snap_desc
 
     interface
         integer(c_int) function swpc_create(desc, ctx) bind(C, name='swpc_create')
@@ -125,6 +136,35 @@ module m_swpc_gpu
             type(c_ptr), value :: ctx
             integer(c_int), value :: it0, nsteps
             real(c_float), intent(out) :: ms(*)
+        end function
+        integer(c_int) function swpc_set_snapshots(ctx, desc, kfs) bind(C, name='swpc_set_snapshots')
+            import :: c_int, c_ptr, swpc_snap_desc
+            type(c_ptr), value :: ctx
+            type(swpc_snap_desc), intent(in) :: desc
+            integer(c_int), intent(in) :: kfs(*)
+        end function
+        integer(c_int) function swpc_snap_write(ctx, it) bind(C, name='swpc_snap_write')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+        integer(c_int) function swpc_get_snapshot(ctx, section, frame, out) bind(C, name='swpc_get_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: section, frame
+            real(c_float), intent(out) :: out(*)
+        end function
+        integer(c_int) function swpc_get_snap_max(ctx, section, out) bind(C, name='swpc_get_snap_max')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: section
+            real(c_float), intent(out) :: out(*)
+        end function
+        integer(c_int) function swpc_snap_dims(ctx, section, dims) bind(C, name='swpc_snap_dims')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: section
+            integer(c_int), intent(out) :: dims(3)
         end function
         integer(c_int) function swpc_kernel_vmax(ctx, out) bind(C, name='swpc_kernel_vmax')
             import :: c_int, c_ptr, c_float

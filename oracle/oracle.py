@@ -89,6 +89,7 @@ def lib(prec: int = 32) -> C.CDLL:
     L.orc_source_bodyforce.argtypes = [vp, i, _i32p, _i32p, _i32p, _f32p, _f32p, _f32p, f, _f32p]
     L.orc_wav_store.argtypes = [vp, i, i, _i32p, _i32p, _i32p, i, i, i, f, f, vp, vp, vp, vp, vp, vp]
     L.orc_kernel_vmax.argtypes = [vp, _f32p]
+    L.orc_snap_write.argtypes = [vp, i, _i32p, _i32p, i, i, i, i, i, i, i, f, f, vp, vp, vp, vp]
     L.orc_prec.restype = i
     _LIBS[prec] = L
     return L
@@ -454,6 +455,7 @@ class OracleRun:
         self.it += 1
         it = self.it
         self.wav_store(it)
+        self.snap_write(it)
         L.orc_kernel_update_stress(p)
         L.orc_absorb_update_stress(p)
         if m.nsrc and not m.cfg.bf_mode:
@@ -475,6 +477,50 @@ class OracleRun:
         self.L.orc_wav_store(self.ptr, it, m.nst, m.ist, m.jst, m.kst, m.cfg.wav_flags, m.cfg.ntdec_w, m.ntw,
                              m.M0, m.UC, vp(self.uacc), vp(self.eacc), vp(self.wav["vel"]), vp(self.wav["disp"]),
                              vp(self.wav["stress"]), vp(self.wav["strain"]))
+
+    # ---- snapshots / sections (m_snap.f90) ----
+    SNAP_PLANES = ("xy", "yz", "xz", "fs", "ob")
+    SNAP_QS = ("ps", "v", "u")
+
+    def snap_setup(self, sections, k0_xy=1, i0_yz=1, j0_xz=1, idec=1, jdec=1, kdec=1, ntdec_s=1, nframe=1):
+        """sections: names like 'xz_ps', 'fs_v', 'ob_u' (m_snap.f90:105-119)"""
+        cfg = self.m.cfg
+        nxs, nys, nzs = ((n + d // 2) // d for n, d in ((cfg.nx, idec), (cfg.ny, jdec), (cfg.nz, kdec)))
+        dims = {"xy": (nxs, nys), "yz": (nys, nzs), "xz": (nxs, nzs), "fs": (nxs, nys), "ob": (nxs, nys)}
+        self.snap_sw = np.zeros(15, np.int32)
+        self.snap_frames, self.snap_u, self.snap_vmax, self.snap_umax = {}, {}, {}, {}
+        sp = self.sp
+        for name in sections:
+            pl, q = name.split("_")
+            sec = self.SNAP_PLANES.index(pl) * 3 + self.SNAP_QS.index(q)
+            self.snap_sw[sec] = 1
+            ns1, ns2 = dims[pl]
+            self.snap_frames[name] = np.zeros((nframe, 4 if q == "ps" else 3, ns2, ns1), sp)
+        for pl in self.SNAP_PLANES:
+            ns1, ns2 = dims[pl]
+            self.snap_u[pl] = np.zeros((3, ns2, ns1), sp)
+            self.snap_vmax[pl] = np.zeros((3, ns2, ns1), sp)
+            self.snap_umax[pl] = np.zeros((3, ns2, ns1), sp)
+        self.snap_par = (k0_xy, i0_yz, j0_xz, idec, jdec, kdec, ntdec_s)
+        dummy = np.zeros(1, sp)
+        self._snap_keep = [dummy]
+        ptrs = []
+        for sec in range(15):
+            name = f"{self.SNAP_PLANES[sec // 3]}_{self.SNAP_QS[sec % 3]}"
+            ptrs.append(self.snap_frames[name].ctypes.data if name in self.snap_frames else dummy.ctypes.data)
+        self._snap_ptrs = (C.c_void_p * 15)(*ptrs)
+        self._snap_u = (C.c_void_p * 5)(*[self.snap_u[p].ctypes.data for p in self.SNAP_PLANES])
+        self._snap_vm = (C.c_void_p * 5)(*[self.snap_vmax[p].ctypes.data for p in self.SNAP_PLANES])
+        self._snap_um = (C.c_void_p * 5)(*[self.snap_umax[p].ctypes.data for p in self.SNAP_PLANES])
+
+    def snap_write(self, it):
+        if not hasattr(self, "snap_sw"):
+            return
+        m = self.m
+        k0, i0, j0, idec, jdec, kdec, nts = self.snap_par
+        self.L.orc_snap_write(self.ptr, it, m.kfs, self.snap_sw, k0, i0, j0, idec, jdec, kdec, nts, m.M0, m.UC,
+                              C.cast(self._snap_ptrs, C.c_void_p), C.cast(self._snap_u, C.c_void_p),
+                              C.cast(self._snap_vm, C.c_void_p), C.cast(self._snap_um, C.c_void_p))
 
     def run(self, nsteps: int):
         for _ in range(nsteps):

@@ -628,3 +628,121 @@ int64_t orc_n3(const orc_state *s)
 int orc_prec(void) { return ORC_PREC; }
 int orc_sizeof_mp(void) { return (int)sizeof(mp_t); }
 int orc_sizeof_sp(void) { return (int)sizeof(sp_t); }
+
+/*
+ * snap__write, src/swpc_3d/m_snap.f90:919-947, with the section buffers of
+ * wbuf_{xy,yz,xz,fs,ob}_{ps,v,u} (:981-2189), native-format semantics: a
+ * frame is taken when mod(it-1, ntdec_s) == 0.
+ *
+ * Section s = plane*3 + q, plane 0..4 = xy yz xz fs ob, q 0..2 = ps v u;
+ * sw[s] != 0 enables it.  Sizes: nxs = (nx + idec/2)/idec (likewise nys,
+ * nzs, :130-132); xy/fs/ob sections are (nxs, nys), yz (nys, nzs), xz
+ * (nxs, nzs), section point ii -> grid i = ii*idec - idec/2 (:137).
+ * frame[s] holds (ns1, ns2, ncomp, nframe) in Fortran order (ncomp 4 for
+ * ps, 3 for v and u).  uacc[plane] (ns1, ns2, 3) accumulates V*dt every call
+ * for u sections; vmax[plane], umax[plane] (nxs, nys, 3) are the running
+ * maxima of the fs and ob v/u sections (:1697-1707 and likewise).  All
+ * persistent buffers start at zero.  kfs is the (i,j) free-surface map.
+ */
+static inline int snap_n(int n, int dec) { return (n + dec / 2) / dec; }
+
+void orc_snap_write(orc_state *s, int it, const int32_t *kfs, const int32_t *sw, int k0_xy, int i0_yz, int j0_xz,
+                    int idec, int jdec, int kdec, int ntdec_s, float M0, float UC, sp_t **frame, sp_t **uacc,
+                    sp_t **vmax, sp_t **umax)
+{
+    const int nxs = snap_n(s->nx, idec), nys = snap_n(s->ny, jdec), nzs = snap_n(s->nz, kdec);
+    const mp_t r20x = (mp_t)1.0 / (mp_t)s->dx, r20y = (mp_t)1.0 / (mp_t)s->dy, r20z = (mp_t)1.0 / (mp_t)s->dz;
+    const sp_t epsl = (sp_t)1.1920928955078125e-07f;
+    const sp_t uc = (sp_t)UC, m0 = (sp_t)M0, dt = (sp_t)s->dt;
+    const int out = ((it - 1) % ntdec_s) == 0;
+    const int f = (it - 1) / ntdec_s;
+    for (int plane = 0; plane < 5; plane++) {
+        int ns1, ns2;
+        if (plane == 1) { ns1 = nys; ns2 = nzs; }
+        else if (plane == 2) { ns1 = nxs; ns2 = nzs; }
+        else { ns1 = nxs; ns2 = nys; }
+        const int64_t npt = (int64_t)ns1 * ns2;
+        for (int q = 0; q < 3; q++) {
+            const int sec = plane * 3 + q;
+            if (!sw[sec]) continue;
+            /* ps and v of xy/yz/xz are evaluated only on output steps; fs/ob
+               v every step (max maps); u every step */
+            const int every = (q == 2) || (q == 1 && plane >= 3);
+            if (!out && !every) continue;
+            const int ncomp = q == 0 ? 4 : 3;
+            sp_t *fr = out ? frame[sec] + (int64_t)f * npt * ncomp : NULL;
+            for (int b = 1; b <= ns2; b++)
+                for (int a = 1; a <= ns1; a++) {
+                    int i, j, k;
+                    switch (plane) {
+                    case 0: i = a * idec - idec / 2; j = b * jdec - jdec / 2; k = k0_xy; break;
+                    case 1: i = i0_yz; j = a * jdec - jdec / 2; k = b * kdec - kdec / 2; break;
+                    case 2: i = a * idec - idec / 2; j = j0_xz; k = b * kdec - kdec / 2; break;
+                    case 3: i = a * idec - idec / 2; j = b * jdec - jdec / 2; k = M2(kfs, i, j) + 1; break;
+                    default: i = a * idec - idec / 2; j = b * jdec - jdec / 2; k = M2(s->kob, i, j) + 1; break;
+                    }
+                    const int64_t p = (int64_t)(b - 1) * ns1 + (a - 1);
+                    if (q == 0) {
+                        /* m_snap.f90:1016-1036 */
+                        sp_t div = (dr_t)((V3(s->Vx, k, i, j) - V3(s->Vx, k, i - 1, j)) * r20x
+                                        + (V3(s->Vy, k, i, j) - V3(s->Vy, k, i, j - 1)) * r20y
+                                        + (V3(s->Vz, k, i, j) - V3(s->Vz, k - 1, i, j)) * r20z);
+                        sp_t rot_x = (dr_t)((V3(s->Vz, k, i, j + 1) - V3(s->Vz, k, i, j)) * r20y
+                                          - (V3(s->Vy, k + 1, i, j) - V3(s->Vy, k, i, j)) * r20z);
+                        sp_t rot_y = (dr_t)((V3(s->Vx, k + 1, i, j) - V3(s->Vx, k, i, j)) * r20z
+                                          - (V3(s->Vz, k, i + 1, j) - V3(s->Vz, k, i, j)) * r20x);
+                        sp_t rot_z = (dr_t)((V3(s->Vy, k, i + 1, j) - V3(s->Vy, k, i, j)) * r20x
+                                          - (V3(s->Vx, k, i, j + 1) - V3(s->Vx, k, i, j)) * r20y);
+                        const sp_t lam = V3(s->lam, k, i, j);
+                        div = div * lam / (sp_t)fabs((double)(lam + epsl));
+                        const mp_t syz = V3(s->Syz, k, i, j), sxz = V3(s->Sxz, k, i, j), sxy = V3(s->Sxy, k, i, j);
+                        rot_x = (sp_t)((mp_t)rot_x * (mp_t)fabs((double)syz) / ((mp_t)fabs((double)syz) + (mp_t)epsl));
+                        rot_y = (sp_t)((mp_t)rot_y * (mp_t)fabs((double)sxz) / ((mp_t)fabs((double)sxz) + (mp_t)epsl));
+                        rot_z = (sp_t)((mp_t)rot_z * (mp_t)fabs((double)sxy) / ((mp_t)fabs((double)sxy) + (mp_t)epsl));
+                        fr[p + 0 * npt] = div * uc * m0 * (sp_t)1e-3f;
+                        fr[p + 1 * npt] = rot_x * uc * m0 * (sp_t)1e-3f;
+                        fr[p + 2 * npt] = rot_y * uc * m0 * (sp_t)1e-3f;
+                        fr[p + 3 * npt] = rot_z * uc * m0 * (sp_t)1e-3f;
+                    } else if (q == 1) {
+                        /* m_snap.f90:1678-1683 */
+                        const sp_t b1 = (sp_t)(V3(s->Vx, k, i, j) * (mp_t)uc * (mp_t)m0);
+                        const sp_t b2 = (sp_t)(V3(s->Vy, k, i, j) * (mp_t)uc * (mp_t)m0);
+                        const sp_t b3 = (sp_t)(V3(s->Vz, k, i, j) * (mp_t)uc * (mp_t)m0);
+                        if (plane >= 3) {
+                            /* m_snap.f90:1697-1707 */
+                            sp_t *mx = vmax[plane];
+                            const sp_t h = (sp_t)sqrt((double)(b1 * b1 + b2 * b2));
+                            const sp_t az = (sp_t)fabs((double)b3);
+                            mx[p] = mx[p] > az ? mx[p] : az;
+                            mx[p + npt] = mx[p + npt] > h ? mx[p + npt] : h;
+                            mx[p + 2 * npt] = (sp_t)sqrt((double)(mx[p] * mx[p] + mx[p + npt] * mx[p + npt]));
+                        }
+                        if (out) {
+                            fr[p] = b1;
+                            fr[p + npt] = b2;
+                            fr[p + 2 * npt] = b3;
+                        }
+                    } else {
+                        /* m_snap.f90:1857-1859 */
+                        sp_t *u = uacc[plane];
+                        u[p] = (sp_t)((mp_t)u[p] + V3(s->Vx, k, i, j) * (mp_t)uc * (mp_t)m0 * (mp_t)dt);
+                        u[p + npt] = (sp_t)((mp_t)u[p + npt] + V3(s->Vy, k, i, j) * (mp_t)uc * (mp_t)m0 * (mp_t)dt);
+                        u[p + 2 * npt] = (sp_t)((mp_t)u[p + 2 * npt] + V3(s->Vz, k, i, j) * (mp_t)uc * (mp_t)m0 * (mp_t)dt);
+                        if (plane >= 3) {
+                            sp_t *mx = umax[plane];
+                            const sp_t h = (sp_t)sqrt((double)(u[p] * u[p] + u[p + npt] * u[p + npt]));
+                            const sp_t az = (sp_t)fabs((double)u[p + 2 * npt]);
+                            mx[p] = mx[p] > az ? mx[p] : az;
+                            mx[p + npt] = mx[p + npt] > h ? mx[p + npt] : h;
+                            mx[p + 2 * npt] = (sp_t)sqrt((double)(mx[p] * mx[p] + mx[p + npt] * mx[p + npt]));
+                        }
+                        if (out) {
+                            fr[p] = u[p];
+                            fr[p + npt] = u[p + npt];
+                            fr[p + 2 * npt] = u[p + 2 * npt];
+                        }
+                    }
+                }
+        }
+    }
+}

@@ -1,0 +1,54 @@
+"""Sections (snap__write) on the GPU against the FP32 oracle, bit for bit:
+all 15 sections, decimation, graph-replayed steps, maximum maps."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from gpu_helpers import gpu_from_model, small_config, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+ALL = [f"{p}_{q}" for p in ("xy", "yz", "xz", "fs", "ob") for q in ("ps", "v", "u")]
+
+
+@pytest.mark.parametrize("dec,graph", [((1, 1, 1), False), ((2, 3, 2), True), ((3, 2, 1), True)])
+def test_sections_match_oracle(orc, dec, graph):
+    nt, ntdec_s = 20, 3
+    nframe = (nt - 1) // ntdec_s + 1
+    cfg = small_config(nt=nt)
+    m = O.build_model(cfg)
+    par = dict(k0_xy=9, i0_yz=17, j0_xz=15, idec=dec[0], jdec=dec[1], kdec=dec[2], ntdec_s=ntdec_s, nframe=nframe)
+    run = O.OracleRun(m, 32)
+    run.snap_setup(ALL, **par)
+    run.run(nt)
+    g = gpu_from_model(m)
+    g.set_snapshots(ALL, m.kfs, M0=m.M0, UC=m.UC, **par)
+    if graph:
+        g.step(1, nt, use_graph=True, block=5)
+    else:
+        g.step(1, nt)
+    g.synchronize()
+    nonzero = 0
+    for name in ALL:
+        for f in range(nframe):
+            got = g.snapshot(name, f)
+            want = run.snap_frames[name][f]
+            assert got.shape == want.shape, name
+            assert ulp_diff(got, want) == 0, (name, f)
+            nonzero += int(np.any(want != 0))
+    for pl in ("fs", "ob"):
+        assert ulp_diff(g.snap_max(f"{pl}_v"), run.snap_vmax[pl]) == 0, pl
+        assert ulp_diff(g.snap_max(f"{pl}_u"), run.snap_umax[pl]) == 0, pl
+    assert np.any(run.snap_vmax["fs"] != 0)
+    assert nonzero > 0
+
+
+def test_frame_store_bound_is_checked(orc):
+    cfg = small_config(nt=10)
+    m = O.build_model(cfg)
+    g = gpu_from_model(m)
+    g.set_snapshots(["xy_v"], m.kfs, k0_xy=5, ntdec_s=2, nframe=2, M0=m.M0, UC=m.UC)
+    g.step(1, 4)        # frames 0, 1
+    from openswpc_amd.gpu import SwpcError
+    with pytest.raises(SwpcError):
+        g.step(5, 1)    # would take frame 2
