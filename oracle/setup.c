@@ -11,13 +11,16 @@
  *   STFs), orc_sdr2moment, orc_x2i/orc_i2x, orc_fdm_stable_dt
  *       - pinned bit-for-bit against src/shared/m_fdtool.f90 compiled from the
  *         reference sources (oracle/ref_fdtool) -> tests/golden/fdtool_*.
- *   orc_kernel_coef (c1,c2,d1,d2)  - closed form of the tau-method recursion
- *         (trapezoidal discretisation of the Zener ODE), tested in double.
- *   orc_damping_profile            - closed form: the ADE recursion equals the
- *         Crank-Nicolson step of the CFS-PML ODE; R0 formula by hand.
- *   orc_relaxed_medium             - phase velocity at fq_ref recomputed in
- *         complex128 equals the input velocity; complex rounding of the
- *         Fortran runtime itself is parity unpinned (1-ulp level).
+ *   orc_kernel_coef (c1,c2,d1,d2)  - c1 is the trapezoidal step of
+ *         exp(-dt/ts) and the recursion settles at -F/NM (test_oracle_setup);
+ *         d1, d2 end to end through the measured Q(f) (test_oracle_physics).
+ *   orc_damping_profile            - the ADE pair's steady state equals the
+ *         w = 0 limit of the CFS stretching, R0 = 1e-4 at na = 20
+ *         (test_oracle_setup); PML reflection (test_oracle_physics).
+ *   orc_relaxed_medium             - the example.out set-up summary (vmin
+ *         3.122 relaxed from 3.14, vmax 7.977 from 8.0, c = 0.645) and the
+ *         measured Q(f); complex rounding of the Fortran runtime itself is
+ *         parity unpinned (1-ulp level).
  *   orc_surface_detection, orc_kbeg_a, orc_homogenize_absorber,
  *   orc_vmodel_lhm, orc_vmodel_lgm, orc_cerjan_profile
  *       - index/branch logic pinned by hand-built cases in tests/test_oracle_setup.py;

@@ -7,19 +7,28 @@
  * with the same kind promotions), so that the FP32 variant rounds exactly as
  * the reference FP32 build compiled without FMA contraction.
  *
- * Parity status (how each function is pinned; the tests name the checks):
- *   orc_kernel_update_vel / _stress  - example/example.out max-|V| trace (mixed
- *       variant, 384^3 run), FP64-vs-FP32 convergence, closed-form P/S travel
- *       times in a homogeneous full space, zero-field and linearity invariants,
- *       isotropic-source symmetry.
- *   orc_absorb_p_update_*            - same end-to-end pins + PML reflection
- *       (small domain vs large domain) and energy decay.
- *   orc_absorb_c_update_*            - energy decay and reflection; the taper
- *       multiply order is pinned only by reading (parity unpinned beyond that).
- *   orc_source_stressglut/bodyforce  - one-step injection delta against the
- *       closed form, reciprocity of force/strain pairs.
- *   orc_wav_store                    - gather of a known synthetic field.
- *   orc_kernel_vmax                  - example/example.out trace.
+ * Parity status (how each function is pinned; the tests named):
+ *   orc_kernel_update_vel / _stress, orc_absorb_p_update_*, orc_kernel_vmax,
+ *   orc_source_stressglut (normal components)
+ *       - the reference's own example/example.out: all 20 printed max-|V| rows
+ *         of the 384^3 default example reproduced by the mixed variant
+ *         (tests/test_oracle_example.py);
+ *       - closed-form full-space solutions of an explosion and of a point
+ *         force, x <-> y mirror symmetry with free surface, PML and memory
+ *         variables, superposition, zero field, attenuation against the
+ *         tau-method Q(f), PML reflection against a larger domain with a
+ *         no-damping control (tests/test_oracle_physics.py).
+ *   orc_source_bodyforce - the point-force closed form above.
+ *   orc_source_stressglut, shear components (the /4 split onto the four
+ *       staggered neighbours) - mirror symmetry pins their operand pairing;
+ *       their amplitude is parity unpinned beyond reading.
+ *   orc_absorb_c_update_* - the taper profile by its closed form
+ *       (tests/test_oracle_setup.py); the multiply order of its application
+ *       is parity unpinned beyond reading.
+ *   orc_wav_store - parity unpinned beyond reading (checked against the GPU
+ *       only); the same sampling conventions are pinned for orc_snap_write.
+ *   orc_snap_write - linear-field closed forms, index sampling and
+ *       accumulation checks (tests/test_oracle_snap.py).
  */
 #include <math.h>
 #include <stdlib.h>
