@@ -215,6 +215,31 @@ int swpc_set_field(swpc_ctx *ctx, int field, const float *in);
    (src/swpc_3d/m_wav.f90:37-38).  kind = one SWPC_WAV_* flag. */
 int swpc_get_seismograms(swpc_ctx *ctx, int kind, float *out);
 
+/* ---- batches: many independent simulations stepped together ----
+   The reference runs one simulation per process; a batch of Green's
+   functions (one run per source component and station, the workload of
+   src/swpc_3d/m_green.f90) there means many separate runs.  A batch steps
+   member contexts (created and set up as usual, same device and same nm;
+   grids, absorbers, sources and receivers may differ) with one launch per
+   phase for all members.  Members keep their state and outputs: read them
+   with swpc_get_seismograms etc.  Do not step a member on its own while its
+   batch steps (the batch uses its own stream); tuning changes made to
+   members are picked up by the next swpc_batch_step. */
+typedef struct swpc_batch swpc_batch;
+int swpc_batch_create(swpc_ctx *const *members, int n, swpc_batch **out);
+int swpc_batch_destroy(swpc_batch *batch);
+const char *swpc_batch_last_error(const swpc_batch *batch);
+/* nsteps whole steps it0 .. it0+nsteps-1 of every member (main.f90:119-139
+   order), as replays of a captured graph of `block` steps when use_graph. */
+int swpc_batch_step(swpc_batch *batch, int it0, int nsteps, int use_graph, int block);
+int swpc_batch_synchronize(swpc_batch *batch);
+/* Autotuner: times candidate k-chunk depths and tile rows on trial_steps
+   dry steps (step index 0: no source, no recording) and keeps the fastest.
+   Call it after set-up and before the first step, while every field is zero,
+   so the trials leave the state untouched.  Outputs may be NULL. */
+int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, float *ms_per_step);
+void *swpc_batch_stream(swpc_batch *batch);
+
 /* Wait for all work queued on the context. */
 int swpc_synchronize(swpc_ctx *ctx);
 

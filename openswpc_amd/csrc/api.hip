@@ -46,6 +46,7 @@ static void dfree(void *p)
 
 static void reset_graph(swpc_ctx *c)
 {
+    ++c->gen;  // launch arguments may change: captured graphs (own or a batch's) are stale
     if (c->graph_exec) {
         (void)hipGraphExecDestroy(c->graph_exec);
         c->graph_exec = nullptr;

@@ -86,21 +86,21 @@ static void srcargs_fill(swpc_ctx *c, SrcArgs &a, int it, const int *d_it)
     a.d_it = d_it;
 }
 
-hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it)
+hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
 {
     if (c->nsrc <= 0 || c->bf_mode || c->ntarget <= 0) return hipSuccess;
     SrcArgs a;
     srcargs_fill(c, a, it, d_it);
-    hipLaunchKernelGGL(k_stressglut, dim3((c->ntarget + 63) / 64), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(k_stressglut, dim3((c->ntarget + 63) / 64), dim3(64), 0, (s ? s : c->stream), a);
     return hipGetLastError();
 }
 
-hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it)
+hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
 {
     if (c->nsrc <= 0 || !c->bf_mode || c->ntarget <= 0) return hipSuccess;
     SrcArgs a;
     srcargs_fill(c, a, it, d_it);
-    hipLaunchKernelGGL(k_bodyforce, dim3((c->ntarget + 63) / 64), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(k_bodyforce, dim3((c->ntarget + 63) / 64), dim3(64), 0, (s ? s : c->stream), a);
     return hipGetLastError();
 }
 
@@ -202,7 +202,7 @@ __global__ void k_wav_store(WavArgs a)
     }
 }
 
-hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it)
+hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
 {
     if (c->nst <= 0) return hipSuccess;
     WavArgs a;
@@ -220,7 +220,7 @@ hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it)
     a.r41x = 1.0f / 24.0f / dx; a.r41y = 1.0f / 24.0f / dy; a.r41z = 1.0f / 24.0f / dz;
     a.it = it;
     a.d_it = d_it;
-    hipLaunchKernelGGL(k_wav_store, dim3((c->nst + 63) / 64), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(k_wav_store, dim3((c->nst + 63) / 64), dim3(64), 0, (s ? s : c->stream), a);
     return hipGetLastError();
 }
 
@@ -235,6 +235,23 @@ __global__ void k_step_advance(int *d_it)
 hipError_t launch_step_advance(swpc_ctx *c)
 {
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c->stream, c->d_it);
+    return hipGetLastError();
+}
+
+hipError_t launch_counter_advance(int *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, d);
+    return hipGetLastError();
+}
+
+__global__ void k_set_counter(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+hipError_t launch_set_counter(int *d, int v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, s, d, v);
     return hipGetLastError();
 }
 

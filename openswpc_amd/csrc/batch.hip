@@ -1,0 +1,334 @@
+// batch.hip -- many independent simulations stepped together (swpc_batch).
+//
+// A batch references member contexts that were created and set up as usual
+// (swpc_create, swpc_set_*).  Each phase of a step is one launch for all
+// members: the launch walks a tile table whose entries carry the member
+// index, and each member's launch arguments sit in HBM.  Uneven grids pack
+// into one grid of blocks with no per-member launch tail.  Sources,
+// receivers and sections stay per-member launches (a few threads each).
+// Steps replay from a HIP graph of `block` steps driven by one device step
+// counter shared by all members.
+//
+// Tile packing: member-major (each member's tiles contiguous, so a member's
+// neighbouring tiles share an XCD's L2), members ordered by size, largest
+// first, so the smallest members fill the end of the launch.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "swpc_internal.h"
+
+struct swpc_batch {
+    std::vector<swpc_ctx *> mem;
+    std::vector<unsigned> gen;     // member config generations the tables were built for
+    int device;
+    hipStream_t stream;
+    int nm, by;
+    bool nt;
+    void *d_args[2];               // StepArgs of every member: stress, velocity
+    int4 *d_cols, *d_side;
+    int ncols, nside;
+    int *d_it;
+    hipGraphExec_t graph;
+    int graph_block;
+    char err[512];
+};
+
+static int bfail(swpc_batch *b, int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(b->err, sizeof(b->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define BCHK(b, call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return bfail(b, SWPC_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static void bfree(void *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+static void breset_graph(swpc_batch *b)
+{
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graph_block = 0;
+}
+
+static void free_tables(swpc_batch *b)
+{
+    bfree(b->d_args[0]);
+    bfree(b->d_args[1]);
+    void *p = b->d_cols;
+    bfree(p);
+    p = b->d_side;
+    bfree(p);
+    b->d_cols = b->d_side = nullptr;
+    b->ncols = b->nside = 0;
+}
+
+// (re)build the argument arrays and tile tables from the members' current state
+static int build_tables(swpc_batch *b)
+{
+    breset_graph(b);
+    free_tables(b);
+    const int n = (int)b->mem.size();
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const Grid &gx = b->mem[x]->g, &gy = b->mem[y]->g;
+        return (long long)gx.nx * gx.ny * gx.nz > (long long)gy.nx * gy.ny * gy.nz;
+    });
+    const size_t asz = step_args_size();
+    std::vector<char> args[2] = {std::vector<char>(asz * n), std::vector<char>(asz * n)};
+    std::vector<int4> cols, side;
+    for (int m : order) {
+        swpc_ctx *c = b->mem[m];
+        if (c->by != b->by) return bfail(b, SWPC_ESTATE, "member %d has tile rows %d, batch %d", m, c->by, b->by);
+        fill_batch_args(c, args[0].data() + asz * m, 0);
+        fill_batch_args(c, args[1].data() + asz * m, 1);
+        int nbx, nby, nbz;
+        member_tiles(c, nbx, nby, nbz);
+        if (c->g.nxk > 0 && c->g.nyk > 0)
+            for (int bz = 0; bz < nbz; ++bz)
+                for (int by = 0; by < nby; ++by)
+                    for (int bx = 0; bx < nbx; ++bx) cols.push_back(make_int4(m, bx, by, bz));
+        if (c->desc.abc == SWPC_ABC_PML && c->npml_tiles > 0) {
+            std::vector<int4> t(c->npml_tiles);
+            BCHK(b, hipMemcpy(t.data(), c->d_pml_tiles, t.size() * sizeof(int4), hipMemcpyDeviceToHost));
+            for (const int4 &x : t) {
+                if (x.z > 0xFFFF || x.w > 0x7FFF) return bfail(b, SWPC_EINVAL, "member %d: nz too large", m);
+                side.push_back(make_int4(x.x, x.y, x.z | (x.w << 16), m));
+            }
+        }
+    }
+    for (int ph = 0; ph < 2; ++ph) {
+        BCHK(b, hipMalloc(&b->d_args[ph], asz * n));
+        BCHK(b, hipMemcpy(b->d_args[ph], args[ph].data(), asz * n, hipMemcpyHostToDevice));
+    }
+    if (!cols.empty()) {
+        BCHK(b, hipMalloc((void **)&b->d_cols, cols.size() * sizeof(int4)));
+        BCHK(b, hipMemcpy(b->d_cols, cols.data(), cols.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    if (!side.empty()) {
+        BCHK(b, hipMalloc((void **)&b->d_side, side.size() * sizeof(int4)));
+        BCHK(b, hipMemcpy(b->d_side, side.data(), side.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    b->ncols = (int)cols.size();
+    b->nside = (int)side.size();
+    for (int m = 0; m < n; ++m) b->gen[m] = b->mem[m]->gen;
+    return SWPC_OK;
+}
+
+static bool tables_stale(const swpc_batch *b)
+{
+    for (size_t m = 0; m < b->mem.size(); ++m)
+        if (b->mem[m]->gen != b->gen[m]) return true;
+    return false;
+}
+
+// one step of every member, in the order of src/swpc_3d/main.f90:122-138
+static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
+{
+    hipError_t e;
+    hipStream_t s = b->stream;
+    for (swpc_ctx *c : b->mem) {
+        if ((e = launch_wav_store(c, it, d_it, s)) != hipSuccess) return e;
+        if ((e = launch_snap(c, it, d_it, s)) != hipSuccess) return e;
+    }
+    if ((e = launch_batch_phase(0, b->nm, b->nt, b->by, b->d_args[0], b->d_cols, b->ncols, b->d_side, b->nside, s)) !=
+        hipSuccess)
+        return e;
+    for (swpc_ctx *c : b->mem) {
+        if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_stress(c, s)) != hipSuccess) return e;
+        if ((e = launch_stressglut(c, it, d_it, s)) != hipSuccess) return e;
+    }
+    if ((e = launch_batch_phase(1, b->nm, b->nt, b->by, b->d_args[1], b->d_cols, b->ncols, b->d_side, b->nside, s)) !=
+        hipSuccess)
+        return e;
+    for (swpc_ctx *c : b->mem) {
+        if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_vel(c, s)) != hipSuccess) return e;
+        if ((e = launch_bodyforce(c, it, d_it, s)) != hipSuccess) return e;
+    }
+    if (d_it) e = launch_counter_advance(b->d_it, s);
+    return e;
+}
+
+extern "C" const char *swpc_batch_last_error(const swpc_batch *b) { return b ? b->err : "null batch"; }
+
+extern "C" void *swpc_batch_stream(swpc_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+extern "C" int swpc_batch_destroy(swpc_batch *b)
+{
+    if (!b) return SWPC_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    breset_graph(b);
+    free_tables(b);
+    void *p = b->d_it;
+    bfree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SWPC_OK;
+}
+
+extern "C" int swpc_batch_create(swpc_ctx *const *members, int n, swpc_batch **out)
+{
+    if (!members || n < 1 || !out) return SWPC_EINVAL;
+    *out = nullptr;
+    for (int m = 0; m < n; ++m) {
+        swpc_ctx *c = members[m];
+        if (!c) return SWPC_EINVAL;
+        if (c->desc.device != members[0]->desc.device || c->desc.nm != members[0]->desc.nm) return SWPC_EINVAL;
+        if (!c->has_medium || (c->desc.nm > 0 && !c->has_visco) ||
+            (c->desc.abc == SWPC_ABC_PML && !c->has_pml) || (c->desc.abc == SWPC_ABC_CERJAN && !c->has_cerjan))
+            return SWPC_ESTATE;
+        for (int q = 0; q < m; ++q)
+            if (members[q] == c) return SWPC_EINVAL;
+    }
+    swpc_batch *b = new swpc_batch();
+    b->mem.assign(members, members + n);
+    b->gen.assign(n, 0);
+    b->device = members[0]->desc.device;
+    b->nm = members[0]->desc.nm;
+    b->by = members[0]->by;
+    b->nt = (members[0]->variant & 1) != 0;
+    int rc = SWPC_OK;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&b->d_it, sizeof(int)) != hipSuccess) {
+        swpc_batch_destroy(b);
+        return SWPC_EHIP;
+    }
+    // members share the batch's tile rows and variant
+    for (swpc_ctx *c : b->mem)
+        if (c->by != b->by || c->variant != members[0]->variant) {
+            if ((rc = swpc_set_tuning(c, 0, b->by, members[0]->variant)) != SWPC_OK) {
+                swpc_batch_destroy(b);
+                return rc;
+            }
+        }
+    if ((rc = build_tables(b)) != SWPC_OK) {
+        swpc_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return SWPC_OK;
+}
+
+static int batch_sync_members(swpc_batch *b)
+{
+    // members may have work queued on their own streams (uploads, own steps)
+    for (swpc_ctx *c : b->mem) BCHK(b, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_batch_step(swpc_batch *b, int it0, int nsteps, int use_graph, int block)
+{
+    if (!b) return SWPC_EINVAL;
+    if (nsteps < 0) return bfail(b, SWPC_EINVAL, "swpc_batch_step: negative step count");
+    BCHK(b, hipSetDevice(b->device));
+    int rc;
+    if ((rc = batch_sync_members(b))) return rc;
+    if (tables_stale(b) && (rc = build_tables(b))) return rc;
+    for (swpc_ctx *c : b->mem)
+        if (c->snap_ntask > 0 && nsteps > 0 && it0 + nsteps - 1 >= 1 &&
+            (it0 + nsteps - 2) / c->snap.ntdec_s >= c->snap.nframe)
+            return bfail(b, SWPC_EINVAL, "a member's snapshot frame store is too small for these steps");
+    int done = 0;
+    if (use_graph && block > 0 && nsteps >= block) {
+        if (!b->graph || b->graph_block != block) {
+            breset_graph(b);
+            hipGraph_t graph;
+            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int s = 0; s < block && e == hipSuccess; ++s) e = enqueue_batch_step(b, 0, b->d_it);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+            if (e != hipSuccess) return bfail(b, SWPC_EHIP, "batch graph capture: %s", hipGetErrorString(e));
+            if (e2 != hipSuccess) return bfail(b, SWPC_EHIP, "batch graph capture end: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                b->graph = nullptr;
+                return bfail(b, SWPC_EHIP, "batch graph instantiate: %s", hipGetErrorString(e));
+            }
+            b->graph_block = block;
+        }
+        BCHK(b, launch_set_counter(b->d_it, it0, b->stream));
+        for (; done + block <= nsteps; done += block) BCHK(b, hipGraphLaunch(b->graph, b->stream));
+    }
+    for (; done < nsteps; ++done) BCHK(b, enqueue_batch_step(b, it0 + done, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_batch_synchronize(swpc_batch *b)
+{
+    if (!b) return SWPC_EINVAL;
+    BCHK(b, hipSetDevice(b->device));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return SWPC_OK;
+}
+
+// Autotuner.  Candidate launch shapes (k chunk depth, tile rows) are timed
+// on dry steps: step index 0 injects no source and records nothing, so on
+// the all-zero state before the first step the trials change no field.
+extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, int *by_out, float *ms_out)
+{
+    if (!b || trial_steps < 1) return SWPC_EINVAL;
+    BCHK(b, hipSetDevice(b->device));
+    int rc;
+    if ((rc = batch_sync_members(b))) return rc;
+    const int kc_cand[] = {16, 24, 32, 48, 64};
+    const int by_cand[] = {2, 4};
+    const int variant = b->mem[0]->variant;
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_kc = b->mem[0]->kchunk, best_by = b->by;
+    for (int byv : by_cand)
+        for (int kc : kc_cand) {
+            for (swpc_ctx *c : b->mem)
+                if ((rc = swpc_set_tuning(c, kc, byv, variant)) != SWPC_OK) goto out;
+            b->by = byv;
+            if ((rc = build_tables(b)) != SWPC_OK) goto out;
+            {
+                hipError_t e = enqueue_batch_step(b, 0, nullptr);  // warm
+                if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+                for (int s = 0; s < trial_steps && e == hipSuccess; ++s) e = enqueue_batch_step(b, 0, nullptr);
+                if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) {
+                    rc = bfail(b, SWPC_EHIP, "tuning trial: %s", hipGetErrorString(e));
+                    goto out;
+                }
+                ms /= trial_steps;
+                if (ms < best) {
+                    best = ms;
+                    best_kc = kc;
+                    best_by = byv;
+                }
+            }
+        }
+    for (swpc_ctx *c : b->mem)
+        if ((rc = swpc_set_tuning(c, best_kc, best_by, variant)) != SWPC_OK) goto out;
+    b->by = best_by;
+    rc = build_tables(b);
+    if (kchunk_out) *kchunk_out = best_kc;
+    if (by_out) *by_out = best_by;
+    if (ms_out) *ms_out = best;
+out:
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}

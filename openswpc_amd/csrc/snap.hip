@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) k_snap(SnapArgs a)
     }
 }
 
-hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it)
+hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
 {
     if (c->snap_ntask <= 0) return hipSuccess;
     SnapArgs a;
@@ -143,6 +143,6 @@ hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it)
     a.M0 = d.M0; a.UC = d.UC; a.dt = c->desc.dt;
     a.it = it;
     a.d_it = d_it;
-    hipLaunchKernelGGL(k_snap, dim3(c->snap_nblk), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_snap, dim3(c->snap_nblk), dim3(256), 0, (s ? s : c->stream), a);
     return hipGetLastError();
 }

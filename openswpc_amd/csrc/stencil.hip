@@ -291,15 +291,21 @@ __device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, lon
 
 // kernel__update_stress, m_kernel.f90:142-348 (normal and shear parts fused:
 // both read V and write disjoint arrays)
+// logical tile (bx, by, bz) of a plain launch: blocks of one XCD take a
+// contiguous run of tiles, x fastest
+__device__ static inline void plain_tile(const StepArgs &a, int &bx, int &byy, int &bz)
+{
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    bx = bid % a.nbx;
+    const int t = bid / a.nbx;
+    byy = t % a.nby;
+    bz = t / a.nby;
+}
+
 template <int NM, bool NT>
-__device__ __forceinline__ void stress_int_body(const StepArgs &a)
+__device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int bx = bid % a.nbx;
-    const int t = bid / a.nbx;
-    const int byy = t % a.nby;
-    const int bz = t / a.nby;
     const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
@@ -460,18 +466,23 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a)
 template <int NM, bool NT>
 __global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
 {
-    stress_int_body<NM, NT>(a);
+    int bx, byy, bz;
+    plain_tile(a, bx, byy, bz);
+    stress_int_body<NM, NT>(a, bx, byy, bz);
+}
+
+// batched launch: tile = (member, bx, by, bz), member arguments in HBM
+template <int NM, bool NT>
+__global__ void __launch_bounds__(256) k_stress_int_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    stress_int_body<NM, NT>(mem[t.x], t.y, t.z, t.w);
 }
 
 // kernel__update_vel, m_kernel.f90:75-140, then the bottom PML band
-__global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
+__device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int bx = bid % a.nbx;
-    const int t = bid / a.nbx;
-    const int byy = t % a.nby;
-    const int bz = t / a.nby;
     const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
     const int j = g.jb + byy * a.by + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
@@ -565,6 +576,19 @@ __global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
     }
 }
 
+__global__ void __launch_bounds__(256) k_vel_int(StepArgs a)
+{
+    int bx, byy, bz;
+    plain_tile(a, bx, byy, bz);
+    vel_int_body(a, bx, byy, bz);
+}
+
+__global__ void __launch_bounds__(256) k_vel_int_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    vel_int_body(mem[t.x], t.y, t.z, t.w);
+}
+
 // ---------------------------------------------------------------------------
 // Side columns (kbeg_a = 1, m_global.f90:334-343): y slabs as full-row tiles,
 // x slabs of interior rows as tiles na columns wide with several rows per
@@ -576,10 +600,9 @@ struct PmlLane {
     long long abase, astride;
 };
 
-__device__ static inline bool pml_lane(const StepArgs &a, PmlLane &L)
+__device__ static inline bool pml_lane(const StepArgs &a, const int4 tl, PmlLane &L)
 {
     const Grid &g = a.g;
-    const int4 tl = a.tiles[xcd_remap(blockIdx.x, gridDim.x)];
     const int w = (tl.x >> 20) & 0xFF;
     const bool xslab = (tl.x >> 28) & 1;
     const int lane = threadIdx.y * blockDim.x + threadIdx.x;
@@ -606,10 +629,10 @@ __device__ static inline bool pml_lane(const StepArgs &a, PmlLane &L)
 }
 
 // absorb_p__update_stress, m_absorb_p.f90:448-519
-__global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
+__device__ __forceinline__ void stress_pml_body(const StepArgs &a, const int4 tl)
 {
     PmlLane L;
-    if (!pml_lane(a, L)) return;
+    if (!pml_lane(a, tl, L)) return;
     const Grid &g = a.g;
     const int i = L.i, j = L.j;
     const long long P = g.plane;
@@ -637,10 +660,10 @@ __global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
 }
 
 // absorb_p__update_vel, m_absorb_p.f90:261-306
-__global__ void __launch_bounds__(256) k_vel_pml(StepArgs a)
+__device__ __forceinline__ void vel_pml_body(const StepArgs &a, const int4 tl)
 {
     PmlLane L;
-    if (!pml_lane(a, L)) return;
+    if (!pml_lane(a, tl, L)) return;
     const Grid &g = a.g;
     const int i = L.i, j = L.j;
     const long long P = g.plane;
@@ -661,7 +684,35 @@ __global__ void __launch_bounds__(256) k_vel_pml(StepArgs a)
     }
 }
 
-static void step_args(swpc_ctx *c, StepArgs &a, int mask)
+__global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
+{
+    stress_pml_body(a, a.tiles[xcd_remap(blockIdx.x, gridDim.x)]);
+}
+
+__global__ void __launch_bounds__(256) k_vel_pml(StepArgs a)
+{
+    vel_pml_body(a, a.tiles[xcd_remap(blockIdx.x, gridDim.x)]);
+}
+
+// batched side-column tiles: (i0 | width << 20 | xslab << 28, j0, k0 | k1 << 16, member)
+__device__ static inline int4 unpack_bside(const int4 t) { return make_int4(t.x, t.y, t.z & 0xFFFF, t.z >> 16); }
+
+__global__ void __launch_bounds__(256) k_stress_pml_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    stress_pml_body(mem[t.w], unpack_bside(t));
+}
+
+__global__ void __launch_bounds__(256) k_vel_pml_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    vel_pml_body(mem[t.w], unpack_bside(t));
+}
+
+void fill_step_args(swpc_ctx *c, StepArgs &a, int mask);
+static void step_args(swpc_ctx *c, StepArgs &a, int mask) { fill_step_args(c, a, mask); }
+
+void fill_step_args(swpc_ctx *c, StepArgs &a, int mask)
 {
     a.g = c->g;
     a.f.Vx = c->fields[SWPC_VX]; a.f.Vy = c->fields[SWPC_VY]; a.f.Vz = c->fields[SWPC_VZ];
@@ -819,20 +870,76 @@ static void cerjan_fill(swpc_ctx *c, CerjanArgs &a, int kend)
     a.kend = kend;
 }
 
-hipError_t launch_cerjan_stress(swpc_ctx *c)
+hipError_t launch_cerjan_stress(swpc_ctx *c, hipStream_t s)
 {
     CerjanArgs a;
     cerjan_fill(c, a, c->g.ke);
     dim3 grid((c->g.nx + 255) / 256, c->g.ny, c->g.ke);
-    hipLaunchKernelGGL(k_cerjan_stress, grid, dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_cerjan_stress, grid, dim3(256), 0, (s ? s : c->stream), a);
     return hipGetLastError();
 }
 
-hipError_t launch_cerjan_vel(swpc_ctx *c)
+hipError_t launch_cerjan_vel(swpc_ctx *c, hipStream_t s)
 {
     CerjanArgs a;
     cerjan_fill(c, a, c->g.nz);
     dim3 grid((c->g.nx + 255) / 256, c->g.ny, c->g.nz);
-    hipLaunchKernelGGL(k_cerjan_vel, grid, dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_cerjan_vel, grid, dim3(256), 0, (s ? s : c->stream), a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// batched launches: the arguments of every member live in HBM, the tile
+// tables carry the member index (swpc_batch, batch.hip)
+// ---------------------------------------------------------------------------
+size_t step_args_size() { return sizeof(StepArgs); }
+
+// arguments of one member for the stress (phase 0) or velocity (phase 1)
+// launches of a whole step
+void fill_batch_args(swpc_ctx *c, void *out, int phase)
+{
+    StepArgs a;
+    step_args(c, a, MASK_INT | MASK_PML);
+    if (phase == 1) a.zc = c->d_zchunks + c->nbz + 1;
+    *(StepArgs *)out = a;
+}
+
+// interior-column grid of one member: tiles (bx, by, bz)
+void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz)
+{
+    StepArgs a;
+    step_args(c, a, MASK_INT | MASK_PML);
+    nbx = a.nbx;
+    nby = a.nby;
+    nbz = a.nbz;
+}
+
+hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, const void *d_args, const int4 *d_cols, int ncols,
+                              const int4 *d_side, int nside, hipStream_t s)
+{
+    const StepArgs *mem = (const StepArgs *)d_args;
+    const dim3 block(BX, by);
+    if (nside > 0) {
+        if (phase == 0) hipLaunchKernelGGL(k_stress_pml_b, dim3(nside), block, 0, s, mem, d_side);
+        else hipLaunchKernelGGL(k_vel_pml_b, dim3(nside), block, 0, s, mem, d_side);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ncols <= 0) return hipSuccess;
+    if (phase == 1) {
+        hipLaunchKernelGGL(k_vel_int_b, dim3(ncols), block, 0, s, mem, d_cols);
+        return hipGetLastError();
+    }
+#define SWPC_BSTRESS_CASE(NMV)                                                                      \
+    case NMV:                                                                                       \
+        if (nt) hipLaunchKernelGGL((k_stress_int_b<NMV, true>), dim3(ncols), block, 0, s, mem, d_cols); \
+        else hipLaunchKernelGGL((k_stress_int_b<NMV, false>), dim3(ncols), block, 0, s, mem, d_cols);   \
+        break;
+    switch (nm) {
+        SWPC_BSTRESS_CASE(0) SWPC_BSTRESS_CASE(1) SWPC_BSTRESS_CASE(2) SWPC_BSTRESS_CASE(3) SWPC_BSTRESS_CASE(4)
+        SWPC_BSTRESS_CASE(5) SWPC_BSTRESS_CASE(6) SWPC_BSTRESS_CASE(7) SWPC_BSTRESS_CASE(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef SWPC_BSTRESS_CASE
     return hipGetLastError();
 }

@@ -156,6 +156,7 @@ struct swpc_ctx {
     int *d_it;
     hipGraphExec_t graph_exec;
     int graph_block;
+    unsigned gen;      // bumped whenever launch arguments change
 
     // staging for layout transposes
     float *staging;
@@ -183,13 +184,23 @@ hipError_t launch_kernel_stress(swpc_ctx *c);
 hipError_t launch_kernel_vel(swpc_ctx *c);
 hipError_t launch_pml_stress(swpc_ctx *c);
 hipError_t launch_pml_vel(swpc_ctx *c);
-hipError_t launch_cerjan_stress(swpc_ctx *c);
-hipError_t launch_cerjan_vel(swpc_ctx *c);
-hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it);
-hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it);
-hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it);
+hipError_t launch_cerjan_stress(swpc_ctx *c, hipStream_t s = nullptr);
+hipError_t launch_cerjan_vel(swpc_ctx *c, hipStream_t s = nullptr);
+hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
+hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
+hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
 hipError_t launch_step_advance(swpc_ctx *c);
-hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it);
+hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
+
+hipError_t launch_set_counter(int *d, int v, hipStream_t s);
+hipError_t launch_counter_advance(int *d, hipStream_t s);
+
+// ---- batched launches (stencil.hip) ----
+size_t step_args_size();
+void fill_batch_args(swpc_ctx *c, void *out, int phase);
+void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz);
+hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, const void *d_args, const int4 *d_cols, int ncols,
+                              const int4 *d_side, int nside, hipStream_t s);
 hipError_t launch_vmax(swpc_ctx *c);
 hipError_t launch_ref_to_dev(swpc_ctx *c, const float *src_ref, float *dst, hipStream_t s);
 hipError_t launch_dev_to_ref(swpc_ctx *c, const float *src, float *dst_ref, hipStream_t s);

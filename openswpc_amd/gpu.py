@@ -103,6 +103,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_set_tuning.argtypes = [vp, i, i, i]
     L.swpc_stream.argtypes = [vp]
     L.swpc_stream.restype = vp
+    L.swpc_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
+    L.swpc_batch_destroy.argtypes = [vp]
+    L.swpc_batch_last_error.argtypes = [vp]
+    L.swpc_batch_last_error.restype = C.c_char_p
+    L.swpc_batch_step.argtypes = [vp, i, i, i, i]
+    L.swpc_batch_synchronize.argtypes = [vp]
+    L.swpc_batch_tune.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    L.swpc_batch_stream.argtypes = [vp]
+    L.swpc_batch_stream.restype = vp
     _lib = L
     return L
 
@@ -284,3 +293,47 @@ class SwpcGPU:
 
     def stream(self):
         return self.L.swpc_stream(self.h)
+
+
+class SwpcBatch:
+    """Member simulations stepped together, one launch per phase (swpc_batch_*)."""
+
+    def __init__(self, members):
+        self.L = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        rc = self.L.swpc_batch_create(arr, len(self.members), C.byref(h))
+        if rc != 0:
+            raise SwpcError(f"swpc_batch_create failed ({rc})")
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self.L.swpc_batch_last_error(self.h)
+            raise SwpcError(f"libswpc_gpu batch error {rc}: {msg.decode() if msg else ''}")
+
+    def step(self, it0, nsteps, use_graph=False, block=100):
+        self._chk(self.L.swpc_batch_step(self.h, int(it0), int(nsteps), 1 if use_graph else 0, int(block)))
+
+    def synchronize(self):
+        self._chk(self.L.swpc_batch_synchronize(self.h))
+
+    def tune(self, trial_steps=3):
+        kc, by, ms = C.c_int(), C.c_int(), C.c_float()
+        self._chk(self.L.swpc_batch_tune(self.h, int(trial_steps), C.byref(kc), C.byref(by), C.byref(ms)))
+        return kc.value, by.value, ms.value
+
+    def stream(self):
+        return self.L.swpc_batch_stream(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.swpc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

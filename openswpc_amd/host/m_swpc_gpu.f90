@@ -189,6 +189,40 @@ module m_swpc_gpu
             integer(c_int), value :: kind
             real(c_float), intent(out) :: out(*)
         end function
+        integer(c_int) function swpc_batch_create(members, n, batch) bind(C, name='swpc_batch_create')
+            import :: c_int, c_ptr
+            type(c_ptr), intent(in) :: members(*)
+            integer(c_int), value :: n
+            type(c_ptr), intent(out) :: batch
+        end function
+        integer(c_int) function swpc_batch_destroy(batch) bind(C, name='swpc_batch_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+        end function
+        type(c_ptr) function swpc_batch_last_error(batch) bind(C, name='swpc_batch_last_error')
+            import :: c_ptr
+            type(c_ptr), value :: batch
+        end function
+        integer(c_int) function swpc_batch_step(batch, it0, nsteps, use_graph, block) bind(C, name='swpc_batch_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: it0, nsteps, use_graph, block
+        end function
+        integer(c_int) function swpc_batch_synchronize(batch) bind(C, name='swpc_batch_synchronize')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+        end function
+        integer(c_int) function swpc_batch_tune(batch, trial_steps, kchunk, by, ms) bind(C, name='swpc_batch_tune')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: batch
+            integer(c_int), value :: trial_steps
+            integer(c_int), intent(out) :: kchunk, by
+            real(c_float), intent(out) :: ms
+        end function
+        type(c_ptr) function swpc_batch_stream(batch) bind(C, name='swpc_batch_stream')
+            import :: c_ptr
+            type(c_ptr), value :: batch
+        end function
         integer(c_int) function swpc_synchronize(ctx) bind(C, name='swpc_synchronize')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

@@ -1,0 +1,76 @@
+"""Batches of uneven simulations stepped together (swpc_batch_*): every
+member must end bit-identical to its own oracle run, with and without graph
+replays, with Cerjan and PML members mixed, and after the autotuner's dry
+trials."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from gpu_helpers import compare_fields, gpu_from_model, small_config, ulp_diff
+from openswpc_amd.gpu import SwpcBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _members():
+    mt = dict(x=0.3, y=-0.6, z=4.2, t0=0.05, tr=0.8, mo=1e15, strike=30.0, dip=60.0, rake=90.0)
+    bf = dict(x=-0.7, y=0.4, z=3.3, t0=0.0, tr=0.6, fx=0.3, fy=-1.0, fz=0.5)
+    return [
+        small_config(nx=40, ny=36, nz=32, na=8, nm=2, sources=[mt]),
+        small_config(nx=37, ny=41, nz=29, na=6, nm=2, sources=[bf], bf_mode=True),
+        small_config(nx=48, ny=34, nz=36, na=10, nm=2, sources=[mt, dict(mt, x=-1.1, strike=120.0)]),
+        small_config(nx=33, ny=30, nz=27, na=5, nm=2, abc="cerjan", sources=[bf], bf_mode=True),
+    ]
+
+
+def _check_members(gs, runs):
+    for n, (g, run) in enumerate(zip(gs, runs)):
+        res = compare_fields(g, run)
+        for name, (ulp, rel) in res.items():
+            assert ulp == 0, (n, name, ulp, rel)
+        assert ulp_diff(g.seismograms("vel"), run.wav["vel"]) == 0, n
+        assert np.any(run.field("Vz") != 0), n
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_members_match_their_oracles(orc, graph):
+    nt = 18
+    cfgs = _members()
+    models = [O.build_model(c) for c in cfgs]
+    runs = []
+    for m in models:
+        r = O.OracleRun(m, 32)
+        r.run(nt)
+        runs.append(r)
+    gs = [gpu_from_model(m) for m in models]
+    b = SwpcBatch(gs)
+    if graph:
+        b.step(1, 12, use_graph=True, block=4)
+        b.step(13, nt - 12, use_graph=True, block=3)
+    else:
+        b.step(1, nt)
+    b.synchronize()
+    _check_members(gs, runs)
+    b.close()
+
+
+def test_tuner_dry_trials_leave_state_untouched(orc):
+    nt = 10
+    cfgs = _members()[:3]
+    models = [O.build_model(c) for c in cfgs]
+    runs = []
+    for m in models:
+        r = O.OracleRun(m, 32)
+        r.run(nt)
+        runs.append(r)
+    gs = [gpu_from_model(m) for m in models]
+    b = SwpcBatch(gs)
+    kc, by, ms = b.tune(trial_steps=2)
+    assert kc > 0 and by in (2, 4) and ms > 0
+    for g in gs:
+        for name in ("Vx", "Sxx"):
+            assert not np.any(g.get_field(name))
+    b.step(1, nt, use_graph=True, block=5)
+    b.synchronize()
+    _check_members(gs, runs)
+    b.close()
