@@ -83,6 +83,118 @@ def plain_spec(seed_src=1, seed_st=2):
                      fq_min=0.05, fq_max=5.0, fq_ref=1.0)
 
 
+def batch_specs(n=64, seed_sizes=6, seed_prof=10, seed_src=11, seed_st=12, nt=1500):
+    """Batched Green's functions: n independent runs of uneven size, NM=2,
+    10-cell PML, one unit force along a seeded axis at a seeded depth, 8
+    surface receivers, each on one of four seeded 1D layered profiles."""
+    from openswpc_amd.host import ModelSpec
+    rs = np.random.default_rng(seed_sizes)
+    sizes = np.stack([rs.integers(96, 257, n), rs.integers(96, 257, n), rs.integers(96, 193, n)], axis=1)
+    rp = np.random.default_rng(seed_prof)
+    profiles = []
+    for _ in range(4):
+        tops = np.concatenate([[0.0], np.sort(rp.uniform(2.0, 30.0, 2))])
+        vs = np.sort(rp.uniform(2.0, 4.5, 3))
+        profiles.append([[float(t), float(2.2 + 0.2 * v), float(v * math.sqrt(3)), float(v), 100.0 * v, 50.0 * v]
+                         for t, v in zip(tops, vs)])
+    rsrc = np.random.default_rng(seed_src)
+    rst = np.random.default_rng(seed_st)
+    dx, dt = 0.5, 0.025
+    specs = []
+    for m in range(n):
+        nx, ny, nz = (int(v) for v in sizes[m])
+        axis = int(rsrc.integers(0, 3))
+        f = [0.0, 0.0, 0.0]
+        f[axis] = 1.0
+        depth = float(rsrc.uniform(2.0, min(20.0, (nz - 15) * dx)))
+        src = [dict(x=0.0, y=0.0, z=depth, t0=0.0, tr=1.0, fx=f[0], fy=f[1], fz=f[2])]
+        hx, hy = (nx / 2 - 14) * dx, (ny / 2 - 14) * dx
+        st = [(float(rst.uniform(-hx, hx)), float(rst.uniform(-hy, hy)), 0.0, "fsb") for _ in range(8)]
+        specs.append(ModelSpec(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx, dt=dt, nt=nt, na=10, nm=2, zbeg=-1.0,
+                               vmodel="lhm", layers=profiles[m % 4], sources=src, bf_mode=True, stations=st,
+                               fq_min=0.05, fq_max=5.0, fq_ref=1.0))
+    return specs
+
+
+def run_batch(args):
+    """--workload batch: the batched launch against the same members stepped
+    one after another, each as graph replays."""
+    from openswpc_amd.gpu import SwpcBatch
+    from openswpc_amd.host import build_inputs, make_simulation
+    t0 = time.perf_counter()
+    specs = batch_specs()
+    sims = []
+    for sp in specs:
+        inp = build_inputs(sp)
+        sims.append(make_simulation(inp))
+        del inp
+    cells = sum(sp.nx * sp.ny * sp.nz for sp in specs)
+    log(f"[bench] batch of {len(specs)} members, {cells / 1e6:.1f} M cells, set-up {time.perf_counter() - t0:.1f}s")
+    b = SwpcBatch(sims)
+    t0 = time.perf_counter()
+    kc, by, tune_ms = b.tune(trial_steps=3)
+    log(f"[bench] autotuned kchunk={kc} by={by} ({tune_ms:.2f} ms/step in trials, {time.perf_counter() - t0:.1f}s)")
+    ev = HipEvents()
+    block = min(args.block, max(args.steps, 1))
+    it = 1
+    b.step(it, args.warmup, use_graph=True, block=min(block, max(args.warmup, 1)))
+    it += args.warmup
+    b.synchronize()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, b.stream())
+    b.step(it, args.steps, use_graph=True, block=block)
+    ev.record(e1, b.stream())
+    ms_batch = ev.elapsed_ms(e0, e1) / args.steps
+    it += args.steps
+    # the same members one after another (each its own graph of `block` steps)
+    for s in sims:
+        s.step(it, block, use_graph=True, block=block)
+    for s in sims:
+        s.synchronize()
+    it += block
+    nseq = max(block, (args.steps // block) * block)
+    t_seq = 0.0
+    for s in sims:
+        a0, a1 = ev.create(), ev.create()
+        ev.record(a0, s.stream())
+        s.step(it, nseq, use_graph=True, block=block)
+        ev.record(a1, s.stream())
+        t_seq += ev.elapsed_ms(a0, a1)
+    ms_seq = t_seq / nseq
+    nm = 2
+    n_int = sum(cell_counts(sp.nx, sp.ny, sp.nz, sp.na)[0] for sp in specs)
+    n_pml = cells - n_int
+    b_si, b_vi, b_sp, b_vp = bytes_model(nm)
+    bytes_step = n_int * (b_si + b_vi) + n_pml * (b_sp + b_vp)
+    value = cells / (ms_batch * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched Green's functions: {len(specs)} uneven grids 96..256 x 96..256 x 96..192 "
+                               f"(seed 6), NM=2, 10-cell PML, unit single forces, 8 receivers each, 4 layered "
+                               f"profiles", "members": len(specs), "cells": cells, "graph_block": block,
+                   "autotuned": {"kchunk": kc, "by": by}},
+        "step_roofline": {"bytes_per_step": bytes_step, "achieved": round(bytes_step / (ms_batch * 1e-3) / 1e9, 2),
+                          "unit": "GB/s", "frac": round(bytes_step / (ms_batch * 1e-3) / HBM_PEAK, 4)},
+        "sequential": {"ms_per_step": round(ms_seq, 4), "value": round(cells / (ms_seq * 1e-3) / 1e9, 4),
+                       "speedup_batched": round(ms_seq / ms_batch, 3)},
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+    for s in sims:
+        s.close()
+
+
 def cell_counts(nx, ny, nz, na):
     n_int = (nx - 2 * na) * (ny - 2 * na) * (nz - na)
     return n_int, nx * ny * nz - n_int
@@ -171,7 +283,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["production", "plain"], default="production")
+    ap.add_argument("--workload", choices=["production", "plain", "batch"], default="production")
     ap.add_argument("--block", type=int, default=50, help="steps per captured HIP graph")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -186,6 +298,10 @@ def main():
                                    "domain on one device; multi-GPU sharding is not built (see DESIGN.md)",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
+
+    if args.workload == "batch":
+        run_batch(args)
+        return
 
     from openswpc_amd.host import build_inputs, make_simulation
 
