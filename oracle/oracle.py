@@ -544,3 +544,127 @@ def rel_l2(a, b) -> float:
     if den == 0:
         return float(np.linalg.norm(a.ravel()))
     return float(np.linalg.norm((a - b).ravel()) / den)
+
+
+# ----------------------------------------------------------------------------
+# Discrete adjoint (elastic, PML): forward with every step kept, misfit,
+# backward pass with gradients (adjoint.c)
+# ----------------------------------------------------------------------------
+FIELDS_AV = FIELDS_ADE[:9]
+FIELDS_AS = FIELDS_ADE[9:]
+
+
+def _adj_type(mp_double: bool, sp_double: bool):
+    mp = C.c_void_p
+    fields = ([(n, mp) for n in FIELDS_V + FIELDS_S]
+              + [("aV", C.c_void_p * 9), ("aS", C.c_void_p * 9), ("D", C.c_void_p * 9)]
+              + [("g_lam", C.c_void_p), ("g_mu", C.c_void_p), ("g_rho", C.c_void_p)])
+    return type("OrcAdj", (C.Structure,), {"_fields_": fields})
+
+
+class OracleAdjoint:
+    """Misfit J = 1/2 sum (d - d_obs)^2 over the velocity traces of every step
+    (the wav__store formula, m_wav.f90:517-531) and its gradient with respect
+    to lam, mu and rho, by the discrete adjoint of the elastic PML step."""
+
+    def __init__(self, model: Model, prec: int = 64):
+        cfg = model.cfg
+        if cfg.nm != 0 or cfg.abc != "pml":
+            raise ValueError("the adjoint covers the elastic (NM = 0) PML scheme")
+        self.run = OracleRun(model, prec)
+        self.m = model
+        L = self.run.L
+        L.orc_adj_vel.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_adj_stress.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_adj_record.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _i32p, _i32p, _i32p, C.c_float, C.c_float,
+                                     np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+        self.L = L
+
+    def traces(self):
+        """velocity traces (nst, 3) of the current state"""
+        m, a = self.m, self.run.arr
+        sc = np.float64(np.float32(m.M0)) * np.float64(np.float32(m.UC)) * 1e9
+        out = np.zeros((m.nst, 3))
+        for n in range(m.nst):
+            i, j, k = int(m.ist[n]) + OFF, int(m.jst[n]) + OFF, int(m.kst[n]) + OFF
+            out[n, 0] = (a["Vx"][j, i, k] + a["Vx"][j, i - 1, k]) / 2 * sc
+            out[n, 1] = (a["Vy"][j, i, k] + a["Vy"][j - 1, i, k]) / 2 * sc
+            out[n, 2] = -(a["Vz"][j, i, k] + a["Vz"][j, i, k - 1]) / 2 * sc
+        return out
+
+    def forward(self, nt, keep=True):
+        """nt steps from rest; returns traces (nt, nst, 3) of V_0 .. V_{nt-1}"""
+        run, L, p, m = self.run, self.L, self.run.ptr, self.m
+        self.A, self.B = [], []
+        d = np.zeros((nt, m.nst, 3))
+        for n in range(nt):
+            it = n + 1
+            d[n] = self.traces()
+            if keep:
+                self.A.append({k: run.arr[k].copy() for k in FIELDS_V + FIELDS_AV})
+            L.orc_kernel_update_stress(p)
+            L.orc_absorb_update_stress(p)
+            if m.nsrc and not m.cfg.bf_mode:
+                st = np.ascontiguousarray(m.stf[it - 1])
+                L.orc_source_stressglut(p, m.nsrc, m.isrc, m.jsrc, m.ksrc, m.mo, m.M0,
+                                        *[np.ascontiguousarray(m.mij[c]) for c in range(6)], st)
+            if keep:
+                self.B.append({k: run.arr[k].copy() for k in FIELDS_S + FIELDS_AS})
+            L.orc_kernel_update_vel(p)
+            L.orc_absorb_update_vel(p)
+            if m.nsrc and m.cfg.bf_mode:
+                st = np.ascontiguousarray(m.stf[it - 1])
+                L.orc_source_bodyforce(p, m.nsrc, m.isrc, m.jsrc, m.ksrc, np.ascontiguousarray(m.mij[0]),
+                                       np.ascontiguousarray(m.mij[1]), np.ascontiguousarray(m.mij[2]), m.M0, st)
+        return d
+
+    def new_adjoint(self):
+        run = self.run
+        shp = n3(self.m.cfg)
+        self.adj = {k: np.zeros(shp, run.mp) for k in FIELDS_V + FIELDS_S}
+        for k in FIELDS_ADE:
+            self.adj["a" + k] = np.zeros_like(run.arr[k])
+        self.dens = [np.zeros(shp, run.mp) for _ in range(9)]
+        self.grad = {k: np.zeros(shp) for k in ("lam", "mu", "rho")}
+        st = _adj_type(True, True)()
+        for k in FIELDS_V + FIELDS_S:
+            setattr(st, k, self.adj[k].ctypes.data)
+        for q, k in enumerate(FIELDS_AV):
+            st.aV[q] = self.adj["a" + k].ctypes.data
+        for q, k in enumerate(FIELDS_AS):
+            st.aS[q] = self.adj["a" + k].ctypes.data
+        for q in range(9):
+            st.D[q] = self.dens[q].ctypes.data
+        st.g_lam, st.g_mu, st.g_rho = (self.grad[k].ctypes.data for k in ("lam", "mu", "rho"))
+        self.adj_st = st
+        self.adj_ptr = C.cast(C.pointer(st), C.c_void_p)
+
+    def adj_vel(self):
+        self.L.orc_adj_vel(self.run.ptr, self.adj_ptr)
+
+    def adj_stress(self):
+        self.L.orc_adj_stress(self.run.ptr, self.adj_ptr)
+
+    def backward(self, res):
+        """res (nt, nst, 3) = d - d_obs; returns gradients (lam, mu, rho)"""
+        nt = res.shape[0]
+        run, m = self.run, self.m
+        self.new_adjoint()
+        for n in range(nt - 1, -1, -1):
+            for k, v in self.B[n].items():
+                run.arr[k][...] = v
+            self.adj_vel()
+            for k, v in self.A[n].items():
+                run.arr[k][...] = v
+            self.adj_stress()
+            self.L.orc_adj_record(run.ptr, self.adj_ptr, m.nst, m.ist, m.jst, m.kst, m.M0, m.UC,
+                                  np.ascontiguousarray(res[n], np.float64).reshape(-1))
+        return self.grad["lam"], self.grad["mu"], self.grad["rho"]
+
+
+def misfit_and_gradient(model: Model, nt: int, d_obs: np.ndarray, prec: int = 64):
+    adj = OracleAdjoint(model, prec)
+    d = adj.forward(nt)
+    res = d - d_obs
+    g = adj.backward(res)
+    return 0.5 * float(np.sum(res * res)), g, d

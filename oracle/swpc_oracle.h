@@ -83,6 +83,9 @@ typedef struct orc_state {
     sp_t *cgx_c, *cgx_b, *cgy_c, *cgy_b, *cgz_c, *cgz_b;
 } orc_state;
 
+/* elements of one (k,i,j) halo array (step.c) */
+int64_t orc_n3(const orc_state *s);
+
 /* index helpers */
 static inline int64_t orc_i3(const orc_state *s, int k, int i, int j)
 {
