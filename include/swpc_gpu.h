@@ -240,6 +240,23 @@ int swpc_batch_synchronize(swpc_batch *batch);
 int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, float *ms_per_step);
 void *swpc_batch_stream(swpc_batch *batch);
 
+/* ---- discrete adjoint (elastic, nm = 0, ADE-CFS PML) ----
+   Misfit J = 1/2 sum over recorded samples, stations and components of
+   (d - d_obs)^2, d the velocity traces of swpc_get_seismograms(SWPC_WAV_V),
+   and its gradient with respect to the medium arrays lam, mu, rho of
+   swpc_set_medium, by the discrete adjoint of the time step.  Runs nt steps
+   from rest with the context's sources, checkpointing the state every ckp
+   steps (device memory: ceil(nt/ckp) + ckp copies of the fields and PML
+   auxiliaries), then steps backward segment by segment, recomputing each
+   segment's forward states from its checkpoint.  obs is (ntw, 3, nst) like
+   the traces; gradients come back in the reference (k,i,j) halo layout for
+   cells 1..n (the halo entries are zero); any output pointer may be NULL.
+   The context's fields are left in an intermediate state.  There is no
+   counterpart in the reference (which has no adjoint); the forward it
+   differentiates is the one of src/swpc_3d/main.f90:119-139. */
+int swpc_adj_gradient(swpc_ctx *ctx, int nt, const float *obs, int ckp, double *misfit, float *g_lam, float *g_mu,
+                      float *g_rho);
+
 /* Wait for all work queued on the context. */
 int swpc_synchronize(swpc_ctx *ctx);
 

@@ -1,0 +1,593 @@
+// adjoint.hip -- discrete adjoint of the elastic PML time step and the
+// checkpointed gradient driver (swpc_adj_gradient).
+//
+// The forward step is linear in (V, S, PML auxiliaries) for a fixed medium;
+// its adjoint applies the transposed updates in reverse order.  Per phase:
+//   1. density kernel (one thread per cell): the adjoint of every spatial
+//      derivative the cell's forward update used (9 per phase), the updated
+//      auxiliary adjoints, and the cell's gradient terms;
+//   2. gather kernel (one thread per field point): each point collects the
+//      densities of the cells whose stencils read it, with those cells'
+//      coefficients -- the band-switched 4th-order pair in the interior,
+//      1/h in the PML.
+// Gathers replace scatters, so no atomics and a fixed summation order.
+// The forward state the gradients need (V_n and its auxiliaries for the
+// stress update, S_{n+1} and the stress auxiliaries for the velocity
+// update) is recomputed segment by segment from checkpoints taken every
+// `ckp` steps of a first forward pass.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "swpc_internal.h"
+
+namespace {
+
+struct AdjArgs {
+    Grid g;
+    Coef4 c;
+    float dt, hx, hy, hz;
+    const float *rho, *lam, *mu;
+    const int *kfs_top, *kfs_bot, *kob_top, *kob_bot;
+    PmlBox box[5];
+    long long npml;
+    int has_pml;
+    const float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
+    float *aF[9];       // adjoint fields: Vx Vy Vz Sxx Syy Szz Syz Sxz Sxy
+    float *aA;          // adjoint auxiliaries, 18 x npml
+    float *D[9];        // per-cell derivative adjoints
+    float *q[3];        // per-cell gradient terms gathered in phase 2
+    const float *fF[9]; // forward fields of the step (same order as aF)
+    const float *fA;    // forward auxiliaries of the phase, 9 x npml
+    float *g_lam, *g_mu, *g_rho;
+};
+
+enum { FVX = 0, FVY, FVZ, FSXX, FSYY, FSZZ, FSYZ, FSXZ, FSXY };
+
+__device__ static inline bool in_band(int k, int kft, int kfb, int kot, int kob)
+{
+    const int a = (k - kft) * (kfb - k);
+    const int b = (k - kot) * (kob - k);
+    return (a > b ? a : b) >= 0;
+}
+
+__device__ static inline bool is_int(const Grid &g, int i, int j, int k)
+{
+    return i >= g.ib && i <= g.ie && j >= g.jb && j <= g.je && k >= 1 && k <= g.ke;
+}
+
+// compact auxiliary index of a PML cell, -1 elsewhere
+__device__ static inline long long ade_index(const AdjArgs &a, int i, int j, int k)
+{
+    const Grid &g = a.g;
+    if (!a.has_pml || i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz || is_int(g, i, j, k)) return -1;
+    const int na = g.na;
+    int q;
+    if (j <= na) q = 0;
+    else if (j > g.ny - na) q = 1;
+    else if (i <= na) q = 2;
+    else if (i > g.nx - na) q = 3;
+    else q = 4;
+    const PmlBox b = a.box[q];
+    return b.off + ((long long)(k - b.k0) * b.nyb + (j - b.j0)) * b.nxb + (i - b.i0);
+}
+
+// 4th-order pair of a cell along an axis (the forward kernel's band switch)
+__device__ static inline void pair_at(const AdjArgs &a, int axis, int i, int j, int k, float &p40, float &p41)
+{
+    const long long c2 = g2idx(a.g, i, j);
+    const int in = in_band(k, a.kfs_top[c2], a.kfs_bot[c2], a.kob_top[c2], a.kob_bot[c2]) ? 1 : 0;
+    if (axis == 0) { p40 = a.c.re40x[in]; p41 = a.c.re41x[in]; }
+    else if (axis == 1) { p40 = a.c.re40y[in]; p41 = a.c.re41y[in]; }
+    else { p40 = a.c.re40z[in]; p41 = a.c.re41z[in]; }
+}
+
+// 0 = not updated, 1 = interior (4th order), 2 = PML (2nd order)
+__device__ static inline int cell_kind(const AdjArgs &a, int i, int j, int k)
+{
+    const Grid &g = a.g;
+    if (i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz) return 0;
+    if (is_int(g, i, j, k)) return 1;
+    return a.has_pml ? 2 : 0;
+}
+
+__device__ static inline float hinv(const AdjArgs &a, int axis) { return axis == 0 ? a.hx : axis == 1 ? a.hy : a.hz; }
+
+// gather of the densities of a backward-type derivative (cell c reads f(c),
+// f(c-1) and at 4th order f(c+1), f(c-2)) onto point (i,j,k)
+__device__ static float gather_bwd(const AdjArgs &a, const float *D, int axis, int i, int j, int k)
+{
+    const int off[4] = {0, 1, -1, 2};
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int o = off[t];
+        const int ii = i + (axis == 0 ? o : 0), jj = j + (axis == 1 ? o : 0), kk = k + (axis == 2 ? o : 0);
+        const int kind = cell_kind(a, ii, jj, kk);
+        if (!kind) continue;
+        const float d = D[gidx(a.g, ii, jj, kk)];
+        if (kind == 1) {
+            float p40, p41;
+            pair_at(a, axis, ii, jj, kk, p40, p41);
+            acc += (o == 0 ? p40 : o == 1 ? -p40 : o == -1 ? -p41 : p41) * d;
+        } else if (o == 0) {
+            acc += hinv(a, axis) * d;
+        } else if (o == 1) {
+            acc -= hinv(a, axis) * d;
+        }
+    }
+    return acc;
+}
+
+// gather of a forward-type derivative (cell c reads f(c+1), f(c) and at 4th
+// order f(c+2), f(c-1))
+__device__ static float gather_fwd(const AdjArgs &a, const float *D, int axis, int i, int j, int k)
+{
+    const int off[4] = {-1, 0, -2, 1};
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int o = off[t];
+        const int ii = i + (axis == 0 ? o : 0), jj = j + (axis == 1 ? o : 0), kk = k + (axis == 2 ? o : 0);
+        const int kind = cell_kind(a, ii, jj, kk);
+        if (!kind) continue;
+        const float d = D[gidx(a.g, ii, jj, kk)];
+        if (kind == 1) {
+            float p40, p41;
+            pair_at(a, axis, ii, jj, kk, p40, p41);
+            acc += (o == -1 ? p40 : o == 0 ? -p40 : o == -2 ? -p41 : p41) * d;
+        } else if (o == -1) {
+            acc += hinv(a, axis) * d;
+        } else if (o == 0) {
+            acc -= hinv(a, axis) * d;
+        }
+    }
+    return acc;
+}
+
+#define AT(F, di, dj, dk) (F)[gidx(g, i + (di), j + (dj), k + (dk))]
+#define BWD4(F, di, dj, dk, p40, p41) ((AT(F, 0, 0, 0) - AT(F, -(di), -(dj), -(dk))) * (p40) \
+                                       - (AT(F, di, dj, dk) - AT(F, -2 * (di), -2 * (dj), -2 * (dk))) * (p41))
+#define FWD4(F, di, dj, dk, p40, p41) ((AT(F, di, dj, dk) - AT(F, 0, 0, 0)) * (p40) \
+                                       - (AT(F, 2 * (di), 2 * (dj), 2 * (dk)) - AT(F, -(di), -(dj), -(dk))) * (p41))
+#define BWD2(F, di, dj, dk, h) ((AT(F, 0, 0, 0) - AT(F, -(di), -(dj), -(dk))) * (h))
+#define FWD2(F, di, dj, dk, h) ((AT(F, di, dj, dk) - AT(F, 0, 0, 0)) * (h))
+
+__device__ static inline float4 prof(const float4 *p, int n) { return p[n - 1]; }
+
+// adjoint of the velocity update: densities, stress-auxiliary adjoints, rho terms
+__global__ void __launch_bounds__(256) k_adj_vel_dens(AdjArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    const Grid &g = a.g;
+    if (i > g.nx || j > g.ny) return;
+    const int kind = cell_kind(a, i, j, k);
+    if (!kind) return;
+    const long long p = gidx(g, i, j, k);
+    const float r0 = a.rho[p];
+    const float sx = r0 + AT(a.rho, 1, 0, 0), sy = r0 + AT(a.rho, 0, 1, 0), sz = r0 + AT(a.rho, 0, 0, 1);
+    const float vx = a.aF[FVX][p], vy = a.aF[FVY][p], vz = a.aF[FVZ][p];
+    const float wx = 2.0f / sx * a.dt * vx, wy = 2.0f / sy * a.dt * vy, wz = 2.0f / sz * a.dt * vz;
+    const float *Sxx = a.fF[FSXX], *Syy = a.fF[FSYY], *Szz = a.fF[FSZZ];
+    const float *Syz = a.fF[FSYZ], *Sxz = a.fF[FSXZ], *Sxy = a.fF[FSXY];
+    float fx, fy, fz;
+    if (kind == 1) {
+        float ax, bx, ay, by, az, bz;
+        pair_at(a, 0, i, j, k, ax, bx);
+        pair_at(a, 1, i, j, k, ay, by);
+        pair_at(a, 2, i, j, k, az, bz);
+        a.D[0][p] = wx; a.D[1][p] = wx; a.D[2][p] = wx;
+        a.D[3][p] = wy; a.D[4][p] = wy; a.D[5][p] = wy;
+        a.D[6][p] = wz; a.D[7][p] = wz; a.D[8][p] = wz;
+        fx = FWD4(Sxx, 1, 0, 0, ax, bx) + BWD4(Sxy, 0, 1, 0, ay, by) + BWD4(Sxz, 0, 0, 1, az, bz);
+        fy = BWD4(Sxy, 1, 0, 0, ax, bx) + FWD4(Syy, 0, 1, 0, ay, by) + BWD4(Syz, 0, 0, 1, az, bz);
+        fz = BWD4(Sxz, 1, 0, 0, ax, bx) + BWD4(Syz, 0, 1, 0, ay, by) + FWD4(Szz, 0, 0, 1, az, bz);
+    } else {
+        const long long q = ade_index(a, i, j, k);
+        const long long n = a.npml;
+        const float4 gp[9] = {prof(a.gxe, i), prof(a.gyc, j), prof(a.gzc, k), prof(a.gxc, i), prof(a.gye, j),
+                              prof(a.gzc, k), prof(a.gxc, i), prof(a.gyc, j), prof(a.gze, k)};
+        const float w[9] = {wx, wx, wx, wy, wy, wy, wz, wz, wz};
+        const float der[9] = {FWD2(Sxx, 1, 0, 0, a.hx), BWD2(Sxy, 0, 1, 0, a.hy), BWD2(Sxz, 0, 0, 1, a.hz),
+                              BWD2(Sxy, 1, 0, 0, a.hx), FWD2(Syy, 0, 1, 0, a.hy), BWD2(Syz, 0, 0, 1, a.hz),
+                              BWD2(Sxz, 1, 0, 0, a.hx), BWD2(Syz, 0, 1, 0, a.hy), FWD2(Szz, 0, 0, 1, a.hz)};
+        float f[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            float *ah = a.aA + (long long)(A_axSxx + c) * n + q;
+            const float ahat = *ah;
+            a.D[c][p] = gp[c].x * w[c] + gp[c].w * a.dt * ahat;
+            *ah = gp[c].y * w[c] + gp[c].z * ahat;
+            f[c / 3] += gp[c].x * der[c] + gp[c].y * a.fA[(long long)c * n + q];
+        }
+        fx = f[0]; fy = f[1]; fz = f[2];
+    }
+    a.q[0][p] = -2.0f / (sx * sx) * a.dt * fx * vx;
+    a.q[1][p] = -2.0f / (sy * sy) * a.dt * fy * vy;
+    a.q[2][p] = -2.0f / (sz * sz) * a.dt * fz * vz;
+}
+
+// adjoint of the velocity update: stress adjoints and the rho gradient
+__global__ void __launch_bounds__(256) k_adj_vel_gather(AdjArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    const Grid &g = a.g;
+    if (i > g.nx || j > g.ny) return;
+    const long long p = gidx(g, i, j, k);
+    a.aF[FSXX][p] += gather_fwd(a, a.D[0], 0, i, j, k);
+    a.aF[FSYY][p] += gather_fwd(a, a.D[4], 1, i, j, k);
+    a.aF[FSZZ][p] += gather_fwd(a, a.D[8], 2, i, j, k);
+    a.aF[FSXY][p] += gather_bwd(a, a.D[1], 1, i, j, k) + gather_bwd(a, a.D[3], 0, i, j, k);
+    a.aF[FSXZ][p] += gather_bwd(a, a.D[2], 2, i, j, k) + gather_bwd(a, a.D[6], 0, i, j, k);
+    a.aF[FSYZ][p] += gather_bwd(a, a.D[5], 2, i, j, k) + gather_bwd(a, a.D[7], 1, i, j, k);
+    // each cell's density enters the buoyancy of its own point and of the
+    // point one cell back along each axis
+    float gr = a.q[0][p] + a.q[1][p] + a.q[2][p];
+    if (cell_kind(a, i - 1, j, k)) gr += AT(a.q[0], -1, 0, 0);
+    if (cell_kind(a, i, j - 1, k)) gr += AT(a.q[1], 0, -1, 0);
+    if (cell_kind(a, i, j, k - 1)) gr += AT(a.q[2], 0, 0, -1);
+    if (cell_kind(a, i, j, k)) a.g_rho[p] += gr;
+}
+
+__device__ static inline void hmean4_grad(float a, float b, float c, float d, float &h, float gr[4])
+{
+    const float eps = 1.1920928955078125e-07f;
+    const float N = 4.0f * a * b * c * d, Dn = a * b * c + a * b * d + a * c * d + b * c * d + eps;
+    h = N / Dn;
+    const float D2 = Dn * Dn;
+    gr[0] = (4.0f * b * c * d * Dn - N * (b * c + b * d + c * d)) / D2;
+    gr[1] = (4.0f * a * c * d * Dn - N * (a * c + a * d + c * d)) / D2;
+    gr[2] = (4.0f * a * b * d * Dn - N * (a * b + a * d + b * d)) / D2;
+    gr[3] = (4.0f * a * b * c * Dn - N * (a * b + a * c + b * c)) / D2;
+}
+
+// adjoint of the stress update: densities, velocity-auxiliary adjoints,
+// lam and normal mu gradients, shear terms for the mu gather
+__global__ void __launch_bounds__(256) k_adj_stress_dens(AdjArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    const Grid &g = a.g;
+    if (i > g.nx || j > g.ny) return;
+    const int kind = cell_kind(a, i, j, k);
+    if (!kind) return;
+    const long long p = gidx(g, i, j, k);
+    const float lam = a.lam[p], mu = a.mu[p];
+    const float sxx = a.aF[FSXX][p], syy = a.aF[FSYY][p], szz = a.aF[FSZZ][p];
+    const float X = a.dt * ((lam + 2.0f * mu) * sxx + lam * (syy + szz));
+    const float Y = a.dt * ((lam + 2.0f * mu) * syy + lam * (sxx + szz));
+    const float Z = a.dt * ((lam + 2.0f * mu) * szz + lam * (sxx + syy));
+    const float *mup = a.mu;
+    float hyz, hxz, hxy, gtmp[4];
+    hmean4_grad(mu, AT(mup, 0, 0, 1), AT(mup, 0, 1, 0), AT(mup, 0, 1, 1), hyz, gtmp);
+    hmean4_grad(mu, AT(mup, 0, 0, 1), AT(mup, 1, 0, 0), AT(mup, 1, 0, 1), hxz, gtmp);
+    hmean4_grad(mu, AT(mup, 1, 0, 0), AT(mup, 0, 1, 0), AT(mup, 1, 1, 0), hxy, gtmp);
+    const float syz = a.aF[FSYZ][p], sxz = a.aF[FSXZ][p], sxy = a.aF[FSXY][p];
+    const float Wyz = a.dt * hyz * syz, Wxz = a.dt * hxz * sxz, Wxy = a.dt * hxy * sxy;
+    const float *Vx = a.fF[FVX], *Vy = a.fF[FVY], *Vz = a.fF[FVZ];
+    float ex, ey, ez, tyz, txz, txy;
+    if (kind == 1) {
+        float ax, bx, ay, by, az, bz;
+        pair_at(a, 0, i, j, k, ax, bx);
+        pair_at(a, 1, i, j, k, ay, by);
+        pair_at(a, 2, i, j, k, az, bz);
+        a.D[0][p] = X; a.D[1][p] = Y; a.D[2][p] = Z;
+        a.D[3][p] = Wyz; a.D[4][p] = Wyz; a.D[5][p] = Wxz; a.D[6][p] = Wxz; a.D[7][p] = Wxy; a.D[8][p] = Wxy;
+        ex = BWD4(Vx, 1, 0, 0, ax, bx);
+        ey = BWD4(Vy, 0, 1, 0, ay, by);
+        ez = BWD4(Vz, 0, 0, 1, az, bz);
+        tyz = FWD4(Vz, 0, 1, 0, ay, by) + FWD4(Vy, 0, 0, 1, az, bz);
+        txz = FWD4(Vz, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 0, 1, az, bz);
+        txy = FWD4(Vy, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 1, 0, ay, by);
+    } else {
+        const long long q = ade_index(a, i, j, k);
+        const long long n = a.npml;
+        // derivatives dxVx dyVy dzVz dyVz dzVy dxVz dzVx dxVy dyVx and their auxiliaries
+        const float4 gp[9] = {prof(a.gxc, i), prof(a.gyc, j), prof(a.gzc, k), prof(a.gye, j), prof(a.gze, k),
+                              prof(a.gxe, i), prof(a.gze, k), prof(a.gxe, i), prof(a.gye, j)};
+        const int ai[9] = {A_axVx, A_ayVy, A_azVz, A_ayVz, A_azVy, A_axVz, A_azVx, A_axVy, A_ayVx};
+        const float w[9] = {X, Y, Z, Wyz, Wyz, Wxz, Wxz, Wxy, Wxy};
+        const float der[9] = {BWD2(Vx, 1, 0, 0, a.hx), BWD2(Vy, 0, 1, 0, a.hy), BWD2(Vz, 0, 0, 1, a.hz),
+                              FWD2(Vz, 0, 1, 0, a.hy), FWD2(Vy, 0, 0, 1, a.hz), FWD2(Vz, 1, 0, 0, a.hx),
+                              FWD2(Vx, 0, 0, 1, a.hz), FWD2(Vy, 1, 0, 0, a.hx), FWD2(Vx, 0, 1, 0, a.hy)};
+        float t[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            float *ah = a.aA + (long long)ai[c] * n + q;
+            const float ahat = *ah;
+            a.D[c][p] = gp[c].x * w[c] + gp[c].w * a.dt * ahat;
+            *ah = gp[c].y * w[c] + gp[c].z * ahat;
+            t[c] = gp[c].x * der[c] + gp[c].y * a.fA[(long long)ai[c] * n + q];
+        }
+        ex = t[0]; ey = t[1]; ez = t[2];
+        tyz = t[3] + t[4]; txz = t[5] + t[6]; txy = t[7] + t[8];
+    }
+    a.g_lam[p] += a.dt * (ex + ey + ez) * (sxx + syy + szz);
+    a.g_mu[p] += 2.0f * a.dt * (ex * sxx + ey * syy + ez * szz);
+    a.q[0][p] = a.dt * tyz * syz;
+    a.q[1][p] = a.dt * txz * sxz;
+    a.q[2][p] = a.dt * txy * sxy;
+}
+
+// adjoint of the stress update: velocity adjoints and the shear part of the
+// mu gradient (each shear point's harmonic mean spans four cells)
+__global__ void __launch_bounds__(256) k_adj_stress_gather(AdjArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    const Grid &g = a.g;
+    if (i > g.nx || j > g.ny) return;
+    const long long p = gidx(g, i, j, k);
+    a.aF[FVX][p] += gather_bwd(a, a.D[0], 0, i, j, k) + gather_fwd(a, a.D[8], 1, i, j, k) + gather_fwd(a, a.D[6], 2, i, j, k);
+    a.aF[FVY][p] += gather_bwd(a, a.D[1], 1, i, j, k) + gather_fwd(a, a.D[7], 0, i, j, k) + gather_fwd(a, a.D[4], 2, i, j, k);
+    a.aF[FVZ][p] += gather_bwd(a, a.D[2], 2, i, j, k) + gather_fwd(a, a.D[5], 0, i, j, k) + gather_fwd(a, a.D[3], 1, i, j, k);
+    if (!cell_kind(a, i, j, k)) return;
+    // quads whose harmonic mean includes this cell: origin o = this cell
+    // minus the corner offset; corner index per (yz, xz, xy) ordering of
+    // the forward hmean4 arguments
+    const float *mu = a.mu;
+    float gm = 0.0f, h, gr[4];
+    // yz: args (o, o+z, o+y, o+y+z)
+    {
+        const int oy[4] = {0, 0, -1, -1}, oz[4] = {0, -1, 0, -1};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int ii = i, jj = j + oy[c], kk = k + oz[c];
+            if (!cell_kind(a, ii, jj, kk)) continue;
+            const long long o = gidx(g, ii, jj, kk);
+            hmean4_grad(mu[o], mu[gidx(g, ii, jj, kk + 1)], mu[gidx(g, ii, jj + 1, kk)], mu[gidx(g, ii, jj + 1, kk + 1)], h, gr);
+            gm += a.q[0][o] * gr[c];
+        }
+    }
+    // xz: args (o, o+z, o+x, o+x+z)
+    {
+        const int ox[4] = {0, 0, -1, -1}, oz[4] = {0, -1, 0, -1};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int ii = i + ox[c], jj = j, kk = k + oz[c];
+            if (!cell_kind(a, ii, jj, kk)) continue;
+            const long long o = gidx(g, ii, jj, kk);
+            hmean4_grad(mu[o], mu[gidx(g, ii, jj, kk + 1)], mu[gidx(g, ii + 1, jj, kk)], mu[gidx(g, ii + 1, jj, kk + 1)], h, gr);
+            gm += a.q[1][o] * gr[c];
+        }
+    }
+    // xy: args (o, o+x, o+y, o+x+y)
+    {
+        const int ox[4] = {0, -1, 0, -1}, oy[4] = {0, 0, -1, -1};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int ii = i + ox[c], jj = j + oy[c], kk = k;
+            if (!cell_kind(a, ii, jj, kk)) continue;
+            const long long o = gidx(g, ii, jj, kk);
+            hmean4_grad(mu[o], mu[gidx(g, ii + 1, jj, kk)], mu[gidx(g, ii, jj + 1, kk)], mu[gidx(g, ii + 1, jj + 1, kk)], h, gr);
+            gm += a.q[2][o] * gr[c];
+        }
+    }
+    a.g_mu[p] += gm;
+}
+
+// adjoint of the velocity traces at one recorded step (m_wav.f90:517-531):
+// one thread walks the stations in order, so stations that share points add
+// deterministically
+__global__ void k_adj_record(AdjArgs a, const int *ist, const int *jst, const int *kst, int nst, const float *wav,
+                             const float *obs, int ntw, int itw, float sc)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const Grid &g = a.g;
+    for (int n = 0; n < nst; ++n) {
+        const int i = ist[n], j = jst[n], k = kst[n];
+        float r[3];
+        for (int c = 0; c < 3; ++c) {
+            const long long w = ((long long)n * 3 + c) * ntw + itw;
+            r[c] = sc * (wav[w] - obs[w]);
+        }
+        a.aF[FVX][gidx(g, i, j, k)] += r[0];
+        a.aF[FVX][gidx(g, i - 1, j, k)] += r[0];
+        a.aF[FVY][gidx(g, i, j, k)] += r[1];
+        a.aF[FVY][gidx(g, i, j - 1, k)] += r[1];
+        a.aF[FVZ][gidx(g, i, j, k)] -= r[2];
+        a.aF[FVZ][gidx(g, i, j, k - 1)] -= r[2];
+    }
+}
+
+__global__ void k_fill(float *p, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        p[t] = 0.0f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// driver
+// ---------------------------------------------------------------------------
+namespace {
+
+struct AdjWork {
+    std::vector<float *> bufs;
+    ~AdjWork()
+    {
+        for (float *p : bufs)
+            if (p) (void)hipFree(p);
+    }
+    float *alloc(size_t n)
+    {
+        float *p = nullptr;
+        if (hipMalloc((void **)&p, n * sizeof(float)) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return p;
+    }
+};
+
+hipError_t zero(float *p, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int adj_fail(swpc_ctx *c, int code, const char *msg);
+
+extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp, double *misfit, float *g_lam,
+                                 float *g_mu, float *g_rho)
+{
+    if (!c) return SWPC_EINVAL;
+    if (!obs || nt < 1 || ckp < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_gradient: bad arguments");
+    if (c->desc.nm != 0 || c->desc.abc != SWPC_ABC_PML || c->npml == 0)
+        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: the adjoint covers the elastic (nm = 0) PML scheme");
+    if (!c->has_medium || !c->has_pml) return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: medium or PML not set");
+    if (c->nst <= 0 || !(c->wav_what & SWPC_WAV_V) || !c->d_wav[0])
+        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: velocity receivers not set");
+    if (c->snap_ntask > 0) return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: disable sections first");
+    if (hipSetDevice(c->device) != hipSuccess) return SWPC_EHIP;
+    hipStream_t s = c->stream;
+    const Grid &g = c->g;
+    const long long nf = g.n, na18 = (long long)A_N * c->npml, na9 = 9 * c->npml;
+    const int nck = (nt + ckp - 1) / ckp;
+    const long long per_state = 9 * nf + na18;
+    AdjWork w;
+    float *ck = w.alloc((size_t)(nck * per_state));
+    float *seg = w.alloc((size_t)(ckp * per_state));
+    float *adjF = w.alloc((size_t)(9 * nf));
+    float *adjA = w.alloc((size_t)na18);
+    float *dens = w.alloc((size_t)(9 * nf));
+    float *qq = w.alloc((size_t)(3 * nf));
+    float *grad = w.alloc((size_t)(3 * nf));
+    const long long nobs = (long long)c->ntw * 3 * c->nst;
+    float *d_obs = w.alloc((size_t)nobs);
+    if (!ck || !seg || !adjF || !adjA || !dens || !qq || !grad || !d_obs)
+        return adj_fail(c, SWPC_ENOMEM, "swpc_adj_gradient: device memory for checkpoints and adjoint state");
+    if (hipMemcpyAsync(d_obs, obs, nobs * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) return SWPC_EHIP;
+
+    // ---- forward pass from rest, checkpoints every ckp steps ----
+    for (int q = 0; q < SWPC_NFIELD; ++q)
+        if (zero(c->fields[q], nf, s) != hipSuccess) return SWPC_EHIP;
+    if (zero(c->ade, na18, s) != hipSuccess) return SWPC_EHIP;
+    if (c->d_uacc && zero(c->d_uacc, 3LL * c->nst, s) != hipSuccess) return SWPC_EHIP;
+    if (c->d_eacc && zero(c->d_eacc, 6LL * c->nst, s) != hipSuccess) return SWPC_EHIP;
+    auto save_state = [&](float *dst) -> hipError_t {
+        for (int q = 0; q < SWPC_NFIELD; ++q) {
+            hipError_t e = hipMemcpyAsync(dst + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipMemcpyAsync(dst + 9 * nf, c->ade, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    };
+    auto load_state = [&](const float *src) -> hipError_t {
+        for (int q = 0; q < SWPC_NFIELD; ++q) {
+            hipError_t e = hipMemcpyAsync(c->fields[q], src + q * nf, nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipMemcpyAsync(c->ade, src + 9 * nf, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    };
+    hipError_t e = hipSuccess;
+    for (int n = 0; n < nt && e == hipSuccess; ++n) {
+        const int it = n + 1;
+        if (n % ckp == 0) e = save_state(ck + (size_t)(n / ckp) * per_state);
+        if (e == hipSuccess) e = launch_wav_store(c, it, nullptr);
+        if (e == hipSuccess) e = launch_stress_side(c, nullptr);
+        if (e == hipSuccess) e = launch_stress_cols(c, 3);
+        if (e == hipSuccess) e = launch_stressglut(c, it, nullptr);
+        if (e == hipSuccess) e = launch_vel_side(c, nullptr);
+        if (e == hipSuccess) e = launch_vel_cols(c, 3);
+        if (e == hipSuccess) e = launch_bodyforce(c, it, nullptr);
+    }
+    if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
+
+    // ---- misfit on the host (double) ----
+    std::vector<float> syn(nobs);
+    if (hipMemcpyAsync(syn.data(), c->d_wav[0], nobs * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return SWPC_EHIP;
+    const int nrec = (nt - 1) / c->ntdec_w + 1 < c->ntw ? (nt - 1) / c->ntdec_w + 1 : c->ntw;
+    double J = 0.0;
+    for (int st = 0; st < c->nst; ++st)
+        for (int cc = 0; cc < 3; ++cc)
+            for (int t = 0; t < nrec; ++t) {
+                const size_t w3 = ((size_t)st * 3 + cc) * c->ntw + t;
+                const double r = (double)syn[w3] - (double)obs[w3];
+                J += 0.5 * r * r;
+            }
+    if (misfit) *misfit = J;
+
+    // ---- backward pass ----
+    AdjArgs a;
+    a.g = g;
+    a.c = c->coef;
+    a.dt = c->desc.dt;
+    a.hx = 1.0f / (float)c->desc.dx;
+    a.hy = 1.0f / (float)c->desc.dy;
+    a.hz = 1.0f / (float)c->desc.dz;
+    a.rho = c->rho; a.lam = c->lam; a.mu = c->mu;
+    a.kfs_top = c->kfs_top; a.kfs_bot = c->kfs_bot; a.kob_top = c->kob_top; a.kob_bot = c->kob_bot;
+    for (int q = 0; q < 5; ++q) a.box[q] = c->box[q];
+    a.npml = c->npml;
+    a.has_pml = 1;
+    a.gxc = c->gxc; a.gxe = c->gxe; a.gyc = c->gyc; a.gye = c->gye; a.gzc = c->gzc; a.gze = c->gze;
+    for (int q = 0; q < 9; ++q) {
+        a.aF[q] = adjF + q * nf;
+        a.D[q] = dens + q * nf;
+    }
+    for (int q = 0; q < 3; ++q) a.q[q] = qq + q * nf;
+    a.aA = adjA;
+    a.g_lam = grad; a.g_mu = grad + nf; a.g_rho = grad + 2 * nf;
+    if (zero(adjF, 9 * nf, s) != hipSuccess || zero(adjA, na18, s) != hipSuccess || zero(grad, 3 * nf, s) != hipSuccess)
+        return SWPC_EHIP;
+    const dim3 blk(64, 4, 1), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    const float sc = 0.5f * c->wav_M0 * c->wav_UC * 1e9f;
+    // per step of a segment: V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
+    const long long sstep = per_state;
+    for (int sg = nck - 1; sg >= 0 && e == hipSuccess; --sg) {
+        const int n0 = sg * ckp, n1 = (n0 + ckp < nt) ? n0 + ckp : nt;
+        e = load_state(ck + (size_t)sg * per_state);
+        for (int n = n0; n < n1 && e == hipSuccess; ++n) {
+            float *st = seg + (size_t)(n - n0) * sstep;
+            const int it = n + 1;
+            for (int q = 0; q < 3 && e == hipSuccess; ++q)
+                e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(st + 9 * nf, c->ade, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = launch_stress_side(c, nullptr);
+            if (e == hipSuccess) e = launch_stress_cols(c, 3);
+            if (e == hipSuccess) e = launch_stressglut(c, it, nullptr);
+            for (int q = 3; q < 9 && e == hipSuccess; ++q)
+                e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(st + 9 * nf + na9, c->ade + na9, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = launch_vel_side(c, nullptr);
+            if (e == hipSuccess) e = launch_vel_cols(c, 3);
+            if (e == hipSuccess) e = launch_bodyforce(c, it, nullptr);
+        }
+        for (int n = n1 - 1; n >= n0 && e == hipSuccess; --n) {
+            const float *st = seg + (size_t)(n - n0) * sstep;
+            for (int q = 0; q < 9; ++q) a.fF[q] = st + q * nf;
+            a.fA = st + 9 * nf + na9;      // stress auxiliaries aS_n
+            hipLaunchKernelGGL(k_adj_vel_dens, grd, blk, 0, s, a);
+            hipLaunchKernelGGL(k_adj_vel_gather, grd, blk, 0, s, a);
+            a.fA = st + 9 * nf;            // velocity auxiliaries aV_n
+            hipLaunchKernelGGL(k_adj_stress_dens, grd, blk, 0, s, a);
+            hipLaunchKernelGGL(k_adj_stress_gather, grd, blk, 0, s, a);
+            if (n % c->ntdec_w == 0 && n / c->ntdec_w < c->ntw)
+                hipLaunchKernelGGL(k_adj_record, dim3(1), dim3(64), 0, s, a, c->d_ist, c->d_jst, c->d_kst, c->nst,
+                                   c->d_wav[0], d_obs, c->ntw, n / c->ntdec_w, sc);
+            e = hipGetLastError();
+        }
+    }
+    if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
+    // gradients out in the reference (k,i,j) halo layout
+    float *outs[3] = {g_lam, g_mu, g_rho};
+    for (int q = 0; q < 3; ++q) {
+        if (!outs[q]) continue;
+        if (launch_dev_to_ref(c, grad + q * nf, c->staging, s) != hipSuccess ||
+            hipMemcpyAsync(outs[q], c->staging, (size_t)c->staging_n * sizeof(float), hipMemcpyDeviceToHost, s) !=
+                hipSuccess)
+            return SWPC_EHIP;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return SWPC_EHIP;
+    return SWPC_OK;
+}

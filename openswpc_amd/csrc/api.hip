@@ -136,6 +136,8 @@ static int build_zchunks(swpc_ctx *c)
     return SWPC_OK;
 }
 
+int adj_fail(swpc_ctx *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
+
 extern "C" const char *swpc_last_error(const swpc_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
 extern "C" int64_t swpc_device_bytes(const swpc_ctx *ctx) { return ctx ? ctx->bytes : 0; }

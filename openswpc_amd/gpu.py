@@ -103,6 +103,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_set_tuning.argtypes = [vp, i, i, i]
     L.swpc_stream.argtypes = [vp]
     L.swpc_stream.restype = vp
+    L.swpc_adj_gradient.argtypes = [vp, i, fp, i, C.POINTER(C.c_double), vp, vp, vp]
     L.swpc_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
     L.swpc_batch_destroy.argtypes = [vp]
     L.swpc_batch_last_error.argtypes = [vp]
@@ -260,6 +261,16 @@ class SwpcGPU:
         out = np.empty((3, ns2, ns1), np.float32)
         self._chk(self.L.swpc_get_snap_max(self.h, snap_section(name), out))
         return out
+
+    def adjoint_gradient(self, nt, obs, ckp=45):
+        """misfit 1/2 sum (d - obs)^2 of the velocity traces after nt steps from
+        rest and its gradients (lam, mu, rho), reference layout"""
+        shp = (self.ny + 6, self.nx + 6, self.nz + 6)
+        gl, gm, gr = (np.zeros(shp, np.float32) for _ in range(3))
+        J = C.c_double()
+        self._chk(self.L.swpc_adj_gradient(self.h, int(nt), _f32(obs), int(ckp), C.byref(J), gl.ctypes.data,
+                                           gm.ctypes.data, gr.ctypes.data))
+        return J.value, gl, gm, gr
 
     def synchronize(self):
         self._chk(self.L.swpc_synchronize(self.h))

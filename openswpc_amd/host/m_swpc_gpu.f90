@@ -189,6 +189,15 @@ module m_swpc_gpu
             integer(c_int), value :: kind
             real(c_float), intent(out) :: out(*)
         end function
+        integer(c_int) function swpc_adj_gradient(ctx, nt, obs, ckp, misfit, g_lam, g_mu, g_rho) &
+            bind(C, name='swpc_adj_gradient')
+            import :: c_int, c_ptr, c_float, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nt, ckp
+            real(c_float), intent(in) :: obs(*)
+            real(c_double), intent(out) :: misfit
+            real(c_float), intent(out) :: g_lam(*), g_mu(*), g_rho(*)
+        end function
         integer(c_int) function swpc_batch_create(members, n, batch) bind(C, name='swpc_batch_create')
             import :: c_int, c_ptr
             type(c_ptr), intent(in) :: members(*)
