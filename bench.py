@@ -195,6 +195,81 @@ def run_batch(args):
         s.close()
 
 
+def adjoint_specs(seed_anom=7, seed_src=13, seed_st=14, nt=2000):
+    """Adjoint gradient workload: 256 x 256 x 192, dx = 0.2 km, elastic,
+    20-cell PML.  True model = layered + a seeded Gaussian velocity anomaly
+    (+/- 8 %, radius 3 km); initial model = the layers alone; one seeded
+    moment tensor, 64 seeded surface receivers."""
+    from openswpc_amd.host import ModelSpec
+    nx, ny, nz, dx = 256, 256, 192, 0.2
+    layers = [[0.0, 2.4, 4.5, 2.6, 1e6, 1e6], [4.0, 2.6, 5.6, 3.2, 1e6, 1e6], [15.0, 2.9, 6.6, 3.8, 1e6, 1e6]]
+    ra = np.random.default_rng(seed_anom)
+    sign = 1.0 if ra.uniform() < 0.5 else -1.0
+    cx, cy, cz = ra.uniform(-8, 8), ra.uniform(-8, 8), ra.uniform(8, 20)
+    k = np.arange(nz + 6) - 2
+    i = np.arange(nx + 6) - 2
+    j = np.arange(ny + 6) - 2
+    z = -1.0 + (k - 0.5) * dx
+    x = -nx / 2 * dx + (i - 0.5) * dx
+    y = -ny / 2 * dx + (j - 0.5) * dx
+    r2 = (y[:, None, None] - cy) ** 2 + (x[None, :, None] - cx) ** 2 + (z[None, None, :] - cz) ** 2
+    xi = (sign * 0.08 * np.exp(-r2 / 3.0 ** 2)).astype(np.float32)
+    rs = np.random.default_rng(seed_src)
+    src = [dict(x=float(rs.uniform(-5, 5)), y=float(rs.uniform(-5, 5)), z=float(rs.uniform(10, 20)), t0=0.0, tr=1.0,
+                mo=1e15, strike=float(rs.uniform(0, 360)), dip=float(rs.uniform(0, 90)),
+                rake=float(rs.uniform(-180, 180)))]
+    rr = np.random.default_rng(seed_st)
+    half = nx * dx / 2 - 25 * dx
+    st = [(float(a), float(b), 0.0, "fsb") for a, b in rr.uniform(-half, half, size=(64, 2))]
+    common = dict(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx, dt=0.012, nt=nt, na=20, nm=0, zbeg=-1.0, vmodel="lhm",
+                  layers=layers, sources=src, stations=st, wav_what=1, fq_min=0.05, fq_max=5.0, fq_ref=1.0)
+    return ModelSpec(xi=xi, **common), ModelSpec(**common), dict(sign=sign, centre=(cx, cy, cz))
+
+
+def run_adjoint(args):
+    """--workload adjoint: misfit gradient with checkpoints every 45 steps,
+    against the plain forward loop on the same grid"""
+    from openswpc_amd.host import build_inputs, make_simulation
+    nt = args.adj_steps
+    t0 = time.perf_counter()
+    true_spec, init_spec, info = adjoint_specs(nt=nt)
+    sim_true = make_simulation(build_inputs(true_spec))
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, sim_true.stream())
+    sim_true.step(1, nt, use_graph=True, block=min(args.block, nt))
+    ev.record(e1, sim_true.stream())
+    ms_fwd = ev.elapsed_ms(e0, e1) / nt
+    obs = sim_true.seismograms("vel")
+    sim_true.close()
+    sim = make_simulation(build_inputs(init_spec))
+    log(f"[bench] adjoint workload set-up and true data {time.perf_counter() - t0:.1f}s")
+    t = time.perf_counter()
+    J, gl, gm, gr = sim.adjoint_gradient(nt, obs, ckp=45)
+    wall = time.perf_counter() - t
+    sim.close()
+    nx, ny, nz = init_spec.nx, init_spec.ny, init_spec.nz
+    cells = nx * ny * nz
+    value = cells * nt / wall / 1e9
+    inner = (slice(3, -3),) * 3
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s (gradient: forward, recomputation and adjoint of every step)",
+        "n_gpus": 1, "steps": nt, "warmup": 0, "ms_per_step": round(wall * 1e3 / nt, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "adjoint gradient, elastic 256x256x192, dx=0.2 km, 20-cell PML, layered model; "
+                               "data from layered + Gaussian anomaly (8 %, radius 3 km, seed 7); 1 moment tensor, "
+                               "64 receivers; checkpoints every 45 steps", "anomaly_sign": info["sign"]},
+        "forward_plain": {"ms_per_step": round(ms_fwd, 4), "value": round(cells / (ms_fwd * 1e-3) / 1e9, 4)},
+        "gradient_over_forward_time": round(wall * 1e3 / nt / ms_fwd, 3),
+        "misfit": J,
+        "grad_norms": {"lam": float(np.linalg.norm(gl[inner])), "mu": float(np.linalg.norm(gm[inner])),
+                       "rho": float(np.linalg.norm(gr[inner]))},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cell_counts(nx, ny, nz, na):
     n_int = (nx - 2 * na) * (ny - 2 * na) * (nz - na)
     return n_int, nx * ny * nz - n_int
@@ -283,7 +358,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["production", "plain", "batch"], default="production")
+    ap.add_argument("--workload", choices=["production", "plain", "batch", "adjoint"], default="production")
+    ap.add_argument("--adj-steps", type=int, default=2000, help="time steps of the adjoint workload")
     ap.add_argument("--block", type=int, default=50, help="steps per captured HIP graph")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -301,6 +377,9 @@ def main():
 
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args)
         return
 
     from openswpc_amd.host import build_inputs, make_simulation
