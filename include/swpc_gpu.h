@@ -134,8 +134,8 @@ int swpc_update_stress(swpc_ctx *ctx);
 int swpc_update_vel(swpc_ctx *ctx);
 
 /* nsteps whole steps it0 .. it0+nsteps-1 in the order of the main loop
-   (src/swpc_3d/main.f90:119-139): wav__store, stress, glut, vel,
-   body force.  With use_graph != 0 the steps run as replays of a captured
+   (src/swpc_3d/main.f90:119-139): green__store, wav__store, snap__write,
+   stress, glut, vel, body force, green__source.  With use_graph != 0 the steps run as replays of a captured
    HIP graph of `block` steps (falls back to direct launches for the tail). */
 int swpc_step(swpc_ctx *ctx, int it0, int nsteps, int use_graph, int block);
 
@@ -184,7 +184,7 @@ int swpc_snap_dims(const swpc_ctx *ctx, int section, int dims[3]);
 
 /* Launch slots timed by swpc_step_timed, in step order. */
 enum swpc_timer {
-    SWPC_T_WAV = 0,       /* wav__store                                          */
+    SWPC_T_WAV = 0,       /* green__store + wav__store                           */
     SWPC_T_SNAP,          /* snap__write                                         */
     SWPC_T_STRESS_SIDE,   /* absorb__update_stress, side columns (x and y slabs)  */
     SWPC_T_STRESS,        /* kernel__update_stress + its bottom PML band         */
@@ -193,7 +193,7 @@ enum swpc_timer {
     SWPC_T_VEL_SIDE,      /* absorb__update_vel, side columns                    */
     SWPC_T_VEL,           /* kernel__update_vel + its bottom PML band            */
     SWPC_T_VEL_ABC,       /* absorb__update_vel with the Cerjan sponge           */
-    SWPC_T_BODYFORCE,     /* source__bodyforce                                   */
+    SWPC_T_BODYFORCE,     /* source__bodyforce + green__source                   */
     SWPC_NTIMER
 };
 
@@ -214,6 +214,30 @@ int swpc_set_field(swpc_ctx *ctx, int field, const float *in);
 /* Seismograms (ntw, ncmp, nst) like wav_vel/wav_disp/wav_stress/wav_strain
    (src/swpc_3d/m_wav.f90:37-38).  kind = one SWPC_WAV_* flag. */
 int swpc_get_seismograms(swpc_ctx *ctx, int kind, float *out);
+
+/* ---- reciprocity Green's-function mode (src/swpc_3d/m_green.f90) ----
+   The device part of green__setup (m_green.f90:66-404): ng GF cells
+   (ig, jg, kg; 1-based indices the host has already filtered to the solid
+   part, m_green.f90:225-296), the virtual source cell (isrc, jsrc, ksrc) of
+   the station named by green_stnm, the force direction cmp (0 = 'x',
+   1 = 'y', 2 = 'z'), body-force Green's functions on (bforce != 0: ncmp = 9)
+   or off (ncmp = 6), output decimation ntdec_w and length ntw, and the
+   moment rate stf[it-1] at t = tbeg + it*dt for it = 1..nt (m_green.f90:632).
+   Accumulators and outputs start at zero, as at it == 1 of green__store.
+   cmp < 0 switches the mode off.  swpc_step runs green__store first and
+   green__source last in every step (main.f90:122,137). */
+int swpc_set_green(swpc_ctx *ctx, int ng, const int32_t *ig, const int32_t *jg, const int32_t *kg,
+                   int isrc, int jsrc, int ksrc, int cmp, int bforce, int ntdec_w, int ntw, int nt,
+                   const float *stf);
+/* green__store(it) (m_green.f90:406-545) and green__source(it)
+   (m_green.f90:614-649) */
+int swpc_green_store(swpc_ctx *ctx, int it);
+int swpc_green_source(swpc_ctx *ctx, int it);
+/* gf(ntw, ncmp*ng) as green__store leaves it (m_green.f90:56), Fortran order:
+   column n*ncmp + c holds mxx myy mzz myz mxz mxy [fx fy fz] of GF cell n.
+   green__export's sign flip for cmp = 'z' (m_green.f90:563-566) is the
+   caller's.  out holds ntw*ncmp*ng floats.  Synchronises. */
+int swpc_get_green(swpc_ctx *ctx, float *out);
 
 /* ---- batches: many independent simulations stepped together ----
    The reference runs one simulation per process; a batch of Green's

@@ -310,6 +310,7 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     dfree(c->d_targets); dfree(c->d_contrib); dfree(c->d_comp); dfree(c->d_mo); dfree(c->d_stf);
     dfree(c->d_ist); dfree(c->d_jst); dfree(c->d_kst); dfree(c->d_uacc); dfree(c->d_eacc);
     for (int q = 0; q < 4; ++q) dfree(c->d_wav[q]);
+    dfree(c->d_green_ijk); dfree(c->d_green_acc); dfree(c->d_gf); dfree(c->d_green_stf);
     dfree(c->staging); dfree(c->d_vmax); dfree(c->d_it);
     free_snapshots(c);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
@@ -579,6 +580,93 @@ extern "C" int swpc_set_receivers(swpc_ctx *c, int nst, const int32_t *ist, cons
     return SWPC_OK;
 }
 
+static void free_green(swpc_ctx *c)
+{
+    dfree(c->d_green_ijk); dfree(c->d_green_acc); dfree(c->d_gf); dfree(c->d_green_stf);
+    c->d_green_ijk = nullptr;
+    c->d_green_acc = c->d_gf = c->d_green_stf = nullptr;
+    c->green_on = 0;
+    c->green_ng = c->green_nt = 0;
+}
+
+extern "C" int swpc_set_green(swpc_ctx *c, int ng, const int32_t *ig, const int32_t *jg, const int32_t *kg,
+                              int isrc, int jsrc, int ksrc, int cmp, int bforce, int ntdec_w, int ntw, int nt,
+                              const float *stf)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_green(c);
+    reset_graph(c);
+    if (cmp < 0) return SWPC_OK;
+    if (cmp > 2 || ng < 0 || nt < 0 || ntdec_w < 1 || ntw < 1 || (ng > 0 && (!ig || !jg || !kg)) ||
+        (nt > 0 && !stf))
+        return fail(c, SWPC_EINVAL, "swpc_set_green: bad arguments");
+    const Grid &g = c->g;
+    if (isrc < 1 || isrc > g.nx || jsrc < 1 || jsrc > g.ny || ksrc < 1 || ksrc > g.nz)
+        return fail(c, SWPC_EINVAL, "swpc_set_green: virtual source cell outside the grid");
+    for (int n = 0; n < ng; ++n)
+        if (ig[n] < 1 || ig[n] > g.nx || jg[n] < 1 || jg[n] > g.ny || kg[n] < 1 || kg[n] > g.nz)
+            return fail(c, SWPC_EINVAL, "swpc_set_green: GF point %d outside the grid", n);
+    const int ncmp = bforce ? 9 : 6;
+    // accumulators and output start at zero (m_green.f90:424-441, :324)
+    if (dalloc(c, (void **)&c->d_green_ijk, (size_t)3 * ng * sizeof(int)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_green_acc, (size_t)12 * ng * sizeof(float)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_gf, (size_t)ntw * ncmp * ng * sizeof(float)) != hipSuccess ||
+        dalloc(c, (void **)&c->d_green_stf, (size_t)nt * sizeof(float)) != hipSuccess) {
+        free_green(c);
+        return fail(c, SWPC_ENOMEM, "swpc_set_green: device allocation failed");
+    }
+    if (ng > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->d_green_ijk, ig, ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_green_ijk + ng, jg, ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_green_ijk + 2 * (size_t)ng, kg, ng * sizeof(int), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    if (nt > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_green_stf, stf, (size_t)nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->green_ng = ng;
+    c->green_ncmp = ncmp;
+    c->green_ntdec = ntdec_w;
+    c->green_ntw = ntw;
+    c->green_nt = nt;
+    c->green_src[0] = isrc; c->green_src[1] = jsrc; c->green_src[2] = ksrc;
+    // unit force along green_cmp (m_green.f90:136-143)
+    for (int q = 0; q < 3; ++q) c->green_f[q] = q == cmp ? 1.0f : 0.0f;
+    // dt_dxyz = real(dt / (dx*dy*dz)) (m_green.f90:145)
+    const float dx = (float)c->desc.dx, dy = (float)c->desc.dy, dz = (float)c->desc.dz;
+    c->green_dt_dxyz = c->desc.dt / (dx * dy * dz);
+    c->green_on = 1;
+    return SWPC_OK;
+}
+
+extern "C" int swpc_green_store(swpc_ctx *c, int it)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, launch_green_store(c, it, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_green_source(swpc_ctx *c, int it)
+{
+    if (!c) return SWPC_EINVAL;
+    HIPCHK(c, launch_green_source(c, it, nullptr));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_get_green(swpc_ctx *c, float *out)
+{
+    if (!c || !out) return SWPC_EINVAL;
+    if (!c->green_on) return fail(c, SWPC_ESTATE, "swpc_get_green: Green's-function mode not set");
+    const size_t n = (size_t)c->green_ntw * c->green_ncmp * c->green_ng;
+    if (n == 0) return SWPC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_gf, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
 static int ready(swpc_ctx *c)
 {
     if (!c->has_medium) return fail(c, SWPC_ESTATE, "medium not set");
@@ -823,6 +911,7 @@ static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it, hipEvent_t 
         return r != hipSuccess ? r : hipStreamWaitEvent(c->stream, c->ev_join, 0);
     };
     if ((e = mark()) != hipSuccess) return e;
+    if ((e = launch_green_store(c, it, d_it)) != hipSuccess) return e;
     if ((e = launch_wav_store(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if ((e = launch_snap(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if ((e = fork_to_side()) != hipSuccess) return e;
@@ -840,7 +929,8 @@ static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it, hipEvent_t 
     if ((e = mark()) != hipSuccess) return e;
     if (!pml && (e = launch_cerjan_vel(c)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
-    if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
+    if ((e = launch_bodyforce(c, it, d_it)) != hipSuccess) return e;
+    if ((e = launch_green_source(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if (d_it) e = launch_step_advance(c);
     return e;
 }

@@ -142,6 +142,7 @@ static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
     hipError_t e;
     hipStream_t s = b->stream;
     for (swpc_ctx *c : b->mem) {
+        if ((e = launch_green_store(c, it, d_it, s)) != hipSuccess) return e;
         if ((e = launch_wav_store(c, it, d_it, s)) != hipSuccess) return e;
         if ((e = launch_snap(c, it, d_it, s)) != hipSuccess) return e;
     }
@@ -158,6 +159,7 @@ static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
     for (swpc_ctx *c : b->mem) {
         if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_vel(c, s)) != hipSuccess) return e;
         if ((e = launch_bodyforce(c, it, d_it, s)) != hipSuccess) return e;
+        if ((e = launch_green_source(c, it, d_it, s)) != hipSuccess) return e;
     }
     if (d_it) e = launch_counter_advance(b->d_it, s);
     return e;

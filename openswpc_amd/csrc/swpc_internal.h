@@ -142,6 +142,17 @@ struct swpc_ctx {
     float *d_uacc, *d_eacc;
     float *d_wav[4];   // V, U, stress, strain: (ntw, ncmp, nst)
 
+    // reciprocity Green's functions (m_green.f90)
+    int green_on;
+    int green_ng, green_ncmp, green_ntdec, green_ntw, green_nt;
+    int green_src[3];  // virtual source cell (a station)
+    float green_f[3];  // unit force direction (fx1, fy1, fz1)
+    float green_dt_dxyz;
+    int *d_green_ijk;  // ig | jg | kg, ng each
+    float *d_green_acc;  // (12, ng)
+    float *d_gf;       // (ntw, ncmp*ng)
+    float *d_green_stf;  // nt
+
     // sections (snapshots)
     swpc_snap_desc snap;
     int snap_ntask, snap_nblk;
@@ -189,6 +200,8 @@ hipError_t launch_cerjan_vel(swpc_ctx *c, hipStream_t s = nullptr);
 hipError_t launch_stressglut(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
 hipError_t launch_bodyforce(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
 hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
+hipError_t launch_green_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
+hipError_t launch_green_source(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
 hipError_t launch_step_advance(swpc_ctx *c);
 hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it, hipStream_t s = nullptr);
 

@@ -85,7 +85,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for nm in ("swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_kernel_update_vel",
                "swpc_absorb_update_vel", "swpc_update_stress", "swpc_update_vel", "swpc_synchronize"):
         getattr(L, nm).argtypes = [vp]
-    for nm in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store"):
+    L.swpc_set_green.argtypes = [vp, i, ip, ip, ip, i, i, i, i, i, i, i, i, fp]
+    L.swpc_get_green.argtypes = [vp, fp]
+    for nm in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store", "swpc_green_store",
+               "swpc_green_source"):
         getattr(L, nm).argtypes = [vp, i]
     L.swpc_step.argtypes = [vp, i, i, i, i]
     L.swpc_step_timed.argtypes = [vp, i, i, fp]
@@ -187,6 +190,41 @@ class SwpcGPU:
         self._chk(self.L.swpc_set_receivers(self.h, int(ist.size), ist, jst, kst, int(what), int(ntdec_w), int(ntw),
                                             float(M0), float(UC)))
         self.nst, self.ntw = int(ist.size), int(ntw)
+
+    def set_green(self, ig, jg, kg, isrc, jsrc, ksrc, cmp, bforce, ntdec_w, ntw, stf):
+        """Reciprocity Green's-function mode (m_green.f90): GF cells
+        (ig, jg, kg), virtual source cell, force direction cmp in 'xyz' (None
+        switches the mode off), body-force components, decimation, samples
+        and the moment rate stf[it-1] at t = tbeg + it*dt."""
+        if cmp is None:
+            z = np.zeros(1, np.int32)
+            self._chk(self.L.swpc_set_green(self.h, 0, z, z, z, 1, 1, 1, -1, 0, 1, 1, 0, np.zeros(1, np.float32)))
+            self.green_shape = None
+            return
+        ig, jg, kg = _i32(ig), _i32(jg), _i32(kg)
+        stf = _f32(stf).reshape(-1)
+        ng = int(ig.size)
+        pad = np.zeros(1, np.int32)
+        self._chk(self.L.swpc_set_green(self.h, ng, ig if ng else pad, jg if ng else pad, kg if ng else pad,
+                                        int(isrc), int(jsrc), int(ksrc), "xyz".index(cmp), 1 if bforce else 0,
+                                        int(ntdec_w), int(ntw), int(stf.size),
+                                        stf if stf.size else np.zeros(1, np.float32)))
+        self.green_shape = (ng * (9 if bforce else 6), int(ntw))
+
+    def green(self):
+        """gf(ntw, ncmp*ng) as a (ncmp*ng, ntw) array: row n*ncmp + c is
+        component c (mxx myy mzz myz mxz mxy [fx fy fz]) of GF cell n, before
+        the export's sign flip for cmp = 'z'"""
+        out = np.empty(self.green_shape, np.float32)
+        if out.size:
+            self._chk(self.L.swpc_get_green(self.h, out))
+        return out
+
+    def green_store(self, it):
+        self._chk(self.L.swpc_green_store(self.h, int(it)))
+
+    def green_source(self, it):
+        self._chk(self.L.swpc_green_source(self.h, int(it)))
 
     # ---- time loop, routine for routine ----
     def kernel_update_stress(self):

@@ -57,8 +57,51 @@ def host_lib() -> C.CDLL:
     L.swpch_momentrate.argtypes = [f, i, f, f]
     L.swpch_stf_table.argtypes = [i, i, i, fp, fp, f, f, f, fp]
     L.swpch_station_index.argtypes = [i, fp, fp, fp, ip, i, i, i, f, f, f, f, f, f, ip, ip, ip, ip, ip, ip]
+    L.swpch_green_points.argtypes = [i, fp, fp, fp, f, f, f, i, i, i, f, f, f, f, f, f, ip, ip, ip, ip, ip]
     _h = L
     return L
+
+
+@dataclass
+class GreenSpec:
+    """The green_* parameters of input.inf (src/swpc_3d/m_green.f90:117-135).
+
+    station: index into ModelSpec.stations of the virtual source (green_stnm);
+    points: (ng, 3) x, y, z of the GF points (fn_glst with green_fmt = 'xyz');
+    gid: their ids."""
+    station: int
+    points: np.ndarray
+    gid: np.ndarray | None = None
+    cmp: str = "z"              # green_cmp
+    trise: float = 1.0          # green_trise
+    bforce: bool = False        # green_bforce
+    maxdist: float = 1e30       # green_maxdist
+
+
+def read_glst(path: str):
+    """fn_glst in green_fmt = 'xyz': 'x y z gid' per line, '#' comments and
+    blank lines skipped (m_green.f90:196-215).  Returns (points, gid)."""
+    pts, gid = [], []
+    with open(path) as fh:
+        for line in fh:
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            w = t.split()
+            if len(w) < 4:
+                raise ValueError(f"{path}: expected 'x y z gid', got {t!r}")
+            g = int(w[3])
+            if not 0 <= g <= 99999999:
+                raise ValueError(f"{path}: green id {g} out of range")
+            pts.append([float(w[0]), float(w[1]), float(w[2])])
+            gid.append(g)
+    return np.array(pts, np.float32).reshape(-1, 3), np.array(gid, np.int64)
+
+
+def green_export(gf: np.ndarray, cmp: str) -> np.ndarray:
+    """Green's functions as green__export writes them: positive upward for
+    a vertical virtual source (m_green.f90:563-566)."""
+    return -gf if cmp == "z" else gf
 
 
 @dataclass
@@ -91,6 +134,7 @@ class ModelSpec:
     stations: list = field(default_factory=list)   # (x, y, z, zsw)
     ntdec_w: int = 1
     wav_what: int = 1
+    green: GreenSpec | None = None   # reciprocity Green's-function mode (green_mode)
 
     def __post_init__(self):
         if self.xbeg is None:
@@ -134,8 +178,8 @@ def build_inputs(spec: ModelSpec) -> Inputs:
     d2 = C.c_float()
     L.swpch_kernel_coef(nm, ts, spec.dt, c1, c2, d1, C.byref(d2))
 
-    # sources
-    srcs = spec.sources
+    # sources; green_mode has none (m_source.f90:84-90)
+    srcs = [] if spec.green is not None else spec.sources
     nsrc = len(srcs)
     dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
     isrc = np.array([L.swpch_x2i(s["x"], spec.xbeg, dx32) for s in srcs], np.int32)
@@ -206,7 +250,44 @@ def build_inputs(spec: ModelSpec) -> Inputs:
     ntw = int(math.floor(float(f32(spec.nt - 1) / f32(spec.ntdec_w) + f32(1.0))))
     scalars = dict(zeta=float(zeta.value), d2=float(d2.value), vmin=float(vmin.value), vmax=float(vmax.value),
                    fcut=float(fcut), M0=float(M0), UC=float(UC), ntw=ntw, nsrc=nsrc)
+    if spec.green is not None:
+        _green_inputs(spec, arrays, scalars, kob)
     return Inputs(spec=spec, arrays=arrays, scalars=scalars)
+
+
+def _green_inputs(spec: ModelSpec, arrays: dict, scalars: dict, kob: np.ndarray) -> None:
+    """green__setup (m_green.f90:66-404): M0 = 1, no regular source (the PML
+    cut-off frequency stays 0), no station output, the virtual source at the
+    station's cell, GF cells filtered by distance and solid part."""
+    L = host_lib()
+    gs = spec.green
+    f32 = np.float32
+    if gs.cmp not in ("x", "y", "z"):
+        raise ValueError("no matching green_cmp")
+    st = spec.stations[gs.station]
+    xs = np.array([st[0]], f32); ys = np.array([st[1]], f32); zs = np.array([st[2]], f32)
+    zsw = np.array([ZSW[st[3]]], np.int32)
+    ist, jst, kst, inside = (np.zeros(1, np.int32) for _ in range(4))
+    dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
+    L.swpch_station_index(1, xs, ys, zs, zsw, spec.nx, spec.ny, spec.nz, spec.xbeg, spec.ybeg, spec.zbeg,
+                          dx32, dy32, dz32, arrays["kfs"], kob, ist, jst, kst, inside)
+    if not inside[0]:
+        raise ValueError("green_stnm: the virtual source station is outside the domain")
+    pts = np.ascontiguousarray(np.asarray(gs.points, f32).reshape(-1, 3))
+    ng0 = pts.shape[0]
+    ig, jg, kg, keep = (np.zeros(max(ng0, 1), np.int32) for _ in range(4))
+    if ng0:
+        L.swpch_green_points(ng0, np.ascontiguousarray(pts[:, 0]), np.ascontiguousarray(pts[:, 1]),
+                             np.ascontiguousarray(pts[:, 2]), xs[0], ys[0], f32(gs.maxdist), spec.nx, spec.ny,
+                             spec.nz, spec.xbeg, spec.ybeg, spec.zbeg, dx32, dy32, dz32, kob, ig, jg, kg, keep)
+    sel = keep[:ng0].astype(bool)
+    gid = np.arange(ng0) if gs.gid is None else np.asarray(gs.gid)
+    stf = np.zeros(spec.nt, f32)
+    L.swpch_stf_table(spec.nt, 1, STF_CODES[spec.stftype], np.zeros(1, f32), np.array([gs.trise], f32),
+                      spec.tbeg, spec.dt, 0.0, stf)
+    arrays.update(green_ig=ig[:ng0][sel].copy(), green_jg=jg[:ng0][sel].copy(), green_kg=kg[:ng0][sel].copy(),
+                  green_gid=gid[sel].copy(), green_stf=stf)
+    scalars.update(M0=1.0, green_src=(int(ist[0]), int(jst[0]), int(kst[0])))
 
 
 def make_simulation(inp: Inputs, device: int = 0):
@@ -223,6 +304,9 @@ def make_simulation(inp: Inputs, device: int = 0):
         g.set_cerjan(a["cgx_c"], a["cgx_b"], a["cgy_c"], a["cgy_b"], a["cgz_c"], a["cgz_b"])
     if sc["nsrc"]:
         g.set_sources(a["isrc"], a["jsrc"], a["ksrc"], a["mo"], a["comp"], sc["M0"], a["stf"], bf_mode=s.bf_mode)
-    if a["ist"].size:
+    if a["ist"].size and s.green is None:
         g.set_receivers(a["ist"], a["jst"], a["kst"], s.wav_what, s.ntdec_w, sc["ntw"], sc["M0"], sc["UC"])
+    if s.green is not None:
+        g.set_green(a["green_ig"], a["green_jg"], a["green_kg"], *sc["green_src"], s.green.cmp, s.green.bforce,
+                    s.ntdec_w, sc["ntw"], a["green_stf"])
     return g

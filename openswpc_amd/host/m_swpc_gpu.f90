@@ -189,6 +189,30 @@ module m_swpc_gpu
             integer(c_int), value :: kind
             real(c_float), intent(out) :: out(*)
         end function
+        !! reciprocity Green's-function mode (src/swpc_3d/m_green.f90)
+        integer(c_int) function swpc_set_green(ctx, ng, ig, jg, kg, isrc, jsrc, ksrc, cmp, bforce, ntdec_w, ntw, &
+                                               nt, stf) bind(C, name='swpc_set_green')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ng, isrc, jsrc, ksrc, cmp, bforce, ntdec_w, ntw, nt
+            integer(c_int), intent(in) :: ig(*), jg(*), kg(*)
+            real(c_float), intent(in) :: stf(*)
+        end function
+        integer(c_int) function swpc_green_store(ctx, it) bind(C, name='swpc_green_store')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+        integer(c_int) function swpc_green_source(ctx, it) bind(C, name='swpc_green_source')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+        integer(c_int) function swpc_get_green(ctx, out) bind(C, name='swpc_get_green')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: out(*)
+        end function
         integer(c_int) function swpc_adj_gradient(ctx, nt, obs, ckp, misfit, g_lam, g_mu, g_rho) &
             bind(C, name='swpc_adj_gradient')
             import :: c_int, c_ptr, c_float, c_double

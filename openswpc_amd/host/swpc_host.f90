@@ -541,4 +541,32 @@ contains
         end do
     end subroutine swpch_station_index
 
+    !! GF cells of green__setup for 'xyz' point lists
+    !! (src/swpc_3d/m_green.f90:225-296): a point is kept when its horizontal
+    !! distance from the virtual source (the station) is within maxdist and its
+    !! cell lies in the grid and in the solid part, kob(i,j) <= k <= nz
+    subroutine swpch_green_points(ng, x, y, z, xsrc, ysrc, maxdist, nx, ny, nz, xbeg, ybeg, zbeg, dx, dy, dz, kob, &
+                                  ig, jg, kg, keep) bind(C, name='swpch_green_points')
+        integer(c_int), value :: ng, nx, ny, nz
+        real(c_float), intent(in) :: x(ng), y(ng), z(ng)
+        real(c_float), value :: xsrc, ysrc, maxdist, xbeg, ybeg, zbeg, dx, dy, dz
+        integer(c_int), intent(in) :: kob(-2:nx + 3, -2:ny + 3)
+        integer(c_int), intent(out) :: ig(ng), jg(ng), kg(ng), keep(ng)
+        integer :: n, ii, jj, kk
+        real(SP) :: dd
+        do n = 1, ng
+            keep(n) = 0
+            ig(n) = 0; jg(n) = 0; kg(n) = 0
+            dd = sqrt((x(n) - xsrc)**2 + (y(n) - ysrc)**2)
+            if (dd > maxdist) cycle
+            ii = swpch_x2i(x(n), xbeg, dx)
+            jj = swpch_x2i(y(n), ybeg, dy)
+            kk = swpch_x2i(z(n), zbeg, dz)
+            ig(n) = ii; jg(n) = jj; kg(n) = kk
+            if (1 <= ii .and. ii <= nx .and. 1 <= jj .and. jj <= ny) then
+                if (kob(ii, jj) <= kk .and. kk <= nz) keep(n) = 1
+            end if
+        end do
+    end subroutine swpch_green_points
+
 end module swpc_host

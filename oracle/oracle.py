@@ -90,6 +90,8 @@ def lib(prec: int = 32) -> C.CDLL:
     L.orc_wav_store.argtypes = [vp, i, i, _i32p, _i32p, _i32p, i, i, i, f, f, vp, vp, vp, vp, vp, vp]
     L.orc_kernel_vmax.argtypes = [vp, _f32p]
     L.orc_snap_write.argtypes = [vp, i, _i32p, _i32p, i, i, i, i, i, i, i, f, f, vp, vp, vp, vp]
+    L.orc_green_store.argtypes = [vp, i, i, _i32p, _i32p, _i32p, i, i, i, vp, vp]
+    L.orc_green_source.argtypes = [vp, i, i, i, f, f, f, f]
     L.orc_prec.restype = i
     _LIBS[prec] = L
     return L
@@ -454,6 +456,7 @@ class OracleRun:
         m, L, p = self.m, self.L, self.ptr
         self.it += 1
         it = self.it
+        self.green_store(it)
         self.wav_store(it)
         self.snap_write(it)
         L.orc_kernel_update_stress(p)
@@ -468,6 +471,38 @@ class OracleRun:
             st = np.ascontiguousarray(m.stf[it - 1])
             L.orc_source_bodyforce(p, m.nsrc, m.isrc, m.jsrc, m.ksrc, np.ascontiguousarray(m.mij[0]),
                                    np.ascontiguousarray(m.mij[1]), np.ascontiguousarray(m.mij[2]), m.M0, st)
+        self.green_source(it)
+
+    # ---- reciprocity Green's functions (m_green.f90) ----
+    GREEN_CMP = {"x": (1.0, 0.0, 0.0), "y": (0.0, 1.0, 0.0), "z": (0.0, 0.0, 1.0)}
+
+    def green_setup(self, ig, jg, kg, isrc, jsrc, ksrc, cmp, bforce, ntdec_w, ntw, stf):
+        """GF points (ig, jg, kg); virtual source cell (isrc, jsrc, ksrc) and
+        its unit force direction cmp in 'xyz' (m_green.f90:136-143); stf[it-1]
+        is the moment rate at t = tbeg + it*dt (m_green.f90:632)."""
+        self.green = dict(ig=np.ascontiguousarray(ig, np.int32), jg=np.ascontiguousarray(jg, np.int32),
+                          kg=np.ascontiguousarray(kg, np.int32), src=(int(isrc), int(jsrc), int(ksrc)),
+                          f=self.GREEN_CMP[cmp], bforce=int(bool(bforce)), ntdec_w=int(ntdec_w), ntw=int(ntw),
+                          stf=np.ascontiguousarray(stf, np.float32))
+        ng = self.green["ig"].size
+        ncmp = 9 if bforce else 6
+        self.green_acc = np.zeros(12 * max(ng, 1), self.sp)
+        self.gf = np.zeros((max(ng, 1) * ncmp, int(ntw)), self.sp)    # gf(ntw, ncmp*ng), Fortran order
+
+    def green_store(self, it):
+        if not hasattr(self, "green"):
+            return
+        g = self.green
+        self.L.orc_green_store(self.ptr, it, g["ig"].size, g["ig"], g["jg"], g["kg"], g["bforce"], g["ntdec_w"],
+                               g["ntw"], self.green_acc.ctypes.data, self.gf.ctypes.data)
+
+    def green_source(self, it):
+        if not hasattr(self, "green"):
+            return
+        g = self.green
+        if it > g["stf"].size:
+            return
+        self.L.orc_green_source(self.ptr, *g["src"], *g["f"], float(g["stf"][it - 1]))
 
     def wav_store(self, it):
         m = self.m
@@ -530,6 +565,17 @@ class OracleRun:
         out = np.zeros(3, np.float32)
         self.L.orc_kernel_vmax(self.ptr, out)
         return out
+
+
+def green_stf(cfg: Config, trise: float, nt: int | None = None) -> np.ndarray:
+    """Moment rate of the virtual source at t = tbeg + it*dt, it = 1..nt,
+    start time 0 and rise time trise (m_green.f90:25,632)."""
+    nt = cfg.nt if nt is None else nt
+    out = np.zeros(nt, np.float32)
+    t0 = np.zeros(1, np.float32)
+    tr = np.array([trise], np.float32)
+    lib(32).orc_stf_table(nt, 1, STF_CODES[cfg.stftype], t0, tr, cfg.tbeg, cfg.dt, 0.0, out)
+    return out
 
 
 def interior(a: np.ndarray) -> np.ndarray:

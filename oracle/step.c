@@ -20,13 +20,20 @@
  *         no-damping control (tests/test_oracle_physics.py).
  *   orc_source_bodyforce - the point-force closed form above.
  *   orc_source_stressglut, shear components (the /4 split onto the four
- *       staggered neighbours) - mirror symmetry pins their operand pairing;
- *       their amplitude is parity unpinned beyond reading.
+ *       staggered neighbours), orc_green_store, orc_green_source and the
+ *       velocity and displacement samples of orc_wav_store - exact discrete
+ *       reciprocity in FP64 (tests/test_oracle_green.py): each of the six
+ *       moment-tensor components and three forces at a cell, recorded at a
+ *       station, equals the Green's function stored at that cell by the
+ *       reciprocal run forced at the station, to 1e-11, with a control
+ *       that a misplaced shear split is detected; mirror symmetry pins the
+ *       shear operand pairing as well.
  *   orc_absorb_c_update_* - the taper profile by its closed form
  *       (tests/test_oracle_setup.py); the multiply order of its application
  *       is parity unpinned beyond reading.
- *   orc_wav_store - parity unpinned beyond reading (checked against the GPU
- *       only); the same sampling conventions are pinned for orc_snap_write.
+ *   orc_wav_store stress and strain samples - parity unpinned beyond reading
+ *       (checked against the GPU only); the same sampling conventions are
+ *       pinned for orc_snap_write.
  *   orc_snap_write - linear-field closed forms, index sampling and
  *       accumulation checks (tests/test_oracle_snap.py).
  */
@@ -754,4 +761,140 @@ void orc_snap_write(orc_state *s, int it, const int32_t *kfs, const int32_t *sw,
                 }
         }
     }
+}
+
+/* ---- reciprocity Green's-function mode (src/swpc_3d/m_green.f90) ---- */
+
+/* UC_DERIV = 10.0**(-15) and UC_BF = 10.0**(-12), single-precision constants
+   as the Fortran compiler folds them (m_green.f90:30-31) */
+#define ORC_UC_DERIV 0x1.203afcp-50f
+#define ORC_UC_BF 0x1.197998p-40f
+
+/*
+ * green__store, src/swpc_3d/m_green.f90:406-545.  acc holds the twelve
+ * accumulators component-major, (12, ng): dxUx dxUy dxUz dyUx dyUy dyUz
+ * dzUx dzUy dzUz Ux Uy Uz (all zero before it == 1, m_green.f90:424-441).
+ * Each shear derivative is the mean of the four staggered 4th-order
+ * derivatives around the cell (m_green.f90:458-490); r40 = C40/d and
+ * r41 = C41/d with C40 = 9/8, C41 = 1/24 (m_green.f90:43-45,110-115).
+ * gf is (ntw, ncmp*ng) in Fortran order with ncmp = 9 when bforce is set,
+ * else 6; column (n*ncmp + c) holds mxx myy mzz myz mxz mxy [fx fy fz]
+ * (m_green.f90:511-541).
+ */
+void orc_green_store(orc_state *s, int it, int ng, const int32_t *ig, const int32_t *jg, const int32_t *kg,
+                     int bforce, int ntdec_w, int ntw, sp_t *acc, sp_t *gf)
+{
+    const mp_t C40 = (mp_t)9.0 / (mp_t)8.0, C41 = (mp_t)1.0 / (mp_t)24.0;
+    const mp_t r40x = C40 / (mp_t)s->dx, r40y = C40 / (mp_t)s->dy, r40z = C40 / (mp_t)s->dz;
+    const mp_t r41x = C41 / (mp_t)s->dx, r41y = C41 / (mp_t)s->dy, r41z = C41 / (mp_t)s->dz;
+    const sp_t dt = (sp_t)s->dt;
+    const int ncmp = bforce ? 9 : 6;
+    for (int n = 0; n < ng; n++) {
+        const int ii = ig[n], jj = jg[n], kk = kg[n];
+#define VX(k_, i_, j_) V3(s->Vx, k_, i_, j_)
+#define VY(k_, i_, j_) V3(s->Vy, k_, i_, j_)
+#define VZ(k_, i_, j_) V3(s->Vz, k_, i_, j_)
+        const mp_t dxVx = (VX(kk, ii, jj) - VX(kk, ii - 1, jj)) * r40x - (VX(kk, ii + 1, jj) - VX(kk, ii - 2, jj)) * r41x;
+        const mp_t dyVy = (VY(kk, ii, jj) - VY(kk, ii, jj - 1)) * r40y - (VY(kk, ii, jj + 1) - VY(kk, ii, jj - 2)) * r41y;
+        const mp_t dzVz = (VZ(kk, ii, jj) - VZ(kk - 1, ii, jj)) * r40z - (VZ(kk + 1, ii, jj) - VZ(kk - 2, ii, jj)) * r41z;
+
+        const mp_t dxVy1 = (VY(kk, ii + 1, jj) - VY(kk, ii, jj)) * r40x - (VY(kk, ii + 2, jj) - VY(kk, ii - 1, jj)) * r41x;
+        const mp_t dxVy2 = (VY(kk, ii + 1, jj - 1) - VY(kk, ii, jj - 1)) * r40x - (VY(kk, ii + 2, jj - 1) - VY(kk, ii - 1, jj - 1)) * r41x;
+        const mp_t dxVy3 = (VY(kk, ii, jj) - VY(kk, ii - 1, jj)) * r40x - (VY(kk, ii + 1, jj) - VY(kk, ii - 2, jj)) * r41x;
+        const mp_t dxVy4 = (VY(kk, ii, jj - 1) - VY(kk, ii - 1, jj - 1)) * r40x - (VY(kk, ii + 1, jj - 1) - VY(kk, ii - 2, jj - 1)) * r41x;
+
+        const mp_t dxVz1 = (VZ(kk, ii + 1, jj) - VZ(kk, ii, jj)) * r40x - (VZ(kk, ii + 2, jj) - VZ(kk, ii - 1, jj)) * r41x;
+        const mp_t dxVz2 = (VZ(kk - 1, ii + 1, jj) - VZ(kk - 1, ii, jj)) * r40x - (VZ(kk - 1, ii + 2, jj) - VZ(kk - 1, ii - 1, jj)) * r41x;
+        const mp_t dxVz3 = (VZ(kk, ii, jj) - VZ(kk, ii - 1, jj)) * r40x - (VZ(kk, ii + 1, jj) - VZ(kk, ii - 2, jj)) * r41x;
+        const mp_t dxVz4 = (VZ(kk - 1, ii, jj) - VZ(kk - 1, ii - 1, jj)) * r40x - (VZ(kk - 1, ii + 1, jj) - VZ(kk - 1, ii - 2, jj)) * r41x;
+
+        const mp_t dyVx1 = (VX(kk, ii, jj + 1) - VX(kk, ii, jj)) * r40y - (VX(kk, ii, jj + 2) - VX(kk, ii, jj - 1)) * r41y;
+        const mp_t dyVx2 = (VX(kk, ii - 1, jj + 1) - VX(kk, ii - 1, jj)) * r40y - (VX(kk, ii - 1, jj + 2) - VX(kk, ii - 1, jj - 1)) * r41y;
+        const mp_t dyVx3 = (VX(kk, ii, jj) - VX(kk, ii, jj - 1)) * r40y - (VX(kk, ii, jj + 1) - VX(kk, ii, jj - 2)) * r41y;
+        const mp_t dyVx4 = (VX(kk, ii - 1, jj) - VX(kk, ii - 1, jj - 1)) * r40y - (VX(kk, ii - 1, jj + 1) - VX(kk, ii - 1, jj - 2)) * r41y;
+
+        const mp_t dyVz1 = (VZ(kk, ii, jj + 1) - VZ(kk, ii, jj)) * r40y - (VZ(kk, ii, jj + 2) - VZ(kk, ii, jj - 1)) * r41y;
+        const mp_t dyVz2 = (VZ(kk - 1, ii, jj + 1) - VZ(kk - 1, ii, jj)) * r40y - (VZ(kk - 1, ii, jj + 2) - VZ(kk - 1, ii, jj - 1)) * r41y;
+        const mp_t dyVz3 = (VZ(kk, ii, jj) - VZ(kk, ii, jj - 1)) * r40y - (VZ(kk, ii, jj + 1) - VZ(kk, ii, jj - 2)) * r41y;
+        const mp_t dyVz4 = (VZ(kk - 1, ii, jj) - VZ(kk - 1, ii, jj - 1)) * r40y - (VZ(kk - 1, ii, jj + 1) - VZ(kk - 1, ii, jj - 2)) * r41y;
+
+        const mp_t dzVx1 = (VX(kk + 1, ii, jj) - VX(kk, ii, jj)) * r40z - (VX(kk + 2, ii, jj) - VX(kk - 1, ii, jj)) * r41z;
+        const mp_t dzVx2 = (VX(kk + 1, ii - 1, jj) - VX(kk, ii - 1, jj)) * r40z - (VX(kk + 2, ii - 1, jj) - VX(kk - 1, ii - 1, jj)) * r41z;
+        const mp_t dzVx3 = (VX(kk, ii, jj) - VX(kk - 1, ii, jj)) * r40z - (VX(kk + 1, ii, jj) - VX(kk - 2, ii, jj)) * r41z;
+        const mp_t dzVx4 = (VX(kk, ii - 1, jj) - VX(kk - 1, ii - 1, jj)) * r40z - (VX(kk + 1, ii - 1, jj) - VX(kk - 2, ii - 1, jj)) * r41z;
+
+        const mp_t dzVy1 = (VY(kk + 1, ii, jj) - VY(kk, ii, jj)) * r40z - (VY(kk + 2, ii, jj) - VY(kk - 1, ii, jj)) * r41z;
+        const mp_t dzVy2 = (VY(kk + 1, ii, jj - 1) - VY(kk, ii, jj - 1)) * r40z - (VY(kk + 2, ii, jj - 1) - VY(kk - 1, ii, jj - 1)) * r41z;
+        const mp_t dzVy3 = (VY(kk, ii, jj) - VY(kk - 1, ii, jj)) * r40z - (VY(kk + 1, ii, jj) - VY(kk - 2, ii, jj)) * r41z;
+        const mp_t dzVy4 = (VY(kk, ii, jj - 1) - VY(kk - 1, ii, jj - 1)) * r40z - (VY(kk + 1, ii, jj - 1) - VY(kk - 2, ii, jj - 1)) * r41z;
+
+        const mp_t quarter = (mp_t)(sp_t)0.25;
+        const mp_t dxVy = (dxVy1 + dxVy2 + dxVy3 + dxVy4) * quarter;
+        const mp_t dxVz = (dxVz1 + dxVz2 + dxVz3 + dxVz4) * quarter;
+        const mp_t dyVx = (dyVx1 + dyVx2 + dyVx3 + dyVx4) * quarter;
+        const mp_t dyVz = (dyVz1 + dyVz2 + dyVz3 + dyVz4) * quarter;
+        const mp_t dzVx = (dzVx1 + dzVx2 + dzVx3 + dzVx4) * quarter;
+        const mp_t dzVy = (dzVy1 + dzVy2 + dzVy3 + dzVy4) * quarter;
+
+        const mp_t d[9] = {dxVx, dxVy, dxVz, dyVx, dyVy, dyVz, dzVx, dzVy, dzVz};
+        for (int q = 0; q < 9; q++)
+            acc[(int64_t)q * ng + n] = acc[(int64_t)q * ng + n] + (sp_t)(dr_t)(d[q] * (mp_t)dt);
+
+        if (bforce) {
+            sp_t *u = acc + (int64_t)9 * ng;
+            u[n] = u[n] + (sp_t)0.5 * (sp_t)(dr_t)(VX(kk, ii, jj) + VX(kk, ii - 1, jj)) * dt;
+            u[(int64_t)ng + n] = u[(int64_t)ng + n] + (sp_t)0.5 * (sp_t)(dr_t)(VY(kk, ii, jj) + VY(kk, ii, jj - 1)) * dt;
+            u[(int64_t)2 * ng + n] = u[(int64_t)2 * ng + n] + (sp_t)0.5 * (sp_t)(dr_t)(VZ(kk, ii, jj) + VZ(kk - 1, ii, jj)) * dt;
+        }
+#undef VX
+#undef VY
+#undef VZ
+    }
+
+    if ((it - 1) % ntdec_w != 0) return;
+    const int itw = (it - 1) / ntdec_w;
+    if (itw < 0 || itw >= ntw) return;
+    const sp_t ucd = (sp_t)ORC_UC_DERIV, ucb = (sp_t)ORC_UC_BF, e9 = (sp_t)1e9f;
+    for (int n = 0; n < ng; n++) {
+#define A(q) acc[(int64_t)(q) * ng + n]
+#define GF(c) gf[((int64_t)n * ncmp + (c)) * ntw + itw]
+        GF(0) = A(0) * ucd * e9;
+        GF(1) = A(4) * ucd * e9;
+        GF(2) = A(8) * ucd * e9;
+        GF(3) = (A(5) + A(7)) * ucd * e9;
+        GF(4) = (A(2) + A(6)) * ucd * e9;
+        GF(5) = (A(1) + A(3)) * ucd * e9;
+        if (bforce) {
+            GF(6) = A(9) * ucb * e9;
+            GF(7) = A(10) * ucb * e9;
+            GF(8) = A(11) * ucb * e9;
+        }
+#undef A
+#undef GF
+    }
+}
+
+/*
+ * green__source, src/swpc_3d/m_green.f90:614-649: the unit force
+ * (fx1, fy1, fz1) times dt_dxyz and the moment rate stf (taken at
+ * t = tbeg + it*dt, m_green.f90:632) split onto the two staggered velocity
+ * nodes around the station cell, each with the averaged buoyancy of its two
+ * cells.  dt_dxyz = real(dt/(dx*dy*dz)) is single precision
+ * (m_green.f90:33,149).  All six updates are made, zero ones included.
+ */
+void orc_green_source(orc_state *s, int isrc, int jsrc, int ksrc, float fx1, float fy1, float fz1, float stf)
+{
+    const sp_t dt_dxyz = (sp_t)(dr_t)((mp_t)s->dt / ((mp_t)s->dx * (mp_t)s->dy * (mp_t)s->dz));
+    const sp_t st = (sp_t)stf;
+    const sp_t fx = (sp_t)fx1 * dt_dxyz * st;
+    const sp_t fy = (sp_t)fy1 * dt_dxyz * st;
+    const sp_t fz = (sp_t)fz1 * dt_dxyz * st;
+    const int ii = isrc, jj = jsrc, kk = ksrc;
+    const sp_t r0 = V3(s->rho, kk, ii, jj);
+    V3(s->Vx, kk, ii, jj) = V3(s->Vx, kk, ii, jj) + ((sp_t)2.0 / (r0 + V3(s->rho, kk, ii + 1, jj))) * fx / 2;
+    V3(s->Vx, kk, ii - 1, jj) = V3(s->Vx, kk, ii - 1, jj) + ((sp_t)2.0 / (r0 + V3(s->rho, kk, ii - 1, jj))) * fx / 2;
+    V3(s->Vy, kk, ii, jj) = V3(s->Vy, kk, ii, jj) + ((sp_t)2.0 / (r0 + V3(s->rho, kk, ii, jj + 1))) * fy / 2;
+    V3(s->Vy, kk, ii, jj - 1) = V3(s->Vy, kk, ii, jj - 1) + ((sp_t)2.0 / (r0 + V3(s->rho, kk, ii, jj - 1))) * fy / 2;
+    V3(s->Vz, kk, ii, jj) = V3(s->Vz, kk, ii, jj) + ((sp_t)2.0 / (r0 + V3(s->rho, kk + 1, ii, jj))) * fz / 2;
+    V3(s->Vz, kk - 1, ii, jj) = V3(s->Vz, kk - 1, ii, jj) + ((sp_t)2.0 / (r0 + V3(s->rho, kk - 1, ii, jj))) * fz / 2;
 }

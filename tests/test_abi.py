@@ -35,7 +35,8 @@ def test_header_declares_the_time_loop():
     names = declared_functions(HEADER)
     for want in ("swpc_create", "swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_source_stressglut",
                  "swpc_kernel_update_vel", "swpc_absorb_update_vel", "swpc_source_bodyforce", "swpc_wav_store",
-                 "swpc_step", "swpc_kernel_vmax", "swpc_get_seismograms"):
+                 "swpc_step", "swpc_kernel_vmax", "swpc_get_seismograms", "swpc_set_green", "swpc_green_store",
+                 "swpc_green_source", "swpc_get_green"):
         assert want in names
 
 
@@ -79,8 +80,12 @@ def test_null_context_is_rejected():
     for name in ("swpc_kernel_update_stress", "swpc_absorb_update_stress", "swpc_kernel_update_vel",
                  "swpc_absorb_update_vel", "swpc_update_stress", "swpc_update_vel", "swpc_synchronize"):
         assert getattr(L, name)(None) == -1, name
-    for name in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store"):
+    for name in ("swpc_source_stressglut", "swpc_source_bodyforce", "swpc_wav_store", "swpc_green_store",
+                 "swpc_green_source"):
         assert getattr(L, name)(None, 1) == -1, name
+    z = np.zeros(1, np.int32)
+    assert L.swpc_set_green(None, 1, z, z, z, 1, 1, 1, 0, 0, 1, 1, 1, np.zeros(1, np.float32)) == -1
+    assert L.swpc_get_green(None, np.zeros(1, np.float32)) == -1
     assert L.swpc_step(None, 1, 1, 0, 1) == -1
     assert L.swpc_kernel_vmax(None, np.zeros(3, np.float32)) == -1
     assert L.swpc_destroy(None) == 0
