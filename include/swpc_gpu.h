@@ -239,6 +239,22 @@ int swpc_green_source(swpc_ctx *ctx, int it);
    caller's.  out holds ntw*ncmp*ng floats.  Synchronises. */
 int swpc_get_green(swpc_ctx *ctx, float *out);
 
+/* ---- random media (src/shared/m_rmedia.f90:40-173, rmedia__3dgen_ani) ----
+   A realisation of a 3D random medium on an n1 x n2 x n3 grid, axis 1
+   fastest in out (n1*n2*n3 floats in host memory): uniform noise, forward
+   FFT, whitening, filtering by the square root of the PSDF of type ptype
+   (1 Gaussian, 2 exponential, 3 von Karman with Hurst number kappa;
+   correlation lengths a1..a3, spacings d1..d3), zero mean, rms scaled to
+   epsil.  The reference's media(nx, ny, nz) is (n1, n2, n3) = (nx, ny, nz);
+   a (k,i,j) array is (nz, nx, ny).  The noise is a counter-based stream of
+   seed, so equal arguments give bit-identical fields on any launch shape;
+   any sizes (the reference's FFT needs powers of two).  Runs on device
+   `device` and synchronises; swpc_rmedia_last_error() gives the text of a
+   failure of the calling thread. */
+int swpc_rmedia_3d(int device, int ptype, int n1, int n2, int n3, float d1, float d2, float d3, float a1,
+                   float a2, float a3, float epsil, float kappa, uint64_t seed, float *out);
+const char *swpc_rmedia_last_error(void);
+
 /* ---- batches: many independent simulations stepped together ----
    The reference runs one simulation per process; a batch of Green's
    functions (one run per source component and station, the workload of

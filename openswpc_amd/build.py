@@ -14,7 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libswpc_gpu.so")
-SOURCES = ["api.hip", "stencil.hip", "aux.hip", "snap.hip", "green.hip", "batch.hip", "adjoint.hip"]
+SOURCES = ["api.hip", "stencil.hip", "aux.hip", "snap.hip", "green.hip", "batch.hip", "adjoint.hip", "rmedia.hip"]
+# hipFFT for the random-media generator
+LIBS = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SWPC_OFFLOAD_ARCH", "gfx950")
 
@@ -29,7 +31,7 @@ FLAGS = [
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES], *LIBS]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

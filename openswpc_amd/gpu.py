@@ -107,6 +107,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_stream.argtypes = [vp]
     L.swpc_stream.restype = vp
     L.swpc_adj_gradient.argtypes = [vp, i, fp, i, C.POINTER(C.c_double), vp, vp, vp]
+    L.swpc_rmedia_3d.argtypes = [i, i, i, i, i, f, f, f, f, f, f, f, f, C.c_uint64, fp]
+    L.swpc_rmedia_last_error.argtypes = []
+    L.swpc_rmedia_last_error.restype = C.c_char_p
     L.swpc_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
     L.swpc_batch_destroy.argtypes = [vp]
     L.swpc_batch_last_error.argtypes = [vp]
@@ -386,3 +389,23 @@ class SwpcBatch:
             self.close()
         except Exception:
             pass
+
+
+RMEDIA_TYPES = {"gauss": 1, "exp": 2, "vonkarman": 3}
+
+
+def rmedia_3d(ptype, shape, d, a, epsil, kappa=0.5, seed=1, device=0):
+    """Random medium on the device (m_rmedia.f90 rmedia__3dgen_ani).
+    shape = (n1, n2, n3) with axis 1 fastest, d and a per axis; returns a
+    float32 numpy array of shape (n3, n2, n1).  ptype: 1/2/3 or
+    'gauss'/'exp'/'vonkarman'."""
+    L = load_library()
+    pt = RMEDIA_TYPES.get(ptype, ptype)
+    n1, n2, n3 = (int(v) for v in shape)
+    out = np.empty((n3, n2, n1), np.float32)
+    rc = L.swpc_rmedia_3d(int(device), int(pt), n1, n2, n3, *(float(v) for v in d), *(float(v) for v in a),
+                          float(epsil), float(kappa), int(seed) & (2**64 - 1), out)
+    if rc != 0:
+        raise SwpcError(f"swpc_rmedia_3d failed ({rc}): {L.swpc_rmedia_last_error().decode()}")
+    return out
+

@@ -86,6 +86,9 @@ def test_null_context_is_rejected():
     z = np.zeros(1, np.int32)
     assert L.swpc_set_green(None, 1, z, z, z, 1, 1, 1, 0, 0, 1, 1, 1, np.zeros(1, np.float32)) == -1
     assert L.swpc_get_green(None, np.zeros(1, np.float32)) == -1
+    # argument checks of the context-free generator happen before any device call
+    assert L.swpc_rmedia_3d(0, 4, 8, 8, 8, 1, 1, 1, 1, 1, 1, 0.1, 0.5, 1, np.zeros(512, np.float32)) == -1
+    assert b"bad arguments" in L.swpc_rmedia_last_error()
     assert L.swpc_step(None, 1, 1, 0, 1) == -1
     assert L.swpc_kernel_vmax(None, np.zeros(3, np.float32)) == -1
     assert L.swpc_destroy(None) == 0
