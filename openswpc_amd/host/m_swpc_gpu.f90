@@ -213,6 +213,15 @@ module m_swpc_gpu
             type(c_ptr), value :: ctx
             real(c_float), intent(out) :: out(*)
         end function
+        !! random media (src/shared/m_rmedia.f90:40-173); media(n1, n2, n3)
+        integer(c_int) function swpc_rmedia_3d(device, ptype, n1, n2, n3, d1, d2, d3, a1, a2, a3, epsil, kappa, &
+                                               seed, media) bind(C, name='swpc_rmedia_3d')
+            import :: c_int, c_float, c_int64_t
+            integer(c_int), value :: device, ptype, n1, n2, n3
+            real(c_float), value :: d1, d2, d3, a1, a2, a3, epsil, kappa
+            integer(c_int64_t), value :: seed
+            real(c_float), intent(out) :: media(*)
+        end function
         integer(c_int) function swpc_adj_gradient(ctx, nt, obs, ckp, misfit, g_lam, g_mu, g_rho) &
             bind(C, name='swpc_adj_gradient')
             import :: c_int, c_ptr, c_float, c_double
