@@ -1,12 +1,18 @@
 /*
- * adjoint.c -- oracle restatement of the discrete adjoint of one elastic
- * time step (NM = 0, ADE-CFS PML, free-surface band).  TEST INFRASTRUCTURE
- * ONLY (see swpc_oracle.h).
+ * adjoint.c -- oracle restatement of the discrete adjoint of one time step
+ * (generalized Zener memory variables, ADE-CFS PML, free-surface band).
+ * TEST INFRASTRUCTURE ONLY (see swpc_oracle.h).
  *
  * The forward step (src/swpc_3d/main.f90:124-133, m_kernel.f90, m_absorb_p.f90)
  * is linear in the state for a fixed medium:
  *     S' = S + Ks(lam, mu)[V, aV],  aV' = g3 aV + g4 dt D V      (stress)
  *     V' = V + Kv(rho)[S', aS],     aS' = g3 aS + g4 dt D S'     (velocity)
+ * With NM mechanisms the interior stress update also advances the memory
+ * variables first (m_kernel.f90:222-238,319-333):
+ *     R'_c[m] = c1[m] R_c[m] - c2[m] E_c dt,   S'_c = S_c + (F_c + sum_m d1[m] R'_c[m]) dt
+ * with E_c, F_c linear in the derivatives (taup/taus and 1 + tau(1+d2)
+ * weights).  Its transpose: rho_c[m] = aR'_c[m] + d1[m] dt aS'_c, the new
+ * aR_c[m] = c1[m] rho_c[m], and E_c receives -dt sum_m c2[m] rho_c[m].
  * Its adjoint applies the transposes in reverse order.  Each forward
  * update reads spatial derivatives of the other field; the adjoint first
  * forms, per cell, the adjoint of every derivative the cell used (a
@@ -21,8 +27,10 @@
  * 2/(rho + rho').
  *
  * Parity status: pinned by the dot-product test (the adjoint is the
- * transpose of the forward step to rounding) and by central finite
- * differences of the misfit (tests/test_oracle_adjoint.py).
+ * transpose of the forward step to rounding, elastic and with three memory
+ * variables, with controls that dropped auxiliary or memory-variable
+ * adjoints are detected) and by central finite differences of the misfit,
+ * elastic and NM = 3 (tests/test_oracle_adjoint.py).
  */
 #include <math.h>
 #include <string.h>
@@ -37,7 +45,16 @@ typedef struct orc_adj {
     sp_t *aS[9];  /* adjoints of axSxx aySxy azSxz axSxy aySyy azSyz axSxz aySyz azSzz */
     mp_t *D[9];   /* per-cell derivative adjoints (scratch), (k,i,j) halo */
     double *g_lam, *g_mu, *g_rho; /* (k,i,j) halo */
+    sp_t *aR[6];  /* adjoints of Rxx Ryy Rzz Ryz Rxz Rxy, (m,k,i,j) on the kernel box */
 } orc_adj;
+
+/* memory-variable index (m,k,i,j) on the kernel box, m 1-based (m_kernel.f90:392-397) */
+static inline int64_t ridx(const orc_state *s, int m, int k, int i, int j)
+{
+    int64_t nzk = s->kend_k - s->kbeg_k + 1, nxk = s->iend_k - s->ibeg_k + 1;
+    return (int64_t)(m - 1) +
+           (int64_t)s->nm * ((k - s->kbeg_k) + nzk * ((i - s->ibeg_k) + nxk * (int64_t)(j - s->jbeg_k)));
+}
 
 static inline int64_t aidx(const orc_state *s, int k, int i, int j)
 {
@@ -274,16 +291,58 @@ void orc_adj_stress(const orc_state *s, orc_adj *a)
                     coef_at(s, 0, k, i, j, &ax, &bxx);
                     coef_at(s, 1, k, i, j, &ay, &byy);
                     coef_at(s, 2, k, i, j, &az, &bzz);
-                    V3(a->D[0], k, i, j) = X; V3(a->D[1], k, i, j) = Y; V3(a->D[2], k, i, j) = Z;
-                    V3(a->D[3], k, i, j) = Wyz; V3(a->D[4], k, i, j) = Wyz;
-                    V3(a->D[5], k, i, j) = Wxz; V3(a->D[6], k, i, j) = Wxz;
-                    V3(a->D[7], k, i, j) = Wxy; V3(a->D[8], k, i, j) = Wxy;
                     ex = BWD4(s->Vx, 0, 1, 0, ax, bxx);
                     ey = BWD4(s->Vy, 0, 0, 1, ay, byy);
                     ez = BWD4(s->Vz, 1, 0, 0, az, bzz);
                     tyz = FWD4(s->Vz, 0, 0, 1, ay, byy) + FWD4(s->Vy, 1, 0, 0, az, bzz);
                     txz = FWD4(s->Vz, 0, 1, 0, ax, bxx) + FWD4(s->Vx, 1, 0, 0, az, bzz);
                     txy = FWD4(s->Vy, 0, 1, 0, ax, bxx) + FWD4(s->Vx, 0, 0, 1, ay, byy);
+                    if (s->nm > 0) {
+                        /* transpose of the memory variables and of the weighted
+                           stress sums (m_kernel.f90:222-238,319-333) */
+                        const double taup = V3(s->taup, k, i, j), taus = V3(s->taus, k, i, j);
+                        const double d2 = (double)s->d2;
+                        const double tp1 = 1.0 + taup * (1.0 + d2), ts1 = 1.0 + taus * (1.0 + d2);
+                        const double as[6] = {sxx, syy, szz, (double)V3(a->Syz, k, i, j),
+                                              (double)V3(a->Sxz, k, i, j), (double)V3(a->Sxy, k, i, j)};
+                        double sig[6] = {0, 0, 0, 0, 0, 0};
+                        for (int m = 1; m <= s->nm; m++) {
+                            const double c1 = s->c1[m - 1], c2 = s->c2[m - 1], d1 = s->d1[m - 1];
+                            const int64_t r = ridx(s, m, k, i, j);
+                            for (int c = 0; c < 6; c++) {
+                                const double rh = (double)a->aR[c][r] + d1 * dt * as[c];
+                                a->aR[c][r] = (sp_t)(c1 * rh);
+                                sig[c] -= dt * c2 * rh;
+                            }
+                        }
+                        double P[3], Q[6];
+                        for (int c = 0; c < 6; c++) {
+                            if (c < 3) P[c] = dt * as[c] * tp1 + sig[c] * taup;
+                            Q[c] = dt * as[c] * ts1 + sig[c] * taus;
+                        }
+                        const double sP = P[0] + P[1] + P[2], l2m = lam + 2 * mu, d3 = ex + ey + ez;
+                        V3(a->D[0], k, i, j) = l2m * sP - 2 * mu * (Q[1] + Q[2]);
+                        V3(a->D[1], k, i, j) = l2m * sP - 2 * mu * (Q[0] + Q[2]);
+                        V3(a->D[2], k, i, j) = l2m * sP - 2 * mu * (Q[0] + Q[1]);
+                        V3(a->D[3], k, i, j) = hyz * Q[3]; V3(a->D[4], k, i, j) = hyz * Q[3];
+                        V3(a->D[5], k, i, j) = hxz * Q[4]; V3(a->D[6], k, i, j) = hxz * Q[4];
+                        V3(a->D[7], k, i, j) = hxy * Q[5]; V3(a->D[8], k, i, j) = hxy * Q[5];
+                        V3(a->g_lam, k, i, j) += d3 * sP;
+                        V3(a->g_mu, k, i, j) += 2 * d3 * sP - 2 * ((d3 - ex) * Q[0] + (d3 - ey) * Q[1] + (d3 - ez) * Q[2]);
+                        const double qyz = tyz * Q[3], qxz = txz * Q[4], qxy = txy * Q[5];
+                        V3(a->g_mu, k, i, j) += qyz * gyz[0] + qxz * gxz[0] + qxy * gxy[0];
+                        V3(a->g_mu, k + 1, i, j) += qyz * gyz[1] + qxz * gxz[1];
+                        V3(a->g_mu, k, i, j + 1) += qyz * gyz[2] + qxy * gxy[2];
+                        V3(a->g_mu, k + 1, i, j + 1) += qyz * gyz[3];
+                        V3(a->g_mu, k, i + 1, j) += qxz * gxz[2] + qxy * gxy[1];
+                        V3(a->g_mu, k + 1, i + 1, j) += qxz * gxz[3];
+                        V3(a->g_mu, k, i + 1, j + 1) += qxy * gxy[3];
+                        continue;
+                    }
+                    V3(a->D[0], k, i, j) = X; V3(a->D[1], k, i, j) = Y; V3(a->D[2], k, i, j) = Z;
+                    V3(a->D[3], k, i, j) = Wyz; V3(a->D[4], k, i, j) = Wyz;
+                    V3(a->D[5], k, i, j) = Wxz; V3(a->D[6], k, i, j) = Wxz;
+                    V3(a->D[7], k, i, j) = Wxy; V3(a->D[8], k, i, j) = Wxy;
                 } else {
                     const int64_t p = aidx(s, k, i, j);
                     const sp_t *gp[9] = {s->gxc, s->gyc, s->gzc, s->gye, s->gze, s->gxe, s->gze, s->gxe, s->gye};

@@ -604,19 +604,20 @@ def _adj_type(mp_double: bool, sp_double: bool):
     mp = C.c_void_p
     fields = ([(n, mp) for n in FIELDS_V + FIELDS_S]
               + [("aV", C.c_void_p * 9), ("aS", C.c_void_p * 9), ("D", C.c_void_p * 9)]
-              + [("g_lam", C.c_void_p), ("g_mu", C.c_void_p), ("g_rho", C.c_void_p)])
+              + [("g_lam", C.c_void_p), ("g_mu", C.c_void_p), ("g_rho", C.c_void_p), ("aR", C.c_void_p * 6)])
     return type("OrcAdj", (C.Structure,), {"_fields_": fields})
 
 
 class OracleAdjoint:
     """Misfit J = 1/2 sum (d - d_obs)^2 over the velocity traces of every step
     (the wav__store formula, m_wav.f90:517-531) and its gradient with respect
-    to lam, mu and rho, by the discrete adjoint of the elastic PML step."""
+    to lam, mu and rho, by the discrete adjoint of the PML step with its
+    memory variables."""
 
     def __init__(self, model: Model, prec: int = 64):
         cfg = model.cfg
-        if cfg.nm != 0 or cfg.abc != "pml":
-            raise ValueError("the adjoint covers the elastic (NM = 0) PML scheme")
+        if cfg.abc != "pml":
+            raise ValueError("the adjoint covers the PML scheme")
         self.run = OracleRun(model, prec)
         self.m = model
         L = self.run.L
@@ -671,6 +672,8 @@ class OracleAdjoint:
         for k in FIELDS_ADE:
             self.adj["a" + k] = np.zeros_like(run.arr[k])
         self.dens = [np.zeros(shp, run.mp) for _ in range(9)]
+        for k in FIELDS_R:
+            self.adj["a" + k] = np.zeros_like(run.arr[k])
         self.grad = {k: np.zeros(shp) for k in ("lam", "mu", "rho")}
         st = _adj_type(True, True)()
         for k in FIELDS_V + FIELDS_S:
@@ -682,6 +685,8 @@ class OracleAdjoint:
         for q in range(9):
             st.D[q] = self.dens[q].ctypes.data
         st.g_lam, st.g_mu, st.g_rho = (self.grad[k].ctypes.data for k in ("lam", "mu", "rho"))
+        for q, k in enumerate(FIELDS_R):
+            st.aR[q] = self.adj["a" + k].ctypes.data
         self.adj_st = st
         self.adj_ptr = C.cast(C.pointer(st), C.c_void_p)
 

@@ -1,9 +1,11 @@
 """Pins of the oracle's discrete adjoint (oracle/adjoint.c), FP64 variant:
 
   * dot-product test: for one whole step M (stress then velocity update, PML
-    auxiliaries included), <M x, y> = <x, M^T y> for random x, y, to rounding;
+    auxiliaries and Zener memory variables included), <M x, y> = <x, M^T y>
+    for random x, y, to rounding;
   * the misfit gradient with respect to lam, mu and rho against central
-    finite differences at seeded cells, interior and PML.
+    finite differences at seeded cells, interior and PML, elastic and with
+    three memory variables.
 """
 import numpy as np
 import pytest
@@ -27,8 +29,20 @@ def _inner(a, b, keys):
     return sum(float(np.sum(a[k].astype(np.float64) * b[k].astype(np.float64))) for k in keys)
 
 
-def test_dot_product_one_step(orc):
-    cfg = _cfg(sources=[])
+def _state_keys(nm):
+    return O.FIELDS_V + O.FIELDS_S + (O.FIELDS_R if nm else ())
+
+
+def _visco(**kw):
+    """three Zener mechanisms with finite Q (m_kernel.f90:222-238)"""
+    base = dict(nm=3, layers=[[0.0, 2.5, 5.0, 2.9, 150.0, 75.0], [3.0, 2.8, 6.2, 3.6, 250.0, 125.0]])
+    base.update(kw)
+    return _cfg(**base)
+
+
+@pytest.mark.parametrize("nm", [0, 3])
+def test_dot_product_one_step(orc, nm):
+    cfg = _cfg(sources=[]) if nm == 0 else _visco(sources=[])
     m = O.build_model(cfg)
     adj = O.OracleAdjoint(m, 64)
     run = adj.run
@@ -39,14 +53,14 @@ def test_dot_product_one_step(orc):
         a[OFF + 1:-(OFF + 4) + 3 or None, OFF + 1:-(OFF + 4) + 3 or None, OFF + 1:-(OFF + 4) + 3 or None] = 0.0
         a[3:-3, 3:-3, 3:-3] = rng.standard_normal(a[3:-3, 3:-3, 3:-3].shape)
         x[k] = a
-    for k in O.FIELDS_ADE:
+    for k in O.FIELDS_ADE + (O.FIELDS_R if nm else ()):
         x[k] = rng.standard_normal(run.arr[k].shape)
     y = {}
     for k in O.FIELDS_V + O.FIELDS_S:
         a = np.zeros_like(run.arr[k])
         a[3:-3, 3:-3, 3:-3] = rng.standard_normal(a[3:-3, 3:-3, 3:-3].shape)
         y[k] = a
-    for k in O.FIELDS_ADE:
+    for k in O.FIELDS_ADE + (O.FIELDS_R if nm else ()):
         y[k] = rng.standard_normal(run.arr[k].shape)
     # forward: M x
     for k, v in x.items():
@@ -61,15 +75,19 @@ def test_dot_product_one_step(orc):
     adj.new_adjoint()
     for k in O.FIELDS_V + O.FIELDS_S:
         adj.adj[k][...] = y[k]
-    for k in O.FIELDS_ADE:
+    for k in O.FIELDS_ADE + (O.FIELDS_R if nm else ()):
         adj.adj["a" + k][...] = y[k]
     adj.adj_vel()
     adj.adj_stress()
     MTy = {k: adj.adj[k] for k in O.FIELDS_V + O.FIELDS_S}
-    MTy.update({k: adj.adj["a" + k] for k in O.FIELDS_ADE})
+    MTy.update({k: adj.adj["a" + k] for k in O.FIELDS_ADE + (O.FIELDS_R if nm else ())})
     keys = list(x)
     lhs, rhs = _inner(Mx, y, keys), _inner(x, MTy, keys)
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs)), (lhs, rhs)
+    if nm:
+        # control: without the memory variables' adjoints the identity fails
+        MTy.update({k: np.zeros_like(MTy[k]) for k in O.FIELDS_R})
+        assert abs(lhs - _inner(x, MTy, keys)) > 1e-3 * abs(lhs)
 
 
 def test_dot_product_detects_a_wrong_transpose(orc):
@@ -108,10 +126,10 @@ def test_dot_product_detects_a_wrong_transpose(orc):
     assert abs(lhs - rhs) > 1e-6 * max(abs(lhs), abs(rhs))
 
 
-@pytest.fixture(scope="module")
-def fd_case(orc):
+@pytest.fixture(scope="module", params=[0, 3], ids=["elastic", "nm3"])
+def fd_case(orc, request):
     nt = 50
-    cfg = _cfg(nt=nt)
+    cfg = _cfg(nt=nt) if request.param == 0 else _visco(nt=nt)
     m = O.build_model(cfg)
     rng = np.random.default_rng(5)
     d_obs = np.zeros((nt, m.nst, 3))
