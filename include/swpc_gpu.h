@@ -280,14 +280,15 @@ int swpc_batch_synchronize(swpc_batch *batch);
 int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, float *ms_per_step);
 void *swpc_batch_stream(swpc_batch *batch);
 
-/* ---- discrete adjoint (elastic, nm = 0, ADE-CFS PML) ----
+/* ---- discrete adjoint (ADE-CFS PML; elastic or with Zener memory variables) ----
    Misfit J = 1/2 sum over recorded samples, stations and components of
    (d - d_obs)^2, d the velocity traces of swpc_get_seismograms(SWPC_WAV_V),
    and its gradient with respect to the medium arrays lam, mu, rho of
    swpc_set_medium, by the discrete adjoint of the time step.  Runs nt steps
    from rest with the context's sources, checkpointing the state every ckp
-   steps (device memory: ceil(nt/ckp) + ckp copies of the fields and PML
-   auxiliaries), then steps backward segment by segment, recomputing each
+   steps (device memory: ceil(nt/ckp) copies of the fields, PML auxiliaries
+   and memory variables, and ckp copies of the fields and auxiliaries), then
+   steps backward segment by segment, recomputing each
    segment's forward states from its checkpoint.  obs is (ntw, 3, nst) like
    the traces; gradients come back in the reference (k,i,j) halo layout for
    cells 1..n (the halo entries are zero); any output pointer may be NULL.

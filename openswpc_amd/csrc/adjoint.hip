@@ -1,8 +1,13 @@
-// adjoint.hip -- discrete adjoint of the elastic PML time step and the
-// checkpointed gradient driver (swpc_adj_gradient).
+// adjoint.hip -- discrete adjoint of the PML time step with its Zener memory
+// variables, and the checkpointed gradient driver (swpc_adj_gradient).
 //
-// The forward step is linear in (V, S, PML auxiliaries) for a fixed medium;
-// its adjoint applies the transposed updates in reverse order.  Per phase:
+// The forward step is linear in (V, S, R, PML auxiliaries) for a fixed
+// medium; its adjoint applies the transposed updates in reverse order.  The
+// memory variables of an interior cell (m_kernel.f90:222-238,319-333),
+//     R'_c[m] = c1[m] R_c[m] - c2[m] E_c dt,  S'_c = S_c + (F_c + sum d1[m] R'_c[m]) dt,
+// transpose to rho = aR' + d1 dt aS', aR = c1 rho, and -dt sum c2 rho on the
+// derivative combination E_c; they need no forward R for the gradients.
+// Per phase:
 //   1. density kernel (one thread per cell): the adjoint of every spatial
 //      derivative the cell's forward update used (9 per phase), the updated
 //      auxiliary adjoints, and the cell's gradient terms;
@@ -27,8 +32,12 @@ struct AdjArgs {
     Grid g;
     Coef4 c;
     float dt, hx, hy, hz;
-    const float *rho, *lam, *mu;
+    const float *rho, *lam, *mu, *taup, *taus;
     const int *kfs_top, *kfs_bot, *kob_top, *kob_bot;
+    float *aR;          // memory-variable adjoints (6 components x nm, rn cells each)
+    long long rn;
+    int nm;
+    float c1[SWPC_MAXNM], c2[SWPC_MAXNM], d1[SWPC_MAXNM], d2;
     PmlBox box[5];
     long long npml;
     int has_pml;
@@ -276,14 +285,49 @@ __global__ void __launch_bounds__(256) k_adj_stress_dens(AdjArgs a)
         pair_at(a, 0, i, j, k, ax, bx);
         pair_at(a, 1, i, j, k, ay, by);
         pair_at(a, 2, i, j, k, az, bz);
-        a.D[0][p] = X; a.D[1][p] = Y; a.D[2][p] = Z;
-        a.D[3][p] = Wyz; a.D[4][p] = Wyz; a.D[5][p] = Wxz; a.D[6][p] = Wxz; a.D[7][p] = Wxy; a.D[8][p] = Wxy;
         ex = BWD4(Vx, 1, 0, 0, ax, bx);
         ey = BWD4(Vy, 0, 1, 0, ay, by);
         ez = BWD4(Vz, 0, 0, 1, az, bz);
         tyz = FWD4(Vz, 0, 1, 0, ay, by) + FWD4(Vy, 0, 0, 1, az, bz);
         txz = FWD4(Vz, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 0, 1, az, bz);
         txy = FWD4(Vy, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 1, 0, ay, by);
+        if (a.nm > 0) {
+            // transposed memory variables and weighted stress sums
+            const float taup = a.taup[p], taus = a.taus[p];
+            const float tp1 = 1.0f + taup * (1.0f + a.d2), ts1 = 1.0f + taus * (1.0f + a.d2);
+            const float as6[6] = {sxx, syy, szz, syz, sxz, sxy};
+            float sig[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            const long long r = ((long long)(k - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - 1);
+            for (int m = 0; m < a.nm; ++m)
+#pragma unroll
+                for (int cc = 0; cc < 6; ++cc) {
+                    float *ar = a.aR + ((long long)cc * a.nm + m) * a.rn + r;
+                    const float rh = *ar + a.d1[m] * a.dt * as6[cc];
+                    *ar = a.c1[m] * rh;
+                    sig[cc] -= a.dt * a.c2[m] * rh;
+                }
+            float P[3], Q[6];
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) {
+                if (cc < 3) P[cc] = a.dt * as6[cc] * tp1 + sig[cc] * taup;
+                Q[cc] = a.dt * as6[cc] * ts1 + sig[cc] * taus;
+            }
+            const float sP = P[0] + P[1] + P[2], l2m = lam + 2.0f * mu, d3 = ex + ey + ez;
+            a.D[0][p] = l2m * sP - 2.0f * mu * (Q[1] + Q[2]);
+            a.D[1][p] = l2m * sP - 2.0f * mu * (Q[0] + Q[2]);
+            a.D[2][p] = l2m * sP - 2.0f * mu * (Q[0] + Q[1]);
+            a.D[3][p] = hyz * Q[3]; a.D[4][p] = hyz * Q[3];
+            a.D[5][p] = hxz * Q[4]; a.D[6][p] = hxz * Q[4];
+            a.D[7][p] = hxy * Q[5]; a.D[8][p] = hxy * Q[5];
+            a.g_lam[p] += d3 * sP;
+            a.g_mu[p] += 2.0f * d3 * sP - 2.0f * ((d3 - ex) * Q[0] + (d3 - ey) * Q[1] + (d3 - ez) * Q[2]);
+            a.q[0][p] = tyz * Q[3];
+            a.q[1][p] = txz * Q[4];
+            a.q[2][p] = txy * Q[5];
+            return;
+        }
+        a.D[0][p] = X; a.D[1][p] = Y; a.D[2][p] = Z;
+        a.D[3][p] = Wyz; a.D[4][p] = Wyz; a.D[5][p] = Wxz; a.D[6][p] = Wxz; a.D[7][p] = Wxy; a.D[8][p] = Wxy;
     } else {
         const long long q = ade_index(a, i, j, k);
         const long long n = a.npml;
@@ -440,9 +484,10 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
 {
     if (!c) return SWPC_EINVAL;
     if (!obs || nt < 1 || ckp < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_gradient: bad arguments");
-    if (c->desc.nm != 0 || c->desc.abc != SWPC_ABC_PML || c->npml == 0)
-        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: the adjoint covers the elastic (nm = 0) PML scheme");
-    if (!c->has_medium || !c->has_pml) return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: medium or PML not set");
+    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0)
+        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: the adjoint covers the PML scheme");
+    if (!c->has_medium || !c->has_pml || (c->desc.nm > 0 && !c->has_visco))
+        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: medium, visco coefficients or PML not set");
     if (c->nst <= 0 || !(c->wav_what & SWPC_WAV_V) || !c->d_wav[0])
         return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: velocity receivers not set");
     if (c->snap_ntask > 0) return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: disable sections first");
@@ -450,19 +495,22 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
     hipStream_t s = c->stream;
     const Grid &g = c->g;
     const long long nf = g.n, na18 = (long long)A_N * c->npml, na9 = 9 * c->npml;
+    const long long nR = 6LL * c->desc.nm * g.rn;   // forward memory variables
     const int nck = (nt + ckp - 1) / ckp;
-    const long long per_state = 9 * nf + na18;
+    const long long per_state = 9 * nf + na18 + nR;  // checkpoint: V, S, auxiliaries, R
+    const long long sstep = 9 * nf + na18;          // segment step: V_n, S_{n+1}, aV_n, aS_n
     AdjWork w;
     float *ck = w.alloc((size_t)(nck * per_state));
-    float *seg = w.alloc((size_t)(ckp * per_state));
+    float *seg = w.alloc((size_t)(ckp * sstep));
     float *adjF = w.alloc((size_t)(9 * nf));
     float *adjA = w.alloc((size_t)na18);
     float *dens = w.alloc((size_t)(9 * nf));
     float *qq = w.alloc((size_t)(3 * nf));
     float *grad = w.alloc((size_t)(3 * nf));
+    float *adjR = nR ? w.alloc((size_t)nR) : nullptr;
     const long long nobs = (long long)c->ntw * 3 * c->nst;
     float *d_obs = w.alloc((size_t)nobs);
-    if (!ck || !seg || !adjF || !adjA || !dens || !qq || !grad || !d_obs)
+    if (!ck || !seg || !adjF || !adjA || !dens || !qq || !grad || !d_obs || (nR && !adjR))
         return adj_fail(c, SWPC_ENOMEM, "swpc_adj_gradient: device memory for checkpoints and adjoint state");
     if (hipMemcpyAsync(d_obs, obs, nobs * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) return SWPC_EHIP;
 
@@ -470,6 +518,7 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
     for (int q = 0; q < SWPC_NFIELD; ++q)
         if (zero(c->fields[q], nf, s) != hipSuccess) return SWPC_EHIP;
     if (zero(c->ade, na18, s) != hipSuccess) return SWPC_EHIP;
+    if (nR && zero(c->R, nR, s) != hipSuccess) return SWPC_EHIP;
     if (c->d_uacc && zero(c->d_uacc, 3LL * c->nst, s) != hipSuccess) return SWPC_EHIP;
     if (c->d_eacc && zero(c->d_eacc, 6LL * c->nst, s) != hipSuccess) return SWPC_EHIP;
     auto save_state = [&](float *dst) -> hipError_t {
@@ -477,14 +526,18 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
             hipError_t e = hipMemcpyAsync(dst + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
             if (e != hipSuccess) return e;
         }
-        return hipMemcpyAsync(dst + 9 * nf, c->ade, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        hipError_t e = hipMemcpyAsync(dst + 9 * nf, c->ade, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess || !nR) return e;
+        return hipMemcpyAsync(dst + 9 * nf + na18, c->R, nR * sizeof(float), hipMemcpyDeviceToDevice, s);
     };
     auto load_state = [&](const float *src) -> hipError_t {
         for (int q = 0; q < SWPC_NFIELD; ++q) {
             hipError_t e = hipMemcpyAsync(c->fields[q], src + q * nf, nf * sizeof(float), hipMemcpyDeviceToDevice, s);
             if (e != hipSuccess) return e;
         }
-        return hipMemcpyAsync(c->ade, src + 9 * nf, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        hipError_t e = hipMemcpyAsync(c->ade, src + 9 * nf, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess || !nR) return e;
+        return hipMemcpyAsync(c->R, src + 9 * nf + na18, nR * sizeof(float), hipMemcpyDeviceToDevice, s);
     };
     hipError_t e = hipSuccess;
     for (int n = 0; n < nt && e == hipSuccess; ++n) {
@@ -524,7 +577,14 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
     a.hx = 1.0f / (float)c->desc.dx;
     a.hy = 1.0f / (float)c->desc.dy;
     a.hz = 1.0f / (float)c->desc.dz;
-    a.rho = c->rho; a.lam = c->lam; a.mu = c->mu;
+    a.rho = c->rho; a.lam = c->lam; a.mu = c->mu; a.taup = c->taup; a.taus = c->taus;
+    a.aR = adjR;
+    a.rn = g.rn;
+    a.nm = c->desc.nm;
+    for (int m = 0; m < SWPC_MAXNM; ++m) {
+        a.c1[m] = c->visco.c1[m]; a.c2[m] = c->visco.c2[m]; a.d1[m] = c->visco.d1[m];
+    }
+    a.d2 = c->visco.d2;
     a.kfs_top = c->kfs_top; a.kfs_bot = c->kfs_bot; a.kob_top = c->kob_top; a.kob_bot = c->kob_bot;
     for (int q = 0; q < 5; ++q) a.box[q] = c->box[q];
     a.npml = c->npml;
@@ -537,12 +597,12 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
     for (int q = 0; q < 3; ++q) a.q[q] = qq + q * nf;
     a.aA = adjA;
     a.g_lam = grad; a.g_mu = grad + nf; a.g_rho = grad + 2 * nf;
-    if (zero(adjF, 9 * nf, s) != hipSuccess || zero(adjA, na18, s) != hipSuccess || zero(grad, 3 * nf, s) != hipSuccess)
+    if (zero(adjF, 9 * nf, s) != hipSuccess || zero(adjA, na18, s) != hipSuccess || zero(grad, 3 * nf, s) != hipSuccess ||
+        (nR && zero(adjR, nR, s) != hipSuccess))
         return SWPC_EHIP;
     const dim3 blk(64, 4, 1), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
     const float sc = 0.5f * c->wav_M0 * c->wav_UC * 1e9f;
     // per step of a segment: V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
-    const long long sstep = per_state;
     for (int sg = nck - 1; sg >= 0 && e == hipSuccess; --sg) {
         const int n0 = sg * ckp, n1 = (n0 + ckp < nt) ? n0 + ckp : nt;
         e = load_state(ck + (size_t)sg * per_state);
