@@ -284,17 +284,45 @@ void *swpc_batch_stream(swpc_batch *batch);
    Misfit J = 1/2 sum over recorded samples, stations and components of
    (d - d_obs)^2, d the velocity traces of swpc_get_seismograms(SWPC_WAV_V),
    and its gradient with respect to the medium arrays lam, mu, rho of
-   swpc_set_medium, by the discrete adjoint of the time step.  Runs nt steps
-   from rest with the context's sources, checkpointing the state every ckp
-   steps (device memory: ceil(nt/ckp) copies of the fields, PML auxiliaries
-   and memory variables, and ckp copies of the fields and auxiliaries), then
-   steps backward segment by segment, recomputing each
-   segment's forward states from its checkpoint.  obs is (ntw, 3, nst) like
-   the traces; gradients come back in the reference (k,i,j) halo layout for
-   cells 1..n (the halo entries are zero); any output pointer may be NULL.
-   The context's fields are left in an intermediate state.  There is no
-   counterpart in the reference (which has no adjoint); the forward it
-   differentiates is the one of src/swpc_3d/main.f90:119-139. */
+   swpc_set_medium, by the discrete adjoint of the time step.  None of these
+   routines has a counterpart in the reference (which has no adjoint); the
+   forward they differentiate is the one of src/swpc_3d/main.f90:119-139.
+
+   Checkpointing, two levels: a first forward pass of nt steps from rest with
+   the context's sources records the traces and keeps the state (fields, PML
+   auxiliaries, memory variables) every ckp_outer steps; walking backward,
+   each outer segment is recomputed once to keep inner checkpoints every
+   ckp_inner steps, and each inner segment once more into per-step records
+   (V_n, S_{n+1} and their auxiliaries) that the adjoint steps read.  Device
+   memory: ceil(nt/ckp_outer) + ceil(ckp_outer/ckp_inner) - 1 states and
+   ckp_inner step records, besides the adjoint fields.  ckp_inner = 0 or
+   >= ckp_outer is the single-level scheme (ckp_outer states, ckp_outer
+   records, two forward passes).  Every scheme gives the same bits.
+
+   Gradients of several sources sum on the device: swpc_adj_reset, then per
+   source swpc_set_sources / swpc_set_receivers and swpc_adj_accumulate, then
+   swpc_adj_get_gradient.  The context keeps the adjoint's device memory
+   between these calls until swpc_adj_release or swpc_destroy.  The context's
+   fields are left in an intermediate state. */
+
+/* The scheme with the fewest forward steps whose device memory fits budget
+   bytes (budget <= 0: the device's free memory, plus what the adjoint
+   already holds, less 1 GiB).  Outputs may be NULL: outer and inner spacing,
+   the device bytes it needs, and the forward steps it runs. */
+int swpc_adj_plan(swpc_ctx *ctx, int nt, int64_t budget, int *ckp_outer, int *ckp_inner, int64_t *bytes,
+                  int64_t *fwd_steps);
+/* Zero the device gradient sums (allocating the adjoint fields). */
+int swpc_adj_reset(swpc_ctx *ctx);
+/* Misfit of the current sources and receivers against obs ((ntw, 3, nst)
+   like the traces) and its gradient, added to the device sums. */
+int swpc_adj_accumulate(swpc_ctx *ctx, int nt, const float *obs, int ckp_outer, int ckp_inner, double *misfit);
+/* The sums in the reference (k,i,j) halo layout for cells 1..n (halo
+   entries zero); any output pointer may be NULL. */
+int swpc_adj_get_gradient(swpc_ctx *ctx, float *g_lam, float *g_mu, float *g_rho);
+/* Free the adjoint's device memory (sums included). */
+int swpc_adj_release(swpc_ctx *ctx);
+/* One source in one call: reset, accumulate with single-level checkpoints
+   every ckp steps, get, release. */
 int swpc_adj_gradient(swpc_ctx *ctx, int nt, const float *obs, int ckp, double *misfit, float *g_lam, float *g_mu,
                       float *g_rho);
 

@@ -446,28 +446,27 @@ __global__ void k_fill(float *p, long long n)
         p[t] = 0.0f;
 }
 
+// sum += part, one rounding per element
+__global__ void k_add(float *sum, const float *part, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        sum[t] = sum[t] + part[t];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // driver
 // ---------------------------------------------------------------------------
-namespace {
-
-struct AdjWork {
-    std::vector<float *> bufs;
-    ~AdjWork()
-    {
-        for (float *p : bufs)
-            if (p) (void)hipFree(p);
-    }
-    float *alloc(size_t n)
-    {
-        float *p = nullptr;
-        if (hipMalloc((void **)&p, n * sizeof(float)) != hipSuccess) return nullptr;
-        bufs.push_back(p);
-        return p;
-    }
+// Sizes of one context's adjoint, in floats.
+struct AdjSizes {
+    long long nf, na18, nR;
+    long long state() const { return 9 * nf + na18 + nR; }  // checkpoint: V, S, auxiliaries, R
+    long long sstep() const { return 9 * nf + na18; }       // per-step record: V_n, S_{n+1}, aV_n, aS_n
+    long long base() const { return 27 * nf + na18 + nR; }  // adjoint fields, densities, q, gradients, sums, aR
 };
+
+namespace {
 
 hipError_t zero(float *p, long long n, hipStream_t s)
 {
@@ -475,52 +474,220 @@ hipError_t zero(float *p, long long n, hipStream_t s)
     return hipGetLastError();
 }
 
+AdjSizes sizes_of(const swpc_ctx *c)
+{
+    AdjSizes z;
+    z.nf = c->g.n;
+    z.na18 = (long long)A_N * c->npml;
+    z.nR = 6LL * c->desc.nm * c->g.rn;
+    return z;
+}
+
+// Two-level checkpoint scheme: outer checkpoints every c1 steps of the first
+// forward pass; each outer segment is recomputed once to place inner
+// checkpoints every c2 steps, and each inner segment once more into the
+// per-step records the adjoint reads.  c2 == c1 is the single-level scheme.
+long long scheme_fwd_steps(int nt, int c1, int c2)
+{
+    long long f = nt;  // the recording pass
+    for (int n0 = 0; n0 < nt; n0 += c1) {
+        const int L = nt - n0 < c1 ? nt - n0 : c1;
+        f += L;
+        if (c2 < L) f += (long long)((L - 1) / c2) * c2;
+    }
+    return f;
+}
+
+long long scheme_ck_slots(int nt, int c1, int c2)
+{
+    const long long outer = (nt + c1 - 1) / c1;
+    const int L = nt < c1 ? nt : c1;
+    return outer + (c2 < L ? (L - 1) / c2 : 0);  // inner segment 0 starts from its outer checkpoint
+}
+
+long long scheme_seg_slots(int nt, int c1, int c2)
+{
+    int m = c2 < c1 ? c2 : c1;
+    return m < nt ? m : nt;
+}
+
+long long scheme_floats(const AdjSizes &z, int nt, int c1, int c2)
+{
+    return z.base() + scheme_ck_slots(nt, c1, c2) * z.state() + scheme_seg_slots(nt, c1, c2) * z.sstep();
+}
+
 }  // namespace
+
+// Device memory of the adjoint, kept by the context between calls so that
+// the gradients of many sources sum on the device: the adjoint state, the
+// gradient sums, and checkpoint stores that grow to the largest scheme used.
+struct AdjWorkspace {
+    AdjSizes z;
+    float *adjF = nullptr, *adjA = nullptr, *dens = nullptr, *qq = nullptr, *adjR = nullptr;
+    float *gsrc = nullptr;  // one source's gradients, summed over its steps
+    float *grad = nullptr;  // the sums over sources
+    float *ck = nullptr, *seg = nullptr, *obs = nullptr;
+    long long ck_n = 0, seg_n = 0, obs_n = 0;
+    long long bytes = 0;
+    int nsum = 0;  // sources summed since the last reset
+};
 
 int adj_fail(swpc_ctx *c, int code, const char *msg);
 
-extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp, double *misfit, float *g_lam,
-                                 float *g_mu, float *g_rho)
+namespace {
+
+void wfree(swpc_ctx *c, float *&p, long long n)
+{
+    if (p) {
+        (void)hipFree(p);
+        c->adj_ws->bytes -= n * (long long)sizeof(float);
+        c->bytes -= n * (long long)sizeof(float);
+    }
+    p = nullptr;
+}
+
+bool walloc(swpc_ctx *c, float *&p, long long n)
+{
+    p = nullptr;
+    if (n <= 0) return true;
+    if (hipMalloc((void **)&p, (size_t)n * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return false;
+    }
+    c->adj_ws->bytes += n * (long long)sizeof(float);
+    c->bytes += n * (long long)sizeof(float);
+    return true;
+}
+
+// a store of at least n floats (contents not kept)
+bool wgrow(swpc_ctx *c, float *&p, long long &have, long long n)
+{
+    if (have >= n && p) return true;
+    wfree(c, p, have);
+    have = 0;
+    if (!walloc(c, p, n)) return false;
+    have = n;
+    return true;
+}
+
+// the workspace with its fixed part; the sums start at zero
+int ws_open(swpc_ctx *c)
+{
+    if (c->adj_ws) return SWPC_OK;
+    c->adj_ws = new AdjWorkspace();
+    AdjWorkspace &w = *c->adj_ws;
+    w.z = sizes_of(c);
+    const long long nf = w.z.nf;
+    if (!walloc(c, w.adjF, 9 * nf) || !walloc(c, w.adjA, w.z.na18) || !walloc(c, w.dens, 9 * nf) ||
+        !walloc(c, w.qq, 3 * nf) || !walloc(c, w.gsrc, 3 * nf) || !walloc(c, w.grad, 3 * nf) ||
+        !walloc(c, w.adjR, w.z.nR)) {
+        swpc_adj_release(c);
+        return adj_fail(c, SWPC_ENOMEM, "adjoint: device memory for the adjoint fields");
+    }
+    if (zero(w.grad, 3 * nf, c->stream) != hipSuccess) return SWPC_EHIP;
+    return SWPC_OK;
+}
+
+int check_ready(swpc_ctx *c)
+{
+    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0)
+        return adj_fail(c, SWPC_ESTATE, "adjoint: the adjoint covers the PML scheme");
+    if (!c->has_medium || !c->has_pml || (c->desc.nm > 0 && !c->has_visco))
+        return adj_fail(c, SWPC_ESTATE, "adjoint: medium, visco coefficients or PML not set");
+    if (c->nst <= 0 || !(c->wav_what & SWPC_WAV_V) || !c->d_wav[0])
+        return adj_fail(c, SWPC_ESTATE, "adjoint: velocity receivers not set");
+    if (c->snap_ntask > 0) return adj_fail(c, SWPC_ESTATE, "adjoint: disable sections first");
+    return SWPC_OK;
+}
+
+}  // namespace
+
+extern "C" int swpc_adj_release(swpc_ctx *c)
 {
     if (!c) return SWPC_EINVAL;
-    if (!obs || nt < 1 || ckp < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_gradient: bad arguments");
-    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0)
-        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: the adjoint covers the PML scheme");
-    if (!c->has_medium || !c->has_pml || (c->desc.nm > 0 && !c->has_visco))
-        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: medium, visco coefficients or PML not set");
-    if (c->nst <= 0 || !(c->wav_what & SWPC_WAV_V) || !c->d_wav[0])
-        return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: velocity receivers not set");
-    if (c->snap_ntask > 0) return adj_fail(c, SWPC_ESTATE, "swpc_adj_gradient: disable sections first");
+    if (!c->adj_ws) return SWPC_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    AdjWorkspace &w = *c->adj_ws;
+    const long long nf = w.z.nf;
+    wfree(c, w.adjF, 9 * nf); wfree(c, w.adjA, w.z.na18); wfree(c, w.dens, 9 * nf);
+    wfree(c, w.qq, 3 * nf); wfree(c, w.gsrc, 3 * nf); wfree(c, w.grad, 3 * nf); wfree(c, w.adjR, w.z.nR);
+    wfree(c, w.ck, w.ck_n); wfree(c, w.seg, w.seg_n); wfree(c, w.obs, w.obs_n);
+    delete c->adj_ws;
+    c->adj_ws = nullptr;
+    return SWPC_OK;
+}
+
+extern "C" int swpc_adj_plan(swpc_ctx *c, int nt, int64_t budget, int *ckp_outer, int *ckp_inner, int64_t *bytes,
+                             int64_t *fwd_steps)
+{
+    if (!c) return SWPC_EINVAL;
+    if (nt < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_plan: nt < 1");
+    if (budget <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) return SWPC_EHIP;
+        budget = (int64_t)fr + (c->adj_ws ? c->adj_ws->bytes : 0) - (1LL << 30);
+    }
+    const AdjSizes z = sizes_of(c);
+    const long long cap = budget / (long long)sizeof(float);
+    int b1 = 0, b2 = 0;
+    long long bf = 0, bm = 0;
+    // fewest forward steps, then least memory; every (c1, c2 <= c1)
+    for (int c1 = 1; c1 <= nt; ++c1) {
+        for (int c2 = c1; c2 >= 1; --c2) {
+            const long long m = scheme_floats(z, nt, c1, c2);
+            if (m > cap) continue;
+            const long long f = scheme_fwd_steps(nt, c1, c2);
+            if (!b1 || f < bf || (f == bf && m < bm)) { b1 = c1; b2 = c2; bf = f; bm = m; }
+            break;  // a smaller c2 only recomputes more
+        }
+    }
+    if (!b1) return adj_fail(c, SWPC_ENOMEM, "swpc_adj_plan: no checkpoint scheme fits the budget");
+    if (ckp_outer) *ckp_outer = b1;
+    if (ckp_inner) *ckp_inner = b2;
+    if (bytes) *bytes = bm * (int64_t)sizeof(float);
+    if (fwd_steps) *fwd_steps = bf;
+    return SWPC_OK;
+}
+
+extern "C" int swpc_adj_reset(swpc_ctx *c)
+{
+    if (!c) return SWPC_EINVAL;
     if (hipSetDevice(c->device) != hipSuccess) return SWPC_EHIP;
+    int rc = ws_open(c);
+    if (rc != SWPC_OK) return rc;
+    if (zero(c->adj_ws->grad, 3 * c->adj_ws->z.nf, c->stream) != hipSuccess) return SWPC_EHIP;
+    c->adj_ws->nsum = 0;
+    return SWPC_OK;
+}
+
+extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ckp_outer, int ckp_inner,
+                                   double *misfit)
+{
+    if (!c) return SWPC_EINVAL;
+    if (!obs || nt < 1 || ckp_outer < 1 || ckp_inner < 0)
+        return adj_fail(c, SWPC_EINVAL, "swpc_adj_accumulate: bad arguments");
+    int rc = check_ready(c);
+    if (rc != SWPC_OK) return rc;
+    if (hipSetDevice(c->device) != hipSuccess) return SWPC_EHIP;
+    if ((rc = ws_open(c)) != SWPC_OK) return rc;
+    AdjWorkspace &w = *c->adj_ws;
+    const int c1 = ckp_outer;
+    const int c2 = (ckp_inner == 0 || ckp_inner >= c1) ? c1 : ckp_inner;
     hipStream_t s = c->stream;
     const Grid &g = c->g;
-    const long long nf = g.n, na18 = (long long)A_N * c->npml, na9 = 9 * c->npml;
-    const long long nR = 6LL * c->desc.nm * g.rn;   // forward memory variables
-    const int nck = (nt + ckp - 1) / ckp;
-    const long long per_state = 9 * nf + na18 + nR;  // checkpoint: V, S, auxiliaries, R
-    const long long sstep = 9 * nf + na18;          // segment step: V_n, S_{n+1}, aV_n, aS_n
-    AdjWork w;
-    float *ck = w.alloc((size_t)(nck * per_state));
-    float *seg = w.alloc((size_t)(ckp * sstep));
-    float *adjF = w.alloc((size_t)(9 * nf));
-    float *adjA = w.alloc((size_t)na18);
-    float *dens = w.alloc((size_t)(9 * nf));
-    float *qq = w.alloc((size_t)(3 * nf));
-    float *grad = w.alloc((size_t)(3 * nf));
-    float *adjR = nR ? w.alloc((size_t)nR) : nullptr;
+    const long long nf = w.z.nf, na18 = w.z.na18, na9 = 9 * c->npml, nR = w.z.nR;
+    const long long per_state = w.z.state(), sstep = w.z.sstep();
+    const long long nck = scheme_ck_slots(nt, c1, c2), nout = (nt + c1 - 1) / c1;
     const long long nobs = (long long)c->ntw * 3 * c->nst;
-    float *d_obs = w.alloc((size_t)nobs);
-    if (!ck || !seg || !adjF || !adjA || !dens || !qq || !grad || !d_obs || (nR && !adjR))
-        return adj_fail(c, SWPC_ENOMEM, "swpc_adj_gradient: device memory for checkpoints and adjoint state");
-    if (hipMemcpyAsync(d_obs, obs, nobs * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) return SWPC_EHIP;
+    if (!wgrow(c, w.ck, w.ck_n, nck * per_state) || !wgrow(c, w.seg, w.seg_n, scheme_seg_slots(nt, c1, c2) * sstep) ||
+        !wgrow(c, w.obs, w.obs_n, nobs))
+        return adj_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the checkpoints "
+                                        "(swpc_adj_plan finds a scheme that fits)");
+    float *outer_ck = w.ck, *inner_ck = w.ck + nout * per_state;
+    if (hipMemcpyAsync(w.obs, obs, nobs * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) return SWPC_EHIP;
 
-    // ---- forward pass from rest, checkpoints every ckp steps ----
-    for (int q = 0; q < SWPC_NFIELD; ++q)
-        if (zero(c->fields[q], nf, s) != hipSuccess) return SWPC_EHIP;
-    if (zero(c->ade, na18, s) != hipSuccess) return SWPC_EHIP;
-    if (nR && zero(c->R, nR, s) != hipSuccess) return SWPC_EHIP;
-    if (c->d_uacc && zero(c->d_uacc, 3LL * c->nst, s) != hipSuccess) return SWPC_EHIP;
-    if (c->d_eacc && zero(c->d_eacc, 6LL * c->nst, s) != hipSuccess) return SWPC_EHIP;
     auto save_state = [&](float *dst) -> hipError_t {
         for (int q = 0; q < SWPC_NFIELD; ++q) {
             hipError_t e = hipMemcpyAsync(dst + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
@@ -539,17 +706,33 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
         if (e != hipSuccess || !nR) return e;
         return hipMemcpyAsync(c->R, src + 9 * nf + na18, nR * sizeof(float), hipMemcpyDeviceToDevice, s);
     };
-    hipError_t e = hipSuccess;
-    for (int n = 0; n < nt && e == hipSuccess; ++n) {
-        const int it = n + 1;
-        if (n % ckp == 0) e = save_state(ck + (size_t)(n / ckp) * per_state);
-        if (e == hipSuccess) e = launch_wav_store(c, it, nullptr);
-        if (e == hipSuccess) e = launch_stress_side(c, nullptr);
+    // one forward step n (time step it = n + 1), the order of main.f90:119-139
+    auto stress_phase = [&](int it) -> hipError_t {
+        hipError_t e = launch_stress_side(c, nullptr);
         if (e == hipSuccess) e = launch_stress_cols(c, 3);
         if (e == hipSuccess) e = launch_stressglut(c, it, nullptr);
-        if (e == hipSuccess) e = launch_vel_side(c, nullptr);
+        return e;
+    };
+    auto vel_phase = [&](int it) -> hipError_t {
+        hipError_t e = launch_vel_side(c, nullptr);
         if (e == hipSuccess) e = launch_vel_cols(c, 3);
         if (e == hipSuccess) e = launch_bodyforce(c, it, nullptr);
+        return e;
+    };
+
+    // ---- forward pass from rest, outer checkpoints every c1 steps ----
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < SWPC_NFIELD && e == hipSuccess; ++q) e = zero(c->fields[q], nf, s);
+    if (e == hipSuccess) e = zero(c->ade, na18, s);
+    if (e == hipSuccess && nR) e = zero(c->R, nR, s);
+    if (e == hipSuccess && c->d_uacc) e = zero(c->d_uacc, 3LL * c->nst, s);
+    if (e == hipSuccess && c->d_eacc) e = zero(c->d_eacc, 6LL * c->nst, s);
+    for (int n = 0; n < nt && e == hipSuccess; ++n) {
+        const int it = n + 1;
+        if (n % c1 == 0) e = save_state(outer_ck + (size_t)(n / c1) * per_state);
+        if (e == hipSuccess) e = launch_wav_store(c, it, nullptr);
+        if (e == hipSuccess) e = stress_phase(it);
+        if (e == hipSuccess) e = vel_phase(it);
     }
     if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
 
@@ -578,7 +761,7 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
     a.hy = 1.0f / (float)c->desc.dy;
     a.hz = 1.0f / (float)c->desc.dz;
     a.rho = c->rho; a.lam = c->lam; a.mu = c->mu; a.taup = c->taup; a.taus = c->taus;
-    a.aR = adjR;
+    a.aR = w.adjR;
     a.rn = g.rn;
     a.nm = c->desc.nm;
     for (int m = 0; m < SWPC_MAXNM; ++m) {
@@ -591,40 +774,41 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
     a.has_pml = 1;
     a.gxc = c->gxc; a.gxe = c->gxe; a.gyc = c->gyc; a.gye = c->gye; a.gzc = c->gzc; a.gze = c->gze;
     for (int q = 0; q < 9; ++q) {
-        a.aF[q] = adjF + q * nf;
-        a.D[q] = dens + q * nf;
+        a.aF[q] = w.adjF + q * nf;
+        a.D[q] = w.dens + q * nf;
     }
-    for (int q = 0; q < 3; ++q) a.q[q] = qq + q * nf;
-    a.aA = adjA;
-    a.g_lam = grad; a.g_mu = grad + nf; a.g_rho = grad + 2 * nf;
-    if (zero(adjF, 9 * nf, s) != hipSuccess || zero(adjA, na18, s) != hipSuccess || zero(grad, 3 * nf, s) != hipSuccess ||
-        (nR && zero(adjR, nR, s) != hipSuccess))
+    for (int q = 0; q < 3; ++q) a.q[q] = w.qq + q * nf;
+    a.aA = w.adjA;
+    a.g_lam = w.gsrc; a.g_mu = w.gsrc + nf; a.g_rho = w.gsrc + 2 * nf;
+    if (zero(w.adjF, 9 * nf, s) != hipSuccess || zero(w.adjA, na18, s) != hipSuccess ||
+        zero(w.gsrc, 3 * nf, s) != hipSuccess || (nR && zero(w.adjR, nR, s) != hipSuccess))
         return SWPC_EHIP;
     const dim3 blk(64, 4, 1), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
     const float sc = 0.5f * c->wav_M0 * c->wav_UC * 1e9f;
-    // per step of a segment: V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
-    for (int sg = nck - 1; sg >= 0 && e == hipSuccess; --sg) {
-        const int n0 = sg * ckp, n1 = (n0 + ckp < nt) ? n0 + ckp : nt;
-        e = load_state(ck + (size_t)sg * per_state);
+    // records of steps n0 .. n1-1 from the current state; per step:
+    // V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
+    auto record = [&](int n0, int n1) -> hipError_t {
+        hipError_t e = hipSuccess;
         for (int n = n0; n < n1 && e == hipSuccess; ++n) {
-            float *st = seg + (size_t)(n - n0) * sstep;
+            float *st = w.seg + (size_t)(n - n0) * sstep;
             const int it = n + 1;
             for (int q = 0; q < 3 && e == hipSuccess; ++q)
                 e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
             if (e == hipSuccess) e = hipMemcpyAsync(st + 9 * nf, c->ade, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = launch_stress_side(c, nullptr);
-            if (e == hipSuccess) e = launch_stress_cols(c, 3);
-            if (e == hipSuccess) e = launch_stressglut(c, it, nullptr);
+            if (e == hipSuccess) e = stress_phase(it);
             for (int q = 3; q < 9 && e == hipSuccess; ++q)
                 e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(st + 9 * nf + na9, c->ade + na9, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = launch_vel_side(c, nullptr);
-            if (e == hipSuccess) e = launch_vel_cols(c, 3);
-            if (e == hipSuccess) e = launch_bodyforce(c, it, nullptr);
+            if (e == hipSuccess) e = vel_phase(it);
         }
+        return e;
+    };
+    // the adjoint of steps n1-1 down to n0 from their records
+    auto backward = [&](int n0, int n1) -> hipError_t {
+        hipError_t e = hipSuccess;
         for (int n = n1 - 1; n >= n0 && e == hipSuccess; --n) {
-            const float *st = seg + (size_t)(n - n0) * sstep;
+            const float *st = w.seg + (size_t)(n - n0) * sstep;
             for (int q = 0; q < 9; ++q) a.fF[q] = st + q * nf;
             a.fA = st + 9 * nf + na9;      // stress auxiliaries aS_n
             hipLaunchKernelGGL(k_adj_vel_dens, grd, blk, 0, s, a);
@@ -634,20 +818,68 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
             hipLaunchKernelGGL(k_adj_stress_gather, grd, blk, 0, s, a);
             if (n % c->ntdec_w == 0 && n / c->ntdec_w < c->ntw)
                 hipLaunchKernelGGL(k_adj_record, dim3(1), dim3(64), 0, s, a, c->d_ist, c->d_jst, c->d_kst, c->nst,
-                                   c->d_wav[0], d_obs, c->ntw, n / c->ntdec_w, sc);
+                                   c->d_wav[0], w.obs, c->ntw, n / c->ntdec_w, sc);
             e = hipGetLastError();
         }
+        return e;
+    };
+    for (int sg = (int)nout - 1; sg >= 0 && e == hipSuccess; --sg) {
+        const int N0 = sg * c1, N1 = (N0 + c1 < nt) ? N0 + c1 : nt;
+        const int nin = c2 < N1 - N0 ? (N1 - N0 - 1) / c2 : 0;  // inner segments after the first
+        e = load_state(outer_ck + (size_t)sg * per_state);
+        // inner checkpoints: the fields end at the start of the last inner segment
+        for (int n = N0; n < N0 + nin * c2 && e == hipSuccess; ++n) {
+            if (n > N0 && (n - N0) % c2 == 0) e = save_state(inner_ck + (size_t)((n - N0) / c2 - 1) * per_state);
+            if (e == hipSuccess) e = stress_phase(n + 1);
+            if (e == hipSuccess) e = vel_phase(n + 1);
+        }
+        for (int is = nin; is >= 0 && e == hipSuccess; --is) {
+            const int n0 = N0 + is * c2, n1 = (n0 + c2 < N1) ? n0 + c2 : N1;
+            if (is < nin) e = load_state(is == 0 ? outer_ck + (size_t)sg * per_state : inner_ck + (size_t)(is - 1) * per_state);
+            if (e == hipSuccess) e = record(n0, n1);
+            if (e == hipSuccess) e = backward(n0, n1);
+        }
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_add, dim3(1024), dim3(256), 0, s, w.grad, w.gsrc, 3 * nf);
+        e = hipGetLastError();
     }
     if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
+    if (hipStreamSynchronize(s) != hipSuccess) return SWPC_EHIP;
+    w.nsum += 1;
+    return SWPC_OK;
+}
+
+extern "C" int swpc_adj_get_gradient(swpc_ctx *c, float *g_lam, float *g_mu, float *g_rho)
+{
+    if (!c) return SWPC_EINVAL;
+    if (!c->adj_ws || c->adj_ws->nsum == 0)
+        return adj_fail(c, SWPC_ESTATE, "swpc_adj_get_gradient: no gradient summed since the last reset");
+    if (hipSetDevice(c->device) != hipSuccess) return SWPC_EHIP;
+    hipStream_t s = c->stream;
+    const long long nf = c->adj_ws->z.nf;
     // gradients out in the reference (k,i,j) halo layout
     float *outs[3] = {g_lam, g_mu, g_rho};
     for (int q = 0; q < 3; ++q) {
         if (!outs[q]) continue;
-        if (launch_dev_to_ref(c, grad + q * nf, c->staging, s) != hipSuccess ||
+        if (launch_dev_to_ref(c, c->adj_ws->grad + q * nf, c->staging, s) != hipSuccess ||
             hipMemcpyAsync(outs[q], c->staging, (size_t)c->staging_n * sizeof(float), hipMemcpyDeviceToHost, s) !=
                 hipSuccess)
             return SWPC_EHIP;
     }
     if (hipStreamSynchronize(s) != hipSuccess) return SWPC_EHIP;
     return SWPC_OK;
+}
+
+extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp, double *misfit, float *g_lam,
+                                 float *g_mu, float *g_rho)
+{
+    if (!c) return SWPC_EINVAL;
+    if (!obs || nt < 1 || ckp < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_gradient: bad arguments");
+    int rc = check_ready(c);
+    if (rc == SWPC_OK) rc = swpc_adj_reset(c);
+    if (rc == SWPC_OK) rc = swpc_adj_accumulate(c, nt, obs, ckp, ckp, misfit);
+    if (rc == SWPC_OK) rc = swpc_adj_get_gradient(c, g_lam, g_mu, g_rho);
+    if (rc == SWPC_OK) rc = swpc_adj_release(c);
+    return rc;
 }

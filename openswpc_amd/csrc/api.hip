@@ -301,6 +301,7 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     reset_graph(c);
+    swpc_adj_release(c);
     for (int q = 0; q < SWPC_NFIELD; ++q) dfree(c->fields[q]);
     dfree(c->rho); dfree(c->lam); dfree(c->mu); dfree(c->taup); dfree(c->taus);
     dfree(c->kfs_top); dfree(c->kfs_bot); dfree(c->kob_top); dfree(c->kob_bot); dfree(c->kob);

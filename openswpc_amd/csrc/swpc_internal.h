@@ -180,6 +180,7 @@ struct swpc_ctx {
     int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel,
                        // bit 1: side-column PML kernels on a second stream
     long long bytes;
+    struct AdjWorkspace *adj_ws;  // adjoint state, gradient sums and checkpoints (adjoint.hip)
 };
 
 // ---- launchers (stencil.hip / aux.hip) ----

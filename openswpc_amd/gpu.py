@@ -107,6 +107,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_stream.argtypes = [vp]
     L.swpc_stream.restype = vp
     L.swpc_adj_gradient.argtypes = [vp, i, fp, i, C.POINTER(C.c_double), vp, vp, vp]
+    L.swpc_adj_plan.argtypes = [vp, i, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int64)]
+    L.swpc_adj_reset.argtypes = [vp]
+    L.swpc_adj_accumulate.argtypes = [vp, i, fp, i, i, C.POINTER(C.c_double)]
+    L.swpc_adj_get_gradient.argtypes = [vp, vp, vp, vp]
+    L.swpc_adj_release.argtypes = [vp]
     L.swpc_rmedia_3d.argtypes = [i, i, i, i, i, f, f, f, f, f, f, f, f, C.c_uint64, fp]
     L.swpc_rmedia_last_error.argtypes = []
     L.swpc_rmedia_last_error.restype = C.c_char_p
@@ -312,6 +318,36 @@ class SwpcGPU:
         self._chk(self.L.swpc_adj_gradient(self.h, int(nt), _f32(obs), int(ckp), C.byref(J), gl.ctypes.data,
                                            gm.ctypes.data, gr.ctypes.data))
         return J.value, gl, gm, gr
+
+    def adjoint_plan(self, nt, budget=0):
+        """(ckp_outer, ckp_inner, device bytes, forward steps) of the cheapest
+        checkpoint scheme within budget bytes (0: the free device memory)"""
+        c1, c2 = C.c_int(), C.c_int()
+        nb, nf = C.c_int64(), C.c_int64()
+        self._chk(self.L.swpc_adj_plan(self.h, int(nt), int(budget), C.byref(c1), C.byref(c2), C.byref(nb),
+                                       C.byref(nf)))
+        return c1.value, c2.value, nb.value, nf.value
+
+    def adjoint_reset(self):
+        self._chk(self.L.swpc_adj_reset(self.h))
+
+    def adjoint_accumulate(self, nt, obs, ckp_outer, ckp_inner=0):
+        """misfit of the current sources and receivers; the gradient is added
+        to the device sums"""
+        J = C.c_double()
+        self._chk(self.L.swpc_adj_accumulate(self.h, int(nt), _f32(obs), int(ckp_outer), int(ckp_inner),
+                                             C.byref(J)))
+        return J.value
+
+    def adjoint_sums(self):
+        """the device gradient sums (lam, mu, rho), reference layout"""
+        shp = (self.ny + 6, self.nx + 6, self.nz + 6)
+        gl, gm, gr = (np.zeros(shp, np.float32) for _ in range(3))
+        self._chk(self.L.swpc_adj_get_gradient(self.h, gl.ctypes.data, gm.ctypes.data, gr.ctypes.data))
+        return gl, gm, gr
+
+    def adjoint_release(self):
+        self._chk(self.L.swpc_adj_release(self.h))
 
     def synchronize(self):
         self._chk(self.L.swpc_synchronize(self.h))

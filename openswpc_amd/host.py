@@ -150,36 +150,11 @@ class Inputs:
     scalars: dict
 
 
-def build_inputs(spec: ModelSpec) -> Inputs:
+def _source_inputs(spec: ModelSpec, srcs: list):
+    """grid cells, moment tensors or forces, M0, UC and the STF table of the
+    sources (m_source.f90:84-303), and the PML cut-off frequency they set"""
     L = host_lib()
-    nx, ny, nz, na, nm = spec.nx, spec.ny, spec.nz, spec.na, spec.nm
-    shp = (ny + 6, nx + 6, nz + 6)
-    rho, lam, mu, taup, taus = (np.zeros(shp, np.float32) for _ in range(5))
-    layers = np.ascontiguousarray(np.asarray(spec.layers, np.float32).reshape(-1))
-    rc = C.c_int()
-    kind = {"lhm": 0, "lgm": 1}[spec.vmodel]
     f32 = np.float32
-    L.swpch_vmodel_layers(kind, nx, ny, nz, spec.zbeg, spec.dz, spec.vcut, len(spec.layers), layers,
-                          rho, lam, mu, taup, taus, C.byref(rc))
-    if rc.value != 0:
-        raise ValueError(f"velocity model rejected (rc={rc.value})")
-    if spec.xi is not None:
-        dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
-        vmax_cfl = (f32(6) / f32(7)) * f32(1) / np.sqrt(1 / dx32**2 + 1 / dy32**2 + 1 / dz32**2) / f32(spec.dt)
-        L.swpch_perturb(nx, ny, nz, np.ascontiguousarray(spec.xi, np.float32), f32(spec.vcut), f32(vmax_cfl),
-                        f32(1.0), rho, lam, mu)
-    ts = np.zeros(max(nm, 1), np.float32)
-    zeta = C.c_float()
-    kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot = (np.zeros((ny + 6, nx + 6), np.int32) for _ in range(6))
-    vmin, vmax = C.c_float(), C.c_float()
-    L.swpch_medium_setup(nx, ny, nz, na, nm, spec.fq_min, spec.fq_max, spec.fq_ref, rho, lam, mu, taup, taus, ts,
-                         C.byref(zeta), kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot, C.byref(vmin), C.byref(vmax))
-    c1, c2, d1 = (np.zeros(max(nm, 1), np.float32) for _ in range(3))
-    d2 = C.c_float()
-    L.swpch_kernel_coef(nm, ts, spec.dt, c1, c2, d1, C.byref(d2))
-
-    # sources; green_mode has none (m_source.f90:84-90)
-    srcs = [] if spec.green is not None else spec.sources
     nsrc = len(srcs)
     dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
     isrc = np.array([L.swpch_x2i(s["x"], spec.xbeg, dx32) for s in srcs], np.int32)
@@ -213,11 +188,68 @@ def build_inputs(spec: ModelSpec) -> Inputs:
     if nsrc:
         L.swpch_stf_table(spec.nt, nsrc, STF_CODES[spec.stftype], t0, tr, spec.tbeg, spec.dt, shift, stf)
     stf = np.ascontiguousarray(stf[:, :nsrc])
+    arrays = dict(isrc=isrc, jsrc=jsrc, ksrc=ksrc, mo=mo, comp=np.ascontiguousarray(comp), stf=stf)
+    return arrays, dict(fcut=fcut, M0=M0, UC=UC, nsrc=nsrc)
+
+
+def _station_inputs(spec: ModelSpec, kfs: np.ndarray, kob: np.ndarray) -> dict:
+    """grid cells of the stations inside the domain (src/swpc_3d/m_wav.f90:188-240,
+    through swpch_station_index)"""
+    L = host_lib()
+    f32 = np.float32
+    dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
+    nx, ny, nz = spec.nx, spec.ny, spec.nz
+    nst = len(spec.stations)
+    if nst:
+        xs = np.array([s[0] for s in spec.stations], np.float32)
+        ys = np.array([s[1] for s in spec.stations], np.float32)
+        zs = np.array([s[2] for s in spec.stations], np.float32)
+        zsw = np.array([ZSW[s[3]] for s in spec.stations], np.int32)
+        ist, jst, kst, inside = (np.zeros(nst, np.int32) for _ in range(4))
+        L.swpch_station_index(nst, xs, ys, zs, zsw, nx, ny, nz, spec.xbeg, spec.ybeg, spec.zbeg, dx32, dy32, dz32,
+                              kfs, kob, ist, jst, kst, inside)
+        keep = inside.astype(bool)
+        return dict(ist=ist[keep].copy(), jst=jst[keep].copy(), kst=kst[keep].copy())
+    return dict(ist=np.zeros(0, np.int32), jst=np.zeros(0, np.int32), kst=np.zeros(0, np.int32))
+
+
+def build_inputs(spec: ModelSpec) -> Inputs:
+    L = host_lib()
+    nx, ny, nz, na, nm = spec.nx, spec.ny, spec.nz, spec.na, spec.nm
+    shp = (ny + 6, nx + 6, nz + 6)
+    rho, lam, mu, taup, taus = (np.zeros(shp, np.float32) for _ in range(5))
+    layers = np.ascontiguousarray(np.asarray(spec.layers, np.float32).reshape(-1))
+    rc = C.c_int()
+    kind = {"lhm": 0, "lgm": 1}[spec.vmodel]
+    f32 = np.float32
+    L.swpch_vmodel_layers(kind, nx, ny, nz, spec.zbeg, spec.dz, spec.vcut, len(spec.layers), layers,
+                          rho, lam, mu, taup, taus, C.byref(rc))
+    if rc.value != 0:
+        raise ValueError(f"velocity model rejected (rc={rc.value})")
+    if spec.xi is not None:
+        dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
+        vmax_cfl = (f32(6) / f32(7)) * f32(1) / np.sqrt(1 / dx32**2 + 1 / dy32**2 + 1 / dz32**2) / f32(spec.dt)
+        L.swpch_perturb(nx, ny, nz, np.ascontiguousarray(spec.xi, np.float32), f32(spec.vcut), f32(vmax_cfl),
+                        f32(1.0), rho, lam, mu)
+    ts = np.zeros(max(nm, 1), np.float32)
+    zeta = C.c_float()
+    kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot = (np.zeros((ny + 6, nx + 6), np.int32) for _ in range(6))
+    vmin, vmax = C.c_float(), C.c_float()
+    L.swpch_medium_setup(nx, ny, nz, na, nm, spec.fq_min, spec.fq_max, spec.fq_ref, rho, lam, mu, taup, taus, ts,
+                         C.byref(zeta), kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot, C.byref(vmin), C.byref(vmax))
+    c1, c2, d1 = (np.zeros(max(nm, 1), np.float32) for _ in range(3))
+    d2 = C.c_float()
+    L.swpch_kernel_coef(nm, ts, spec.dt, c1, c2, d1, C.byref(d2))
+
+    srcs = [] if spec.green is not None else spec.sources   # green_mode has none (m_source.f90:84-90)
+    sa, ssc = _source_inputs(spec, srcs)
+    fcut, M0, UC, nsrc = ssc["fcut"], ssc["M0"], ssc["UC"], ssc["nsrc"]
+    dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
 
     # absorbers
     arrays = dict(rho=rho, lam=lam, mu=mu, taup=taup, taus=taus, kfs=kfs, kob=kob, kfs_top=kfs_top,
                   kfs_bot=kfs_bot, kob_top=kob_top, kob_bot=kob_bot, ts=ts[:nm].copy(), c1=c1, c2=c2, d1=d1,
-                  isrc=isrc, jsrc=jsrc, ksrc=ksrc, mo=mo, comp=np.ascontiguousarray(comp), stf=stf)
+                  **sa)
     if spec.abc == "pml":
         for ax, n, beg, h in (("x", nx, spec.xbeg, dx32), ("y", ny, spec.ybeg, dy32), ("z", nz, spec.zbeg, dz32)):
             gc = np.zeros(4 * n, np.float32)
@@ -233,20 +265,7 @@ def build_inputs(spec: ModelSpec) -> Inputs:
             arrays[f"cg{ax}_c"] = g_c
             arrays[f"cg{ax}_b"] = g_b
 
-    # stations
-    nst = len(spec.stations)
-    if nst:
-        xs = np.array([s[0] for s in spec.stations], np.float32)
-        ys = np.array([s[1] for s in spec.stations], np.float32)
-        zs = np.array([s[2] for s in spec.stations], np.float32)
-        zsw = np.array([ZSW[s[3]] for s in spec.stations], np.int32)
-        ist, jst, kst, inside = (np.zeros(nst, np.int32) for _ in range(4))
-        L.swpch_station_index(nst, xs, ys, zs, zsw, nx, ny, nz, spec.xbeg, spec.ybeg, spec.zbeg, dx32, dy32, dz32,
-                              kfs, kob, ist, jst, kst, inside)
-        keep = inside.astype(bool)
-        arrays.update(ist=ist[keep].copy(), jst=jst[keep].copy(), kst=kst[keep].copy())
-    else:
-        arrays.update(ist=np.zeros(0, np.int32), jst=np.zeros(0, np.int32), kst=np.zeros(0, np.int32))
+    arrays.update(_station_inputs(spec, kfs, kob))
     ntw = int(math.floor(float(f32(spec.nt - 1) / f32(spec.ntdec_w) + f32(1.0))))
     scalars = dict(zeta=float(zeta.value), d2=float(d2.value), vmin=float(vmin.value), vmax=float(vmax.value),
                    fcut=float(fcut), M0=float(M0), UC=float(UC), ntw=ntw, nsrc=nsrc)
@@ -310,3 +329,23 @@ def make_simulation(inp: Inputs, device: int = 0):
         g.set_green(a["green_ig"], a["green_jg"], a["green_kg"], *sc["green_src"], s.green.cmp, s.green.bforce,
                     s.ntdec_w, sc["ntw"], a["green_stf"])
     return g
+
+
+def set_acquisition(sim, inp: Inputs, spec: ModelSpec) -> dict:
+    """Switch a simulation made by make_simulation(inp) to the sources and
+    stations of spec -- same grid, medium and time step; the sources must
+    give the same PML cut-off frequency, which shaped the absorber profiles.
+    Returns the arrays (ist, jst, kst, ...) and scalars (M0, UC, ntw) used."""
+    s0 = inp.spec
+    for k in ("nx", "ny", "nz", "dx", "dy", "dz", "dt", "na", "nm", "xbeg", "ybeg", "zbeg", "abc", "bf_mode"):
+        if getattr(spec, k) != getattr(s0, k):
+            raise ValueError(f"set_acquisition: {k} differs from the simulation's")
+    sa, ssc = _source_inputs(spec, spec.sources)
+    if np.float32(ssc["fcut"]) != np.float32(inp.scalars["fcut"]):
+        raise ValueError("set_acquisition: the sources change the PML cut-off frequency")
+    st = _station_inputs(spec, inp.arrays["kfs"], inp.arrays["kob"])
+    ntw = int(math.floor(float(np.float32(spec.nt - 1) / np.float32(spec.ntdec_w) + np.float32(1.0))))
+    sim.set_sources(sa["isrc"], sa["jsrc"], sa["ksrc"], sa["mo"], sa["comp"], ssc["M0"], sa["stf"],
+                    bf_mode=spec.bf_mode)
+    sim.set_receivers(st["ist"], st["jst"], st["kst"], spec.wav_what, spec.ntdec_w, ntw, ssc["M0"], ssc["UC"])
+    return dict(arrays={**sa, **st}, scalars=dict(M0=float(ssc["M0"]), UC=float(ssc["UC"]), ntw=ntw))

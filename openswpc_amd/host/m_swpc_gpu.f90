@@ -231,6 +231,36 @@ module m_swpc_gpu
             real(c_double), intent(out) :: misfit
             real(c_float), intent(out) :: g_lam(*), g_mu(*), g_rho(*)
         end function
+        integer(c_int) function swpc_adj_plan(ctx, nt, budget, ckp_outer, ckp_inner, bytes, fwd_steps) &
+            bind(C, name='swpc_adj_plan')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nt
+            integer(c_int64_t), value :: budget
+            integer(c_int), intent(out) :: ckp_outer, ckp_inner
+            integer(c_int64_t), intent(out) :: bytes, fwd_steps
+        end function
+        integer(c_int) function swpc_adj_reset(ctx) bind(C, name='swpc_adj_reset')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function swpc_adj_accumulate(ctx, nt, obs, ckp_outer, ckp_inner, misfit) &
+            bind(C, name='swpc_adj_accumulate')
+            import :: c_int, c_ptr, c_float, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nt, ckp_outer, ckp_inner
+            real(c_float), intent(in) :: obs(*)
+            real(c_double), intent(out) :: misfit
+        end function
+        integer(c_int) function swpc_adj_get_gradient(ctx, g_lam, g_mu, g_rho) bind(C, name='swpc_adj_get_gradient')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: g_lam(*), g_mu(*), g_rho(*)
+        end function
+        integer(c_int) function swpc_adj_release(ctx) bind(C, name='swpc_adj_release')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
         integer(c_int) function swpc_batch_create(members, n, batch) bind(C, name='swpc_batch_create')
             import :: c_int, c_ptr
             type(c_ptr), intent(in) :: members(*)

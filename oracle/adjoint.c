@@ -185,8 +185,12 @@ void orc_adj_vel(const orc_state *s, orc_adj *a)
     const double dt = (double)s->dt;
     const double hx = hinv(s, 0), hy = hinv(s, 1), hz = hinv(s, 2);
     for (int q = 0; q < 9; q++) memset(a->D[q], 0, sizeof(mp_t) * orc_n3(s));
-    /* per-cell densities: D[0..8] = d/d(dxSxx dySxy dzSxz dxSxy dySyy dzSyz dxSxz dySyz dzSzz) */
-    for (int j = 1; j <= s->ny; j++)
+    /* per-cell densities: D[0..8] = d/d(dxSxx dySxy dzSxz dxSxy dySyy dzSyz dxSxz dySyz dzSzz).
+       A cell also adds to the rho gradient of row j + 1: two passes over the
+       rows of one parity keep the rows each thread writes disjoint. */
+    for (int par = 0; par < 2; par++)
+#pragma omp parallel for schedule(static)
+    for (int j = 1 + par; j <= s->ny; j += 2)
         for (int i = 1; i <= s->nx; i++)
             for (int k = 1; k <= s->nz; k++) {
                 const int in = is_int(s, k, i, j), pm = is_pml(s, k, i, j);
@@ -241,6 +245,7 @@ void orc_adj_vel(const orc_state *s, orc_adj *a)
                 V3(a->g_rho, k + 1, i, j) += qz;
             }
     /* gather onto the stress points */
+#pragma omp parallel for schedule(static)
     for (int j = -1; j <= s->ny + 2; j++)
         for (int i = -1; i <= s->nx + 2; i++)
             for (int k = -1; k <= s->nz + 2; k++) {
@@ -266,8 +271,11 @@ void orc_adj_stress(const orc_state *s, orc_adj *a)
     const double hx = hinv(s, 0), hy = hinv(s, 1), hz = hinv(s, 2);
     const double eps = 1.1920928955078125e-07;
     for (int q = 0; q < 9; q++) memset(a->D[q], 0, sizeof(mp_t) * orc_n3(s));
-    /* D[0..8] = d/d(dxVx dyVy dzVz dyVz dzVy dxVz dzVx dxVy dyVx) */
-    for (int j = 1; j <= s->ny; j++)
+    /* D[0..8] = d/d(dxVx dyVy dzVz dyVz dzVy dxVz dzVx dxVy dyVx).  A cell
+       also adds to the mu gradient of row j + 1: rows of one parity per pass. */
+    for (int par = 0; par < 2; par++)
+#pragma omp parallel for schedule(static)
+    for (int j = 1 + par; j <= s->ny; j += 2)
         for (int i = 1; i <= s->nx; i++)
             for (int k = 1; k <= s->nz; k++) {
                 const int in = is_int(s, k, i, j), pm = is_pml(s, k, i, j);
@@ -380,6 +388,7 @@ void orc_adj_stress(const orc_state *s, orc_adj *a)
                 V3(a->g_mu, k, i + 1, j + 1) += qxy * gxy[3];
             }
     /* gather onto the velocity points */
+#pragma omp parallel for schedule(static)
     for (int j = -1; j <= s->ny + 2; j++)
         for (int i = -1; i <= s->nx + 2; i++)
             for (int k = -1; k <= s->nz + 2; k++) {

@@ -90,6 +90,12 @@ def test_null_context_is_rejected():
     assert L.swpc_rmedia_3d(0, 4, 8, 8, 8, 1, 1, 1, 1, 1, 1, 0.1, 0.5, 1, np.zeros(512, np.float32)) == -1
     assert b"bad arguments" in L.swpc_rmedia_last_error()
     assert L.swpc_step(None, 1, 1, 0, 1) == -1
+    for name in ("swpc_adj_reset", "swpc_adj_release"):
+        assert getattr(L, name)(None) == -1, name
+    assert L.swpc_adj_plan(None, 10, 0, None, None, None, None) == -1
+    assert L.swpc_adj_accumulate(None, 10, np.zeros(1, np.float32), 5, 0, None) == -1
+    assert L.swpc_adj_get_gradient(None, None, None, None) == -1
+    assert L.swpc_adj_gradient(None, 10, np.zeros(1, np.float32), 5, None, None, None, None) == -1
     assert L.swpc_kernel_vmax(None, np.zeros(3, np.float32)) == -1
     assert L.swpc_destroy(None) == 0
     assert L.swpc_device_bytes(None) == 0
