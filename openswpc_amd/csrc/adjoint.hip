@@ -22,6 +22,7 @@
 // `ckp` steps of a first forward pass.
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "swpc_internal.h"
@@ -45,7 +46,9 @@ struct AdjArgs {
     float *aF[9];       // adjoint fields: Vx Vy Vz Sxx Syy Szz Syz Sxz Sxy
     float *aA;          // adjoint auxiliaries, 18 x npml
     float *D[9];        // per-cell derivative adjoints
-    float *q[3];        // per-cell gradient terms gathered in phase 2
+    float *q[6];        // per-cell gradient terms gathered in phase 2 (velocity phase: 3, stress phase: 6)
+    const unsigned char *cls;  // class of every grid point (k_adj_class)
+    float tA[3][4], tB[3][4];  // per axis and class: near (p40 or 1/h) and far (p41) weights
     const float *fF[9]; // forward fields of the step (same order as aF)
     const float *fA;    // forward auxiliaries of the phase, 9 x npml
     float *g_lam, *g_mu, *g_rho;
@@ -100,58 +103,61 @@ __device__ static inline int cell_kind(const AdjArgs &a, int i, int j, int k)
     return a.has_pml ? 2 : 0;
 }
 
-__device__ static inline float hinv(const AdjArgs &a, int axis) { return axis == 0 ? a.hx : axis == 1 ? a.hy : a.hz; }
-
-// gather of the densities of a backward-type derivative (cell c reads f(c),
-// f(c-1) and at 4th order f(c+1), f(c-2)) onto point (i,j,k)
-__device__ static float gather_bwd(const AdjArgs &a, const float *D, int axis, int i, int j, int k)
+// Class of every grid point, for the gathers: 0 outside the updated cells
+// (halo), 1 interior off the free-surface band, 2 interior in the band, 3 PML.
+// A cell's weights in a gather depend only on its class and the axis.
+__global__ void k_adj_class(AdjArgs a, unsigned char *cls)
 {
-    const int off[4] = {0, 1, -1, 2};
-    float acc = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int o = off[t];
-        const int ii = i + (axis == 0 ? o : 0), jj = j + (axis == 1 ? o : 0), kk = k + (axis == 2 ? o : 0);
-        const int kind = cell_kind(a, ii, jj, kk);
-        if (!kind) continue;
-        const float d = D[gidx(a.g, ii, jj, kk)];
+    const Grid &g = a.g;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < g.n; p += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(p / g.plane) - 2;
+        const long long r = p % g.plane;
+        const int j = (int)(r / g.pitch) - 2;
+        const int i = (int)(r % g.pitch) - g.ox + 1;
+        const int kind = cell_kind(a, i, j, k);
+        unsigned char c = 0;
         if (kind == 1) {
-            float p40, p41;
-            pair_at(a, axis, ii, jj, kk, p40, p41);
-            acc += (o == 0 ? p40 : o == 1 ? -p40 : o == -1 ? -p41 : p41) * d;
-        } else if (o == 0) {
-            acc += hinv(a, axis) * d;
-        } else if (o == 1) {
-            acc -= hinv(a, axis) * d;
+            const long long c2 = g2idx(g, i, j);
+            c = in_band(k, a.kfs_top[c2], a.kfs_bot[c2], a.kob_top[c2], a.kob_bot[c2]) ? 2 : 1;
+        } else if (kind == 2) {
+            c = 3;
         }
+        cls[p] = c;
     }
-    return acc;
 }
 
-// gather of a forward-type derivative (cell c reads f(c+1), f(c) and at 4th
-// order f(c+2), f(c-1))
-__device__ static float gather_fwd(const AdjArgs &a, const float *D, int axis, int i, int j, int k)
+// weight of a class from a per-class table (t[0] = 0 for points outside)
+__device__ static inline float wt(const float t[4], int c) { return c == 1 ? t[1] : c == 2 ? t[2] : c == 3 ? t[3] : t[0]; }
+
+// Gathers of one axis (stride s): the densities of the cells whose
+// derivative stencils read point p, times those cells' weights (A: the
+// near pair p40 or 1/h, B: the far pair p41 or 0).  Points outside the
+// updated cells hold density 0 and class 0.  The weights of the five
+// points p-2s .. p+2s are read once per axis and shared by its gathers.
+struct AxisW {
+    float A[5], B[5];
+};
+__device__ static inline AxisW axis_w(const unsigned char *cls, long long p, long long s, const float tA[4],
+                                      const float tB[4])
 {
-    const int off[4] = {-1, 0, -2, 1};
-    float acc = 0.0f;
+    AxisW w;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int o = off[t];
-        const int ii = i + (axis == 0 ? o : 0), jj = j + (axis == 1 ? o : 0), kk = k + (axis == 2 ? o : 0);
-        const int kind = cell_kind(a, ii, jj, kk);
-        if (!kind) continue;
-        const float d = D[gidx(a.g, ii, jj, kk)];
-        if (kind == 1) {
-            float p40, p41;
-            pair_at(a, axis, ii, jj, kk, p40, p41);
-            acc += (o == -1 ? p40 : o == 0 ? -p40 : o == -2 ? -p41 : p41) * d;
-        } else if (o == -1) {
-            acc += hinv(a, axis) * d;
-        } else if (o == 0) {
-            acc -= hinv(a, axis) * d;
-        }
+    for (int o = 0; o < 5; ++o) {
+        const int c = cls[p + (o - 2) * s];
+        w.A[o] = wt(tA, c);
+        w.B[o] = wt(tB, c);
     }
-    return acc;
+    return w;
+}
+// forward-type derivative (cell c reads c+1, c, c+2, c-1)
+__device__ static inline float gat_fwd(const float *D, long long p, long long s, const AxisW &w)
+{
+    return w.A[1] * D[p - s] - w.A[2] * D[p] - w.B[0] * D[p - 2 * s] + w.B[3] * D[p + s];
+}
+// backward-type derivative (cell c reads c, c-1, c+1, c-2)
+__device__ static inline float gat_bwd(const float *D, long long p, long long s, const AxisW &w)
+{
+    return w.A[2] * D[p] - w.A[3] * D[p + s] - w.B[1] * D[p - s] + w.B[4] * D[p + 2 * s];
 }
 
 #define AT(F, di, dj, dk) (F)[gidx(g, i + (di), j + (dj), k + (dk))]
@@ -164,280 +170,326 @@ __device__ static float gather_fwd(const AdjArgs &a, const float *D, int axis, i
 
 __device__ static inline float4 prof(const float4 *p, int n) { return p[n - 1]; }
 
+// One thread per cell (i, j, k) of the updated box, 64 x 4 cells per block.
+// The blocks form a 1D grid, plane after plane, each plane's tiles padded to
+// a multiple of 8 blocks: block b goes to XCD b % 8, so a tile and the same
+// tile one plane up or down share an XCD and its L2, and each plane still
+// spreads over all eight XCDs.  (A thread marching over several planes keeps
+// the 64-bit addresses of every array and offset live across its loop and
+// halves the occupancy; keeping whole planes on one XCD overflows its L2
+// with the z neighbours of nine density arrays.  Both measured slower.)
+#define ADJ_CELL(a)                                                \
+    const Grid &g = (a).g;                                         \
+    const int nbx_ = (g.nx + 63) >> 6, nby_ = (g.ny + 3) >> 2;     \
+    const int tpp_ = (nbx_ * nby_ + 7) & ~7;                       \
+    const int k = 1 + (int)(blockIdx.x / tpp_);                    \
+    const int t_ = (int)(blockIdx.x % tpp_);                       \
+    if (t_ >= nbx_ * nby_) return;                                 \
+    const int i = 1 + (t_ % nbx_) * 64 + threadIdx.x;              \
+    const int j = 1 + (t_ / nbx_) * 4 + threadIdx.y;               \
+    if (i > g.nx || j > g.ny) return;
+
 // adjoint of the velocity update: densities, stress-auxiliary adjoints, rho terms
+// (the stores come after every load, which the compiler cannot reorder)
 __global__ void __launch_bounds__(256) k_adj_vel_dens(AdjArgs a)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = 1 + blockIdx.z;
-    const Grid &g = a.g;
-    if (i > g.nx || j > g.ny) return;
-    const int kind = cell_kind(a, i, j, k);
-    if (!kind) return;
-    const long long p = gidx(g, i, j, k);
-    const float r0 = a.rho[p];
-    const float sx = r0 + AT(a.rho, 1, 0, 0), sy = r0 + AT(a.rho, 0, 1, 0), sz = r0 + AT(a.rho, 0, 0, 1);
-    const float vx = a.aF[FVX][p], vy = a.aF[FVY][p], vz = a.aF[FVZ][p];
-    const float wx = 2.0f / sx * a.dt * vx, wy = 2.0f / sy * a.dt * vy, wz = 2.0f / sz * a.dt * vz;
-    const float *Sxx = a.fF[FSXX], *Syy = a.fF[FSYY], *Szz = a.fF[FSZZ];
-    const float *Syz = a.fF[FSYZ], *Sxz = a.fF[FSXZ], *Sxy = a.fF[FSXY];
-    float fx, fy, fz;
-    if (kind == 1) {
-        float ax, bx, ay, by, az, bz;
-        pair_at(a, 0, i, j, k, ax, bx);
-        pair_at(a, 1, i, j, k, ay, by);
-        pair_at(a, 2, i, j, k, az, bz);
-        a.D[0][p] = wx; a.D[1][p] = wx; a.D[2][p] = wx;
-        a.D[3][p] = wy; a.D[4][p] = wy; a.D[5][p] = wy;
-        a.D[6][p] = wz; a.D[7][p] = wz; a.D[8][p] = wz;
-        fx = FWD4(Sxx, 1, 0, 0, ax, bx) + BWD4(Sxy, 0, 1, 0, ay, by) + BWD4(Sxz, 0, 0, 1, az, bz);
-        fy = BWD4(Sxy, 1, 0, 0, ax, bx) + FWD4(Syy, 0, 1, 0, ay, by) + BWD4(Syz, 0, 0, 1, az, bz);
-        fz = BWD4(Sxz, 1, 0, 0, ax, bx) + BWD4(Syz, 0, 1, 0, ay, by) + FWD4(Szz, 0, 0, 1, az, bz);
-    } else {
-        const long long q = ade_index(a, i, j, k);
-        const long long n = a.npml;
-        const float4 gp[9] = {prof(a.gxe, i), prof(a.gyc, j), prof(a.gzc, k), prof(a.gxc, i), prof(a.gye, j),
-                              prof(a.gzc, k), prof(a.gxc, i), prof(a.gyc, j), prof(a.gze, k)};
-        const float w[9] = {wx, wx, wx, wy, wy, wy, wz, wz, wz};
-        const float der[9] = {FWD2(Sxx, 1, 0, 0, a.hx), BWD2(Sxy, 0, 1, 0, a.hy), BWD2(Sxz, 0, 0, 1, a.hz),
-                              BWD2(Sxy, 1, 0, 0, a.hx), FWD2(Syy, 0, 1, 0, a.hy), BWD2(Syz, 0, 0, 1, a.hz),
-                              BWD2(Sxz, 1, 0, 0, a.hx), BWD2(Syz, 0, 1, 0, a.hy), FWD2(Szz, 0, 0, 1, a.hz)};
-        float f[3] = {0.0f, 0.0f, 0.0f};
+    ADJ_CELL(a) {
+        const int kind = cell_kind(a, i, j, k);
+        const long long p = gidx(g, i, j, k);
+        const float r0 = a.rho[p];
+        const float sx = r0 + AT(a.rho, 1, 0, 0), sy = r0 + AT(a.rho, 0, 1, 0), sz = r0 + AT(a.rho, 0, 0, 1);
+        const float vx = a.aF[FVX][p], vy = a.aF[FVY][p], vz = a.aF[FVZ][p];
+        const float wx = 2.0f / sx * a.dt * vx, wy = 2.0f / sy * a.dt * vy, wz = 2.0f / sz * a.dt * vz;
+        const float *Sxx = a.fF[FSXX], *Syy = a.fF[FSYY], *Szz = a.fF[FSZZ];
+        const float *Syz = a.fF[FSYZ], *Sxz = a.fF[FSXZ], *Sxy = a.fF[FSXY];
+        float fx, fy, fz;
+        float d[9];
+        if (kind == 1) {
+            float ax, bx, ay, by, az, bz;
+            pair_at(a, 0, i, j, k, ax, bx);
+            pair_at(a, 1, i, j, k, ay, by);
+            pair_at(a, 2, i, j, k, az, bz);
+            fx = FWD4(Sxx, 1, 0, 0, ax, bx) + BWD4(Sxy, 0, 1, 0, ay, by) + BWD4(Sxz, 0, 0, 1, az, bz);
+            fy = BWD4(Sxy, 1, 0, 0, ax, bx) + FWD4(Syy, 0, 1, 0, ay, by) + BWD4(Syz, 0, 0, 1, az, bz);
+            fz = BWD4(Sxz, 1, 0, 0, ax, bx) + BWD4(Syz, 0, 1, 0, ay, by) + FWD4(Szz, 0, 0, 1, az, bz);
+            d[0] = wx; d[1] = wx; d[2] = wx;
+            d[3] = wy; d[4] = wy; d[5] = wy;
+            d[6] = wz; d[7] = wz; d[8] = wz;
+        } else {
+            const long long q = ade_index(a, i, j, k);
+            const long long n = a.npml;
+            const float4 gp[9] = {prof(a.gxe, i), prof(a.gyc, j), prof(a.gzc, k), prof(a.gxc, i), prof(a.gye, j),
+                                  prof(a.gzc, k), prof(a.gxc, i), prof(a.gyc, j), prof(a.gze, k)};
+            const float w[9] = {wx, wx, wx, wy, wy, wy, wz, wz, wz};
+            const float der[9] = {FWD2(Sxx, 1, 0, 0, a.hx), BWD2(Sxy, 0, 1, 0, a.hy), BWD2(Sxz, 0, 0, 1, a.hz),
+                                  BWD2(Sxy, 1, 0, 0, a.hx), FWD2(Syy, 0, 1, 0, a.hy), BWD2(Syz, 0, 0, 1, a.hz),
+                                  BWD2(Sxz, 1, 0, 0, a.hx), BWD2(Syz, 0, 1, 0, a.hy), FWD2(Szz, 0, 0, 1, a.hz)};
+            float ahat[9], fa[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            float *ah = a.aA + (long long)(A_axSxx + c) * n + q;
-            const float ahat = *ah;
-            a.D[c][p] = gp[c].x * w[c] + gp[c].w * a.dt * ahat;
-            *ah = gp[c].y * w[c] + gp[c].z * ahat;
-            f[c / 3] += gp[c].x * der[c] + gp[c].y * a.fA[(long long)c * n + q];
+            for (int c = 0; c < 9; ++c) {
+                ahat[c] = a.aA[(long long)(A_axSxx + c) * n + q];
+                fa[c] = a.fA[(long long)c * n + q];
+            }
+            float f[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                d[c] = gp[c].x * w[c] + gp[c].w * a.dt * ahat[c];
+                f[c / 3] += gp[c].x * der[c] + gp[c].y * fa[c];
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c) a.aA[(long long)(A_axSxx + c) * n + q] = gp[c].y * w[c] + gp[c].z * ahat[c];
+            fx = f[0]; fy = f[1]; fz = f[2];
         }
-        fx = f[0]; fy = f[1]; fz = f[2];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) a.D[c][p] = d[c];
+        a.q[0][p] = -2.0f / (sx * sx) * a.dt * fx * vx;
+        a.q[1][p] = -2.0f / (sy * sy) * a.dt * fy * vy;
+        a.q[2][p] = -2.0f / (sz * sz) * a.dt * fz * vz;
     }
-    a.q[0][p] = -2.0f / (sx * sx) * a.dt * fx * vx;
-    a.q[1][p] = -2.0f / (sy * sy) * a.dt * fy * vy;
-    a.q[2][p] = -2.0f / (sz * sz) * a.dt * fz * vz;
 }
 
 // adjoint of the velocity update: stress adjoints and the rho gradient
+// (every load before the stores: the class bytes may alias anything)
 __global__ void __launch_bounds__(256) k_adj_vel_gather(AdjArgs a)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = 1 + blockIdx.z;
-    const Grid &g = a.g;
-    if (i > g.nx || j > g.ny) return;
-    const long long p = gidx(g, i, j, k);
-    a.aF[FSXX][p] += gather_fwd(a, a.D[0], 0, i, j, k);
-    a.aF[FSYY][p] += gather_fwd(a, a.D[4], 1, i, j, k);
-    a.aF[FSZZ][p] += gather_fwd(a, a.D[8], 2, i, j, k);
-    a.aF[FSXY][p] += gather_bwd(a, a.D[1], 1, i, j, k) + gather_bwd(a, a.D[3], 0, i, j, k);
-    a.aF[FSXZ][p] += gather_bwd(a, a.D[2], 2, i, j, k) + gather_bwd(a, a.D[6], 0, i, j, k);
-    a.aF[FSYZ][p] += gather_bwd(a, a.D[5], 2, i, j, k) + gather_bwd(a, a.D[7], 1, i, j, k);
-    // each cell's density enters the buoyancy of its own point and of the
-    // point one cell back along each axis
-    float gr = a.q[0][p] + a.q[1][p] + a.q[2][p];
-    if (cell_kind(a, i - 1, j, k)) gr += AT(a.q[0], -1, 0, 0);
-    if (cell_kind(a, i, j - 1, k)) gr += AT(a.q[1], 0, -1, 0);
-    if (cell_kind(a, i, j, k - 1)) gr += AT(a.q[2], 0, 0, -1);
-    if (cell_kind(a, i, j, k)) a.g_rho[p] += gr;
+    ADJ_CELL(a) {
+        const long long p = gidx(g, i, j, k);
+        const long long sx = 1, sy = g.pitch, sz = g.plane;
+        const AxisW wx = axis_w(a.cls, p, sx, a.tA[0], a.tB[0]);
+        const AxisW wy = axis_w(a.cls, p, sy, a.tA[1], a.tB[1]);
+        const AxisW wz = axis_w(a.cls, p, sz, a.tA[2], a.tB[2]);
+        const float dxx = gat_fwd(a.D[0], p, sx, wx);
+        const float dyy = gat_fwd(a.D[4], p, sy, wy);
+        const float dzz = gat_fwd(a.D[8], p, sz, wz);
+        const float dxy = gat_bwd(a.D[1], p, sy, wy) + gat_bwd(a.D[3], p, sx, wx);
+        const float dxz = gat_bwd(a.D[2], p, sz, wz) + gat_bwd(a.D[6], p, sx, wx);
+        const float dyz = gat_bwd(a.D[5], p, sz, wz) + gat_bwd(a.D[7], p, sy, wy);
+        // each cell's density enters the buoyancy of its own point and of the
+        // point one cell back along each axis (q is 0 outside the cells)
+        float gr = a.q[0][p] + a.q[1][p] + a.q[2][p];
+        gr += a.q[0][p - sx];
+        gr += a.q[1][p - sy];
+        gr += a.q[2][p - sz];
+        const float oxx = a.aF[FSXX][p], oyy = a.aF[FSYY][p], ozz = a.aF[FSZZ][p];
+        const float oxy = a.aF[FSXY][p], oxz = a.aF[FSXZ][p], oyz = a.aF[FSYZ][p], org = a.g_rho[p];
+        a.aF[FSXX][p] = oxx + dxx;
+        a.aF[FSYY][p] = oyy + dyy;
+        a.aF[FSZZ][p] = ozz + dzz;
+        a.aF[FSXY][p] = oxy + dxy;
+        a.aF[FSXZ][p] = oxz + dxz;
+        a.aF[FSYZ][p] = oyz + dyz;
+        a.g_rho[p] = org + gr;
+    }
 }
 
-__device__ static inline void hmean4_grad(float a, float b, float c, float d, float &h, float gr[4])
+// harmonic mean of four moduli (the forward's shear modulus, m_kernel.f90:293-309)
+__device__ static inline float hmean4(float a, float b, float c, float d)
+{
+    const float eps = 1.1920928955078125e-07f;
+    return 4.0f * a * b * c * d / (a * b * c + a * b * d + a * c * d + b * c * d + eps);
+}
+
+// its derivatives with respect to each of the four moduli
+__device__ static inline void hmean4_w(float a, float b, float c, float d, float gr[4])
 {
     const float eps = 1.1920928955078125e-07f;
     const float N = 4.0f * a * b * c * d, Dn = a * b * c + a * b * d + a * c * d + b * c * d + eps;
-    h = N / Dn;
-    const float D2 = Dn * Dn;
-    gr[0] = (4.0f * b * c * d * Dn - N * (b * c + b * d + c * d)) / D2;
-    gr[1] = (4.0f * a * c * d * Dn - N * (a * c + a * d + c * d)) / D2;
-    gr[2] = (4.0f * a * b * d * Dn - N * (a * b + a * d + b * d)) / D2;
-    gr[3] = (4.0f * a * b * c * Dn - N * (a * b + a * c + b * c)) / D2;
+    const float r = 1.0f / (Dn * Dn);
+    gr[0] = (4.0f * b * c * d * Dn - N * (b * c + b * d + c * d)) * r;
+    gr[1] = (4.0f * a * c * d * Dn - N * (a * c + a * d + c * d)) * r;
+    gr[2] = (4.0f * a * b * d * Dn - N * (a * b + a * d + b * d)) * r;
+    gr[3] = (4.0f * a * b * c * Dn - N * (a * b + a * c + b * c)) * r;
 }
 
 // adjoint of the stress update: densities, velocity-auxiliary adjoints,
-// lam and normal mu gradients, shear terms for the mu gather
+// lam and mu gradients of the cell, and the shear-mu terms of its
+// neighbours.  A shear point's modulus is the harmonic mean of four cells
+// (origin o and o + one or two unit steps); the cell adds the derivative
+// for its own corner to g_mu and leaves those for the other three corners
+// of each of its three quads, summed by target offset, in q[0..5]:
+//   q0 -> o+z, q1 -> o+y, q2 -> o+x, q3 -> o+y+z, q4 -> o+x+z, q5 -> o+x+y.
+// NM memory variables per component (a template argument, so their loops
+// unroll); the stores come after every load.
+template <int NM>
 __global__ void __launch_bounds__(256) k_adj_stress_dens(AdjArgs a)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = 1 + blockIdx.z;
-    const Grid &g = a.g;
-    if (i > g.nx || j > g.ny) return;
-    const int kind = cell_kind(a, i, j, k);
-    if (!kind) return;
-    const long long p = gidx(g, i, j, k);
-    const float lam = a.lam[p], mu = a.mu[p];
-    const float sxx = a.aF[FSXX][p], syy = a.aF[FSYY][p], szz = a.aF[FSZZ][p];
-    const float X = a.dt * ((lam + 2.0f * mu) * sxx + lam * (syy + szz));
-    const float Y = a.dt * ((lam + 2.0f * mu) * syy + lam * (sxx + szz));
-    const float Z = a.dt * ((lam + 2.0f * mu) * szz + lam * (sxx + syy));
-    const float *mup = a.mu;
-    float hyz, hxz, hxy, gtmp[4];
-    hmean4_grad(mu, AT(mup, 0, 0, 1), AT(mup, 0, 1, 0), AT(mup, 0, 1, 1), hyz, gtmp);
-    hmean4_grad(mu, AT(mup, 0, 0, 1), AT(mup, 1, 0, 0), AT(mup, 1, 0, 1), hxz, gtmp);
-    hmean4_grad(mu, AT(mup, 1, 0, 0), AT(mup, 0, 1, 0), AT(mup, 1, 1, 0), hxy, gtmp);
-    const float syz = a.aF[FSYZ][p], sxz = a.aF[FSXZ][p], sxy = a.aF[FSXY][p];
-    const float Wyz = a.dt * hyz * syz, Wxz = a.dt * hxz * sxz, Wxy = a.dt * hxy * sxy;
-    const float *Vx = a.fF[FVX], *Vy = a.fF[FVY], *Vz = a.fF[FVZ];
-    float ex, ey, ez, tyz, txz, txy;
-    if (kind == 1) {
-        float ax, bx, ay, by, az, bz;
-        pair_at(a, 0, i, j, k, ax, bx);
-        pair_at(a, 1, i, j, k, ay, by);
-        pair_at(a, 2, i, j, k, az, bz);
-        ex = BWD4(Vx, 1, 0, 0, ax, bx);
-        ey = BWD4(Vy, 0, 1, 0, ay, by);
-        ez = BWD4(Vz, 0, 0, 1, az, bz);
-        tyz = FWD4(Vz, 0, 1, 0, ay, by) + FWD4(Vy, 0, 0, 1, az, bz);
-        txz = FWD4(Vz, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 0, 1, az, bz);
-        txy = FWD4(Vy, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 1, 0, ay, by);
-        if (a.nm > 0) {
-            // transposed memory variables and weighted stress sums
-            const float taup = a.taup[p], taus = a.taus[p];
-            const float tp1 = 1.0f + taup * (1.0f + a.d2), ts1 = 1.0f + taus * (1.0f + a.d2);
-            const float as6[6] = {sxx, syy, szz, syz, sxz, sxy};
-            float sig[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            const long long r = ((long long)(k - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - 1);
-            for (int m = 0; m < a.nm; ++m)
+    ADJ_CELL(a) {
+        const int kind = cell_kind(a, i, j, k);
+        const long long p = gidx(g, i, j, k);
+        const float lam = a.lam[p], mu = a.mu[p];
+        const float sxx = a.aF[FSXX][p], syy = a.aF[FSYY][p], szz = a.aF[FSZZ][p];
+        const float X = a.dt * ((lam + 2.0f * mu) * sxx + lam * (syy + szz));
+        const float Y = a.dt * ((lam + 2.0f * mu) * syy + lam * (sxx + szz));
+        const float Z = a.dt * ((lam + 2.0f * mu) * szz + lam * (sxx + syy));
+        const float *mup = a.mu;
+        const float m001 = AT(mup, 0, 0, 1), m010 = AT(mup, 0, 1, 0), m011 = AT(mup, 0, 1, 1);
+        const float m100 = AT(mup, 1, 0, 0), m101 = AT(mup, 1, 0, 1), m110 = AT(mup, 1, 1, 0);
+        const float hyz = hmean4(mu, m001, m010, m011);
+        const float hxz = hmean4(mu, m001, m100, m101);
+        const float hxy = hmean4(mu, m100, m010, m110);
+        const float syz = a.aF[FSYZ][p], sxz = a.aF[FSXZ][p], sxy = a.aF[FSXY][p];
+        const float Wyz = a.dt * hyz * syz, Wxz = a.dt * hxz * sxz, Wxy = a.dt * hxy * sxy;
+        const float *Vx = a.fF[FVX], *Vy = a.fF[FVY], *Vz = a.fF[FVZ];
+        const float glo = a.g_lam[p], gmo = a.g_mu[p];
+        float ex, ey, ez, tyz, txz, txy;
+        float qyz, qxz, qxy;   // shear-mu terms of the three quads
+        float d[9], dgl, dgm;
+        if (kind == 1) {
+            float ax, bx, ay, by, az, bz;
+            pair_at(a, 0, i, j, k, ax, bx);
+            pair_at(a, 1, i, j, k, ay, by);
+            pair_at(a, 2, i, j, k, az, bz);
+            ex = BWD4(Vx, 1, 0, 0, ax, bx);
+            ey = BWD4(Vy, 0, 1, 0, ay, by);
+            ez = BWD4(Vz, 0, 0, 1, az, bz);
+            tyz = FWD4(Vz, 0, 1, 0, ay, by) + FWD4(Vy, 0, 0, 1, az, bz);
+            txz = FWD4(Vz, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 0, 1, az, bz);
+            txy = FWD4(Vy, 1, 0, 0, ax, bx) + FWD4(Vx, 0, 1, 0, ay, by);
+            if (NM > 0) {
+                // transposed memory variables and weighted stress sums
+                const float taup = a.taup[p], taus = a.taus[p];
+                const float tp1 = 1.0f + taup * (1.0f + a.d2), ts1 = 1.0f + taus * (1.0f + a.d2);
+                const float as6[6] = {sxx, syy, szz, syz, sxz, sxy};
+                const long long r = ((long long)(k - g.kb) * g.nyk + (j - g.jb)) * g.rpitch + (i - 1);
+                float rh[6][NM > 0 ? NM : 1];
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+#pragma unroll
+                    for (int cc = 0; cc < 6; ++cc) rh[cc][m] = a.aR[((long long)cc * NM + m) * a.rn + r];
+                float sig[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+#pragma unroll
+                    for (int cc = 0; cc < 6; ++cc) {
+                        rh[cc][m] = rh[cc][m] + a.d1[m] * a.dt * as6[cc];
+                        sig[cc] -= a.dt * a.c2[m] * rh[cc][m];
+                    }
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+#pragma unroll
+                    for (int cc = 0; cc < 6; ++cc) a.aR[((long long)cc * NM + m) * a.rn + r] = a.c1[m] * rh[cc][m];
+                float P[3], Q[6];
 #pragma unroll
                 for (int cc = 0; cc < 6; ++cc) {
-                    float *ar = a.aR + ((long long)cc * a.nm + m) * a.rn + r;
-                    const float rh = *ar + a.d1[m] * a.dt * as6[cc];
-                    *ar = a.c1[m] * rh;
-                    sig[cc] -= a.dt * a.c2[m] * rh;
+                    if (cc < 3) P[cc] = a.dt * as6[cc] * tp1 + sig[cc] * taup;
+                    Q[cc] = a.dt * as6[cc] * ts1 + sig[cc] * taus;
                 }
-            float P[3], Q[6];
-#pragma unroll
-            for (int cc = 0; cc < 6; ++cc) {
-                if (cc < 3) P[cc] = a.dt * as6[cc] * tp1 + sig[cc] * taup;
-                Q[cc] = a.dt * as6[cc] * ts1 + sig[cc] * taus;
+                const float sP = P[0] + P[1] + P[2], l2m = lam + 2.0f * mu, d3 = ex + ey + ez;
+                d[0] = l2m * sP - 2.0f * mu * (Q[1] + Q[2]);
+                d[1] = l2m * sP - 2.0f * mu * (Q[0] + Q[2]);
+                d[2] = l2m * sP - 2.0f * mu * (Q[0] + Q[1]);
+                d[3] = hyz * Q[3]; d[4] = hyz * Q[3];
+                d[5] = hxz * Q[4]; d[6] = hxz * Q[4];
+                d[7] = hxy * Q[5]; d[8] = hxy * Q[5];
+                dgl = d3 * sP;
+                dgm = 2.0f * d3 * sP - 2.0f * ((d3 - ex) * Q[0] + (d3 - ey) * Q[1] + (d3 - ez) * Q[2]);
+                qyz = tyz * Q[3];
+                qxz = txz * Q[4];
+                qxy = txy * Q[5];
+            } else {
+                d[0] = X; d[1] = Y; d[2] = Z;
+                d[3] = Wyz; d[4] = Wyz; d[5] = Wxz; d[6] = Wxz; d[7] = Wxy; d[8] = Wxy;
+                dgl = a.dt * (ex + ey + ez) * (sxx + syy + szz);
+                dgm = 2.0f * a.dt * (ex * sxx + ey * syy + ez * szz);
+                qyz = a.dt * tyz * syz;
+                qxz = a.dt * txz * sxz;
+                qxy = a.dt * txy * sxy;
             }
-            const float sP = P[0] + P[1] + P[2], l2m = lam + 2.0f * mu, d3 = ex + ey + ez;
-            a.D[0][p] = l2m * sP - 2.0f * mu * (Q[1] + Q[2]);
-            a.D[1][p] = l2m * sP - 2.0f * mu * (Q[0] + Q[2]);
-            a.D[2][p] = l2m * sP - 2.0f * mu * (Q[0] + Q[1]);
-            a.D[3][p] = hyz * Q[3]; a.D[4][p] = hyz * Q[3];
-            a.D[5][p] = hxz * Q[4]; a.D[6][p] = hxz * Q[4];
-            a.D[7][p] = hxy * Q[5]; a.D[8][p] = hxy * Q[5];
-            a.g_lam[p] += d3 * sP;
-            a.g_mu[p] += 2.0f * d3 * sP - 2.0f * ((d3 - ex) * Q[0] + (d3 - ey) * Q[1] + (d3 - ez) * Q[2]);
-            a.q[0][p] = tyz * Q[3];
-            a.q[1][p] = txz * Q[4];
-            a.q[2][p] = txy * Q[5];
-            return;
-        }
-        a.D[0][p] = X; a.D[1][p] = Y; a.D[2][p] = Z;
-        a.D[3][p] = Wyz; a.D[4][p] = Wyz; a.D[5][p] = Wxz; a.D[6][p] = Wxz; a.D[7][p] = Wxy; a.D[8][p] = Wxy;
-    } else {
-        const long long q = ade_index(a, i, j, k);
-        const long long n = a.npml;
-        // derivatives dxVx dyVy dzVz dyVz dzVy dxVz dzVx dxVy dyVx and their auxiliaries
-        const float4 gp[9] = {prof(a.gxc, i), prof(a.gyc, j), prof(a.gzc, k), prof(a.gye, j), prof(a.gze, k),
-                              prof(a.gxe, i), prof(a.gze, k), prof(a.gxe, i), prof(a.gye, j)};
-        const int ai[9] = {A_axVx, A_ayVy, A_azVz, A_ayVz, A_azVy, A_axVz, A_azVx, A_axVy, A_ayVx};
-        const float w[9] = {X, Y, Z, Wyz, Wyz, Wxz, Wxz, Wxy, Wxy};
-        const float der[9] = {BWD2(Vx, 1, 0, 0, a.hx), BWD2(Vy, 0, 1, 0, a.hy), BWD2(Vz, 0, 0, 1, a.hz),
-                              FWD2(Vz, 0, 1, 0, a.hy), FWD2(Vy, 0, 0, 1, a.hz), FWD2(Vz, 1, 0, 0, a.hx),
-                              FWD2(Vx, 0, 0, 1, a.hz), FWD2(Vy, 1, 0, 0, a.hx), FWD2(Vx, 0, 1, 0, a.hy)};
-        float t[9];
+        } else {
+            const long long q = ade_index(a, i, j, k);
+            const long long n = a.npml;
+            // derivatives dxVx dyVy dzVz dyVz dzVy dxVz dzVx dxVy dyVx and their auxiliaries
+            const float4 gp[9] = {prof(a.gxc, i), prof(a.gyc, j), prof(a.gzc, k), prof(a.gye, j), prof(a.gze, k),
+                                  prof(a.gxe, i), prof(a.gze, k), prof(a.gxe, i), prof(a.gye, j)};
+            const int ai[9] = {A_axVx, A_ayVy, A_azVz, A_ayVz, A_azVy, A_axVz, A_azVx, A_axVy, A_ayVx};
+            const float w[9] = {X, Y, Z, Wyz, Wyz, Wxz, Wxz, Wxy, Wxy};
+            const float der[9] = {BWD2(Vx, 1, 0, 0, a.hx), BWD2(Vy, 0, 1, 0, a.hy), BWD2(Vz, 0, 0, 1, a.hz),
+                                  FWD2(Vz, 0, 1, 0, a.hy), FWD2(Vy, 0, 0, 1, a.hz), FWD2(Vz, 1, 0, 0, a.hx),
+                                  FWD2(Vx, 0, 0, 1, a.hz), FWD2(Vy, 1, 0, 0, a.hx), FWD2(Vx, 0, 1, 0, a.hy)};
+            float ahat[9], fa[9], t[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            float *ah = a.aA + (long long)ai[c] * n + q;
-            const float ahat = *ah;
-            a.D[c][p] = gp[c].x * w[c] + gp[c].w * a.dt * ahat;
-            *ah = gp[c].y * w[c] + gp[c].z * ahat;
-            t[c] = gp[c].x * der[c] + gp[c].y * a.fA[(long long)ai[c] * n + q];
+            for (int c = 0; c < 9; ++c) {
+                ahat[c] = a.aA[(long long)ai[c] * n + q];
+                fa[c] = a.fA[(long long)ai[c] * n + q];
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                d[c] = gp[c].x * w[c] + gp[c].w * a.dt * ahat[c];
+                t[c] = gp[c].x * der[c] + gp[c].y * fa[c];
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c) a.aA[(long long)ai[c] * n + q] = gp[c].y * w[c] + gp[c].z * ahat[c];
+            ex = t[0]; ey = t[1]; ez = t[2];
+            tyz = t[3] + t[4]; txz = t[5] + t[6]; txy = t[7] + t[8];
+            dgl = a.dt * (ex + ey + ez) * (sxx + syy + szz);
+            dgm = 2.0f * a.dt * (ex * sxx + ey * syy + ez * szz);
+            qyz = a.dt * tyz * syz;
+            qxz = a.dt * txz * sxz;
+            qxy = a.dt * txy * sxy;
         }
-        ex = t[0]; ey = t[1]; ez = t[2];
-        tyz = t[3] + t[4]; txz = t[5] + t[6]; txy = t[7] + t[8];
+        // quads: yz (o, o+z, o+y, o+y+z), xz (o, o+z, o+x, o+x+z), xy (o, o+x, o+y, o+x+y);
+        // their weights are taken here, late, to keep registers free above
+        float gyz[4], gxz[4], gxy[4];
+        hmean4_w(mu, m001, m010, m011, gyz);
+        hmean4_w(mu, m001, m100, m101, gxz);
+        hmean4_w(mu, m100, m010, m110, gxy);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) a.D[c][p] = d[c];
+        a.g_lam[p] = glo + dgl;
+        a.g_mu[p] = (gmo + dgm) + (qyz * gyz[0] + qxz * gxz[0] + qxy * gxy[0]);
+        a.q[0][p] = qyz * gyz[1] + qxz * gxz[1];
+        a.q[1][p] = qyz * gyz[2] + qxy * gxy[2];
+        a.q[2][p] = qxz * gxz[2] + qxy * gxy[1];
+        a.q[3][p] = qyz * gyz[3];
+        a.q[4][p] = qxz * gxz[3];
+        a.q[5][p] = qxy * gxy[3];
     }
-    a.g_lam[p] += a.dt * (ex + ey + ez) * (sxx + syy + szz);
-    a.g_mu[p] += 2.0f * a.dt * (ex * sxx + ey * syy + ez * szz);
-    a.q[0][p] = a.dt * tyz * syz;
-    a.q[1][p] = a.dt * txz * sxz;
-    a.q[2][p] = a.dt * txy * sxy;
 }
 
-// adjoint of the stress update: velocity adjoints and the shear part of the
-// mu gradient (each shear point's harmonic mean spans four cells)
+// adjoint of the stress update: velocity adjoints and the shear terms of
+// the mu gradient left by the neighbouring cells (0 outside the cells)
 __global__ void __launch_bounds__(256) k_adj_stress_gather(AdjArgs a)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = 1 + blockIdx.z;
-    const Grid &g = a.g;
-    if (i > g.nx || j > g.ny) return;
-    const long long p = gidx(g, i, j, k);
-    a.aF[FVX][p] += gather_bwd(a, a.D[0], 0, i, j, k) + gather_fwd(a, a.D[8], 1, i, j, k) + gather_fwd(a, a.D[6], 2, i, j, k);
-    a.aF[FVY][p] += gather_bwd(a, a.D[1], 1, i, j, k) + gather_fwd(a, a.D[7], 0, i, j, k) + gather_fwd(a, a.D[4], 2, i, j, k);
-    a.aF[FVZ][p] += gather_bwd(a, a.D[2], 2, i, j, k) + gather_fwd(a, a.D[5], 0, i, j, k) + gather_fwd(a, a.D[3], 1, i, j, k);
-    if (!cell_kind(a, i, j, k)) return;
-    // quads whose harmonic mean includes this cell: origin o = this cell
-    // minus the corner offset; corner index per (yz, xz, xy) ordering of
-    // the forward hmean4 arguments
-    const float *mu = a.mu;
-    float gm = 0.0f, h, gr[4];
-    // yz: args (o, o+z, o+y, o+y+z)
-    {
-        const int oy[4] = {0, 0, -1, -1}, oz[4] = {0, -1, 0, -1};
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int ii = i, jj = j + oy[c], kk = k + oz[c];
-            if (!cell_kind(a, ii, jj, kk)) continue;
-            const long long o = gidx(g, ii, jj, kk);
-            hmean4_grad(mu[o], mu[gidx(g, ii, jj, kk + 1)], mu[gidx(g, ii, jj + 1, kk)], mu[gidx(g, ii, jj + 1, kk + 1)], h, gr);
-            gm += a.q[0][o] * gr[c];
-        }
+    ADJ_CELL(a) {
+        const long long p = gidx(g, i, j, k);
+        const long long sx = 1, sy = g.pitch, sz = g.plane;
+        const AxisW wx = axis_w(a.cls, p, sx, a.tA[0], a.tB[0]);
+        const AxisW wy = axis_w(a.cls, p, sy, a.tA[1], a.tB[1]);
+        const AxisW wz = axis_w(a.cls, p, sz, a.tA[2], a.tB[2]);
+        const float vx = gat_bwd(a.D[0], p, sx, wx) + gat_fwd(a.D[8], p, sy, wy) + gat_fwd(a.D[6], p, sz, wz);
+        const float vy = gat_bwd(a.D[1], p, sy, wy) + gat_fwd(a.D[7], p, sx, wx) + gat_fwd(a.D[4], p, sz, wz);
+        const float vz = gat_bwd(a.D[2], p, sz, wz) + gat_fwd(a.D[5], p, sx, wx) + gat_fwd(a.D[3], p, sy, wy);
+        const float gm = a.q[0][p - sz] + a.q[1][p - sy] + a.q[2][p - sx] + a.q[3][p - sy - sz] + a.q[4][p - sx - sz] +
+                         a.q[5][p - sx - sy];
+        const float ox = a.aF[FVX][p], oy = a.aF[FVY][p], oz = a.aF[FVZ][p], om = a.g_mu[p];
+        a.aF[FVX][p] = ox + vx;
+        a.aF[FVY][p] = oy + vy;
+        a.aF[FVZ][p] = oz + vz;
+        a.g_mu[p] = om + gm;
     }
-    // xz: args (o, o+z, o+x, o+x+z)
-    {
-        const int ox[4] = {0, 0, -1, -1}, oz[4] = {0, -1, 0, -1};
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int ii = i + ox[c], jj = j, kk = k + oz[c];
-            if (!cell_kind(a, ii, jj, kk)) continue;
-            const long long o = gidx(g, ii, jj, kk);
-            hmean4_grad(mu[o], mu[gidx(g, ii, jj, kk + 1)], mu[gidx(g, ii + 1, jj, kk)], mu[gidx(g, ii + 1, jj, kk + 1)], h, gr);
-            gm += a.q[1][o] * gr[c];
-        }
-    }
-    // xy: args (o, o+x, o+y, o+x+y)
-    {
-        const int ox[4] = {0, -1, 0, -1}, oy[4] = {0, 0, -1, -1};
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int ii = i + ox[c], jj = j + oy[c], kk = k;
-            if (!cell_kind(a, ii, jj, kk)) continue;
-            const long long o = gidx(g, ii, jj, kk);
-            hmean4_grad(mu[o], mu[gidx(g, ii + 1, jj, kk)], mu[gidx(g, ii, jj + 1, kk)], mu[gidx(g, ii + 1, jj + 1, kk)], h, gr);
-            gm += a.q[2][o] * gr[c];
-        }
-    }
-    a.g_mu[p] += gm;
 }
 
-// adjoint of the velocity traces at one recorded step (m_wav.f90:517-531):
-// one thread walks the stations in order, so stations that share points add
-// deterministically
-__global__ void k_adj_record(AdjArgs a, const int *ist, const int *jst, const int *kst, int nst, const float *wav,
-                             const float *obs, int ntw, int itw, float sc)
+// adjoint of the velocity traces at one recorded step (m_wav.f90:517-531).
+// One thread per distinct target point (field, cell): it adds the residuals
+// of the stations that read the point, in station order, so stations that
+// share points add deterministically.  tgt[t] = field << 40 | gidx;
+// the contributions of target t are src[ptr[t] .. ptr[t+1]), each a
+// station-component w = 3 * station + component.
+__global__ void k_adj_record(AdjArgs a, int ntgt, const long long *tgt, const int *ptr, const int *src,
+                             const float *wav, const float *obs, int ntw, int itw, float sc)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const Grid &g = a.g;
-    for (int n = 0; n < nst; ++n) {
-        const int i = ist[n], j = jst[n], k = kst[n];
-        float r[3];
-        for (int c = 0; c < 3; ++c) {
-            const long long w = ((long long)n * 3 + c) * ntw + itw;
-            r[c] = sc * (wav[w] - obs[w]);
-        }
-        a.aF[FVX][gidx(g, i, j, k)] += r[0];
-        a.aF[FVX][gidx(g, i - 1, j, k)] += r[0];
-        a.aF[FVY][gidx(g, i, j, k)] += r[1];
-        a.aF[FVY][gidx(g, i, j - 1, k)] += r[1];
-        a.aF[FVZ][gidx(g, i, j, k)] -= r[2];
-        a.aF[FVZ][gidx(g, i, j, k - 1)] -= r[2];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntgt) return;
+    const int f = (int)(tgt[t] >> 40);
+    float *dst = (f == FVX ? a.aF[FVX] : f == FVY ? a.aF[FVY] : a.aF[FVZ]) + (tgt[t] & ((1LL << 40) - 1));
+    float acc = *dst;
+    for (int e = ptr[t]; e < ptr[t + 1]; ++e) {
+        const long long w = (long long)src[e] * ntw + itw;
+        const float r = sc * (wav[w] - obs[w]);
+        if (f == FVZ) acc -= r;
+        else acc += r;
     }
+    *dst = acc;
 }
 
 __global__ void k_fill(float *p, long long n)
@@ -463,7 +515,8 @@ struct AdjSizes {
     long long nf, na18, nR;
     long long state() const { return 9 * nf + na18 + nR; }  // checkpoint: V, S, auxiliaries, R
     long long sstep() const { return 9 * nf + na18; }       // per-step record: V_n, S_{n+1}, aV_n, aS_n
-    long long base() const { return 27 * nf + na18 + nR; }  // adjoint fields, densities, q, gradients, sums, aR
+    long long cls() const { return (nf + 3) / 4; }           // point classes, one byte each
+    long long base() const { return 30 * nf + cls() + na18 + nR; }  // adjoint fields, densities, q, gradients, sums, classes, aR
 };
 
 namespace {
@@ -526,8 +579,11 @@ struct AdjWorkspace {
     float *adjF = nullptr, *adjA = nullptr, *dens = nullptr, *qq = nullptr, *adjR = nullptr;
     float *gsrc = nullptr;  // one source's gradients, summed over its steps
     float *grad = nullptr;  // the sums over sources
+    float *cls = nullptr;   // point classes (bytes)
     float *ck = nullptr, *seg = nullptr, *obs = nullptr;
-    long long ck_n = 0, seg_n = 0, obs_n = 0;
+    // record table for 6 nst targets: tgt (long long) | ptr (6 nst + 1 int) | src (6 nst int)
+    float *rec = nullptr;
+    long long ck_n = 0, seg_n = 0, obs_n = 0, rec_n = 0;
     long long bytes = 0;
     int nsum = 0;  // sources summed since the last reset
 };
@@ -580,12 +636,16 @@ int ws_open(swpc_ctx *c)
     w.z = sizes_of(c);
     const long long nf = w.z.nf;
     if (!walloc(c, w.adjF, 9 * nf) || !walloc(c, w.adjA, w.z.na18) || !walloc(c, w.dens, 9 * nf) ||
-        !walloc(c, w.qq, 3 * nf) || !walloc(c, w.gsrc, 3 * nf) || !walloc(c, w.grad, 3 * nf) ||
-        !walloc(c, w.adjR, w.z.nR)) {
+        !walloc(c, w.qq, 6 * nf) || !walloc(c, w.gsrc, 3 * nf) || !walloc(c, w.grad, 3 * nf) ||
+        !walloc(c, w.cls, w.z.cls()) || !walloc(c, w.adjR, w.z.nR)) {
         swpc_adj_release(c);
         return adj_fail(c, SWPC_ENOMEM, "adjoint: device memory for the adjoint fields");
     }
-    if (zero(w.grad, 3 * nf, c->stream) != hipSuccess) return SWPC_EHIP;
+    // the density kernels write every cell and never the halo, where the
+    // gathers then read zeros
+    if (zero(w.grad, 3 * nf, c->stream) != hipSuccess || zero(w.dens, 9 * nf, c->stream) != hipSuccess ||
+        zero(w.qq, 6 * nf, c->stream) != hipSuccess)
+        return SWPC_EHIP;
     return SWPC_OK;
 }
 
@@ -612,8 +672,9 @@ extern "C" int swpc_adj_release(swpc_ctx *c)
     AdjWorkspace &w = *c->adj_ws;
     const long long nf = w.z.nf;
     wfree(c, w.adjF, 9 * nf); wfree(c, w.adjA, w.z.na18); wfree(c, w.dens, 9 * nf);
-    wfree(c, w.qq, 3 * nf); wfree(c, w.gsrc, 3 * nf); wfree(c, w.grad, 3 * nf); wfree(c, w.adjR, w.z.nR);
-    wfree(c, w.ck, w.ck_n); wfree(c, w.seg, w.seg_n); wfree(c, w.obs, w.obs_n);
+    wfree(c, w.qq, 6 * nf); wfree(c, w.gsrc, 3 * nf); wfree(c, w.grad, 3 * nf); wfree(c, w.adjR, w.z.nR);
+    wfree(c, w.cls, w.z.cls());
+    wfree(c, w.ck, w.ck_n); wfree(c, w.seg, w.seg_n); wfree(c, w.obs, w.obs_n); wfree(c, w.rec, w.rec_n);
     delete c->adj_ws;
     c->adj_ws = nullptr;
     return SWPC_OK;
@@ -777,13 +838,74 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
         a.aF[q] = w.adjF + q * nf;
         a.D[q] = w.dens + q * nf;
     }
-    for (int q = 0; q < 3; ++q) a.q[q] = w.qq + q * nf;
+    for (int q = 0; q < 6; ++q) a.q[q] = w.qq + q * nf;
+    a.cls = (const unsigned char *)w.cls;
+    {
+        const float h[3] = {a.hx, a.hy, a.hz};
+        const float *p40[3] = {c->coef.re40x, c->coef.re40y, c->coef.re40z};
+        const float *p41[3] = {c->coef.re41x, c->coef.re41y, c->coef.re41z};
+        for (int ax = 0; ax < 3; ++ax) {
+            const float ta[4] = {0.0f, p40[ax][0], p40[ax][1], h[ax]}, tb[4] = {0.0f, p41[ax][0], p41[ax][1], 0.0f};
+            for (int q = 0; q < 4; ++q) { a.tA[ax][q] = ta[q]; a.tB[ax][q] = tb[q]; }
+        }
+    }
     a.aA = w.adjA;
     a.g_lam = w.gsrc; a.g_mu = w.gsrc + nf; a.g_rho = w.gsrc + 2 * nf;
     if (zero(w.adjF, 9 * nf, s) != hipSuccess || zero(w.adjA, na18, s) != hipSuccess ||
         zero(w.gsrc, 3 * nf, s) != hipSuccess || (nR && zero(w.adjR, nR, s) != hipSuccess))
         return SWPC_EHIP;
-    const dim3 blk(64, 4, 1), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_adj_class, dim3(1024), dim3(256), 0, s, a, (unsigned char *)w.cls);
+    if (hipGetLastError() != hipSuccess) return SWPC_EHIP;
+    const dim3 blk(64, 4, 1), grd(((((g.nx + 63) / 64) * ((g.ny + 3) / 4) + 7) & ~7) * g.nz);
+    // record table: the distinct (field, point) targets of the stations'
+    // trace adjoints, each with its station-components in station order
+    const int ntgt_max = 6 * c->nst;
+    if (!wgrow(c, w.rec, w.rec_n, 4LL * ntgt_max + 1))
+        return adj_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the record table");
+    long long *rec_tgt = (long long *)w.rec;
+    int *rec_ptr = (int *)(w.rec + 2LL * ntgt_max);
+    int *rec_src = rec_ptr + ntgt_max + 1;
+    int ntgt = 0;
+    {
+        std::vector<int> hi(c->nst), hj(c->nst), hk(c->nst);
+        if (hipMemcpy(hi.data(), c->d_ist, c->nst * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hj.data(), c->d_jst, c->nst * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hk.data(), c->d_kst, c->nst * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            return SWPC_EHIP;
+        std::map<long long, int> where;
+        std::vector<long long> tg;
+        std::vector<std::vector<int>> lists;
+        for (int n = 0; n < c->nst; ++n) {
+            const int i = hi[n], j = hj[n], k = hk[n];
+            const struct { int f, i, j, k, cmp; } pts[6] = {{FVX, i, j, k, 0}, {FVX, i - 1, j, k, 0},
+                                                            {FVY, i, j, k, 1}, {FVY, i, j - 1, k, 1},
+                                                            {FVZ, i, j, k, 2}, {FVZ, i, j, k - 1, 2}};
+            for (const auto &pt : pts) {
+                const long long key = (long long)pt.f << 40 | gidx(g, pt.i, pt.j, pt.k);
+                auto it = where.find(key);
+                int id;
+                if (it == where.end()) {
+                    id = (int)tg.size();
+                    where[key] = id;
+                    tg.push_back(key);
+                    lists.emplace_back();
+                } else {
+                    id = it->second;
+                }
+                lists[id].push_back(3 * n + pt.cmp);
+            }
+        }
+        ntgt = (int)tg.size();
+        std::vector<int> ptr(ntgt + 1, 0), src;
+        for (int t = 0; t < ntgt; ++t) {
+            ptr[t + 1] = ptr[t] + (int)lists[t].size();
+            src.insert(src.end(), lists[t].begin(), lists[t].end());
+        }
+        if (hipMemcpy(rec_tgt, tg.data(), ntgt * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(rec_ptr, ptr.data(), (ntgt + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(rec_src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            return SWPC_EHIP;
+    }
     const float sc = 0.5f * c->wav_M0 * c->wav_UC * 1e9f;
     // records of steps n0 .. n1-1 from the current state; per step:
     // V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
@@ -814,11 +936,18 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
             hipLaunchKernelGGL(k_adj_vel_dens, grd, blk, 0, s, a);
             hipLaunchKernelGGL(k_adj_vel_gather, grd, blk, 0, s, a);
             a.fA = st + 9 * nf;            // velocity auxiliaries aV_n
-            hipLaunchKernelGGL(k_adj_stress_dens, grd, blk, 0, s, a);
+            switch (a.nm) {
+#define SWPC_ADJ_CASE(NMV) \
+    case NMV: hipLaunchKernelGGL((k_adj_stress_dens<NMV>), grd, blk, 0, s, a); break;
+                SWPC_ADJ_CASE(0) SWPC_ADJ_CASE(1) SWPC_ADJ_CASE(2) SWPC_ADJ_CASE(3) SWPC_ADJ_CASE(4)
+                SWPC_ADJ_CASE(5) SWPC_ADJ_CASE(6) SWPC_ADJ_CASE(7) SWPC_ADJ_CASE(8)
+#undef SWPC_ADJ_CASE
+            default: break;
+            }
             hipLaunchKernelGGL(k_adj_stress_gather, grd, blk, 0, s, a);
             if (n % c->ntdec_w == 0 && n / c->ntdec_w < c->ntw)
-                hipLaunchKernelGGL(k_adj_record, dim3(1), dim3(64), 0, s, a, c->d_ist, c->d_jst, c->d_kst, c->nst,
-                                   c->d_wav[0], w.obs, c->ntw, n / c->ntdec_w, sc);
+                hipLaunchKernelGGL(k_adj_record, dim3((ntgt + 63) / 64), dim3(64), 0, s, a, ntgt, rec_tgt, rec_ptr,
+                                   rec_src, c->d_wav[0], w.obs, c->ntw, n / c->ntdec_w, sc);
             e = hipGetLastError();
         }
         return e;

@@ -19,15 +19,6 @@
 #define MASK_INT 1
 #define MASK_PML 2
 
-// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
-// Give each XCD a contiguous run of logical tiles so tiles that share halo
-// rows share one L2.
-__device__ static inline int xcd_remap(int b, int n)
-{
-    const int x = b & 7, t = b >> 3;
-    const int q = n >> 3, r = n & 7;
-    return x < r ? x * (q + 1) + t : r * (q + 1) + (x - r) * q + t;
-}
 
 // isign of m_kernel.f90:103-104: true inside the 2nd-order band
 __device__ static inline bool in_band(int k, int kft, int kfb, int kot, int kob)

@@ -39,6 +39,16 @@ __host__ __device__ static inline long long g2idx(const Grid &g, int i, int j)
     return (long long)(j + 2) * g.pitch + (i - 1 + g.ox);
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Give each XCD a contiguous run of logical tiles so tiles that share halo
+// rows share one L2.
+__device__ static inline int xcd_remap(int b, int n)
+{
+    const int x = b & 7, t = b >> 3;
+    const int q = n >> 3, r = n & 7;
+    return x < r ? x * (q + 1) + t : r * (q + 1) + (x - r) * q + t;
+}
+
 // one of the five PML boxes (m_global.f90:334-343 kbeg_a partition)
 struct PmlBox {
     int i0, j0, k0;       // first cell
