@@ -52,3 +52,32 @@ def test_frame_store_bound_is_checked(orc):
     from openswpc_amd.gpu import SwpcError
     with pytest.raises(SwpcError):
         g.step(5, 1)    # would take frame 2
+
+
+def test_section_files_from_the_device(orc, tmp_path):
+    """snpio.write_sections over the frames a GPU run holds: read back, the
+    frames are the oracle's, bit for bit, after the header and medium"""
+    from openswpc_amd import snpio
+    nt, ntdec_s = 12, 4
+    nframe = (nt - 1) // ntdec_s + 1
+    cfg = small_config(nt=nt)
+    m = O.build_model(cfg)
+    par = dict(k0_xy=9, i0_yz=17, j0_xz=15, idec=2, jdec=1, kdec=3, ntdec_s=ntdec_s, nframe=nframe)
+    run = O.OracleRun(m, 32)
+    run.snap_setup(ALL, **par)
+    run.run(nt)
+    g = gpu_from_model(m)
+    g.set_snapshots(ALL, m.kfs, M0=m.M0, UC=m.UC, **par)
+    g.step(1, nt, use_graph=True, block=4)
+    g.synchronize()
+    geo = snpio.SnapGeometry(nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, na=cfg.na, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz,
+                             dt=cfg.dt, xbeg=cfg.xbeg, ybeg=cfg.ybeg, zbeg=cfg.zbeg, idec=2, jdec=1, kdec=3,
+                             ntdec_s=ntdec_s, k0_xy=9, i0_yz=17, j0_xz=15)
+    paths = snpio.write_sections(g, geo, ALL, nframe, str(tmp_path), "gpu", 0, m.rho, m.lam, m.mu, m.kfs, m.kob)
+    g.close()
+    assert len(paths) == len(ALL)
+    for name, path in zip(ALL, paths):
+        r = snpio.read_snp(path)
+        assert r["frames"].shape[0] == nframe
+        for f in range(nframe):
+            assert ulp_diff(r["frames"][f], run.snap_frames[name][f]) == 0, (name, f)
