@@ -134,8 +134,9 @@ def run_batch(args):
     log(f"[bench] batch of {len(specs)} members, {cells / 1e6:.1f} M cells, set-up {time.perf_counter() - t0:.1f}s")
     b = SwpcBatch(sims)
     t0 = time.perf_counter()
-    kc, by, tune_ms = b.tune(trial_steps=3)
-    log(f"[bench] autotuned kchunk={kc} by={by} ({tune_ms:.2f} ms/step in trials, {time.perf_counter() - t0:.1f}s)")
+    kc, by, width, tune_ms = b.tune(trial_steps=3)
+    log(f"[bench] autotuned kchunk={kc} by={by} width={width} ({tune_ms:.2f} ms/step in trials, "
+        f"{time.perf_counter() - t0:.1f}s)")
     ev = HipEvents()
     block = min(args.block, max(args.steps, 1))
     it = 1
@@ -185,7 +186,7 @@ def run_batch(args):
         "config": {"workload": f"batched Green's functions: {len(specs)} uneven grids 96..256 x 96..256 x 96..192 "
                                f"(seed 6), NM=2, 10-cell PML, unit single forces, 8 receivers each, 4 layered "
                                f"profiles", "members": len(specs), "cells": cells, "graph_block": block,
-                   "autotuned": {"kchunk": kc, "by": by}},
+                   "autotuned": {"kchunk": kc, "by": by, "tile_width": width}},
         "step_roofline": {"bytes_per_step": bytes_step, "achieved": round(bytes_step / (ms_batch * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "frac": round(bytes_step / (ms_batch * 1e-3) / HBM_PEAK, 4)},
         "sequential": {"ms_per_step": round(ms_seq, 4), "value": round(cells / (ms_seq * 1e-3) / 1e9, 4),

@@ -273,11 +273,12 @@ const char *swpc_batch_last_error(const swpc_batch *batch);
    order), as replays of a captured graph of `block` steps when use_graph. */
 int swpc_batch_step(swpc_batch *batch, int it0, int nsteps, int use_graph, int block);
 int swpc_batch_synchronize(swpc_batch *batch);
-/* Autotuner: times candidate k-chunk depths and tile rows on trial_steps
-   dry steps (step index 0: no source, no recording) and keeps the fastest.
-   Call it after set-up and before the first step, while every field is zero,
-   so the trials leave the state untouched.  Outputs may be NULL. */
-int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, float *ms_per_step);
+/* Autotuner: times candidate k-chunk depths, tile rows and interior tile
+   widths (64 or 32 cells) on trial_steps dry steps (step index 0: no
+   source, no recording) and keeps the fastest.  Call it after set-up and
+   before the first step, while every field is zero, so the trials leave the
+   state untouched.  Outputs may be NULL. */
+int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, int *width, float *ms_per_step);
 void *swpc_batch_stream(swpc_batch *batch);
 
 /* ---- discrete adjoint (ADE-CFS PML; elastic or with Zener memory variables) ----
@@ -336,8 +337,9 @@ int64_t swpc_device_bytes(const swpc_ctx *ctx);
    (0 = keep); by: rows per 64-wide tile, 1, 2 or 4 (0 = keep); variant:
    kernel variant flags (-1 = keep; default 1): bit 0 = non-temporal loads and
    stores for the single-use streams of the stress update, bit 1 = side-column
-   PML kernels on a second stream alongside the interior ones.  All variants
-   give bit-identical results. */
+   PML kernels on a second stream alongside the interior ones, bit 2 =
+   interior tiles 32 cells wide with 2 by rows (narrow grids idle fewer
+   lanes).  All variants give bit-identical results. */
 int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a

@@ -136,6 +136,9 @@ static bool tables_stale(const swpc_batch *b)
     return false;
 }
 
+// interior tile width of the members (variant bit 2: 32 cells, else 64)
+static int batch_bxw(const swpc_batch *b) { return (b->mem[0]->variant & 4) ? 32 : 64; }
+
 // one step of every member, in the order of src/swpc_3d/main.f90:122-138
 static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
 {
@@ -146,15 +149,15 @@ static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
         if ((e = launch_wav_store(c, it, d_it, s)) != hipSuccess) return e;
         if ((e = launch_snap(c, it, d_it, s)) != hipSuccess) return e;
     }
-    if ((e = launch_batch_phase(0, b->nm, b->nt, b->by, b->d_args[0], b->d_cols, b->ncols, b->d_side, b->nside, s)) !=
-        hipSuccess)
+    if ((e = launch_batch_phase(0, b->nm, b->nt, b->by, batch_bxw(b), b->d_args[0], b->d_cols, b->ncols, b->d_side,
+                                b->nside, s)) != hipSuccess)
         return e;
     for (swpc_ctx *c : b->mem) {
         if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_stress(c, s)) != hipSuccess) return e;
         if ((e = launch_stressglut(c, it, d_it, s)) != hipSuccess) return e;
     }
-    if ((e = launch_batch_phase(1, b->nm, b->nt, b->by, b->d_args[1], b->d_cols, b->ncols, b->d_side, b->nside, s)) !=
-        hipSuccess)
+    if ((e = launch_batch_phase(1, b->nm, b->nt, b->by, batch_bxw(b), b->d_args[1], b->d_cols, b->ncols, b->d_side,
+                                b->nside, s)) != hipSuccess)
         return e;
     for (swpc_ctx *c : b->mem) {
         if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_vel(c, s)) != hipSuccess) return e;
@@ -282,7 +285,8 @@ extern "C" int swpc_batch_synchronize(swpc_batch *b)
 // Autotuner.  Candidate launch shapes (k chunk depth, tile rows) are timed
 // on dry steps: step index 0 injects no source and records nothing, so on
 // the all-zero state before the first step the trials change no field.
-extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, int *by_out, float *ms_out)
+extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, int *by_out, int *width_out,
+                               float *ms_out)
 {
     if (!b || trial_steps < 1) return SWPC_EINVAL;
     BCHK(b, hipSetDevice(b->device));
@@ -290,44 +294,49 @@ extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, 
     if ((rc = batch_sync_members(b))) return rc;
     const int kc_cand[] = {16, 24, 32, 48, 64};
     const int by_cand[] = {2, 4};
-    const int variant = b->mem[0]->variant;
+    const int w_cand[] = {64, 32};
+    const int base = b->mem[0]->variant & ~4;
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
     float best = 1e30f;
-    int best_kc = b->mem[0]->kchunk, best_by = b->by;
-    for (int byv : by_cand)
-        for (int kc : kc_cand) {
-            for (swpc_ctx *c : b->mem)
-                if ((rc = swpc_set_tuning(c, kc, byv, variant)) != SWPC_OK) goto out;
-            b->by = byv;
-            if ((rc = build_tables(b)) != SWPC_OK) goto out;
-            {
-                hipError_t e = enqueue_batch_step(b, 0, nullptr);  // warm
-                if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-                for (int s = 0; s < trial_steps && e == hipSuccess; ++s) e = enqueue_batch_step(b, 0, nullptr);
-                if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
-                float ms = 0.0f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess) {
-                    rc = bfail(b, SWPC_EHIP, "tuning trial: %s", hipGetErrorString(e));
-                    goto out;
-                }
-                ms /= trial_steps;
-                if (ms < best) {
-                    best = ms;
-                    best_kc = kc;
-                    best_by = byv;
+    int best_kc = b->mem[0]->kchunk, best_by = b->by, best_w = batch_bxw(b);
+    for (int w : w_cand)
+        for (int byv : by_cand)
+            for (int kc : kc_cand) {
+                const int variant = base | (w == 32 ? 4 : 0);
+                for (swpc_ctx *c : b->mem)
+                    if ((rc = swpc_set_tuning(c, kc, byv, variant)) != SWPC_OK) goto out;
+                b->by = byv;
+                if ((rc = build_tables(b)) != SWPC_OK) goto out;
+                {
+                    hipError_t e = enqueue_batch_step(b, 0, nullptr);  // warm
+                    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+                    for (int s = 0; s < trial_steps && e == hipSuccess; ++s) e = enqueue_batch_step(b, 0, nullptr);
+                    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+                    if (e == hipSuccess) e = hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                    if (e != hipSuccess) {
+                        rc = bfail(b, SWPC_EHIP, "tuning trial: %s", hipGetErrorString(e));
+                        goto out;
+                    }
+                    ms /= trial_steps;
+                    if (ms < best) {
+                        best = ms;
+                        best_kc = kc;
+                        best_by = byv;
+                        best_w = w;
+                    }
                 }
             }
-        }
     for (swpc_ctx *c : b->mem)
-        if ((rc = swpc_set_tuning(c, best_kc, best_by, variant)) != SWPC_OK) goto out;
+        if ((rc = swpc_set_tuning(c, best_kc, best_by, base | (best_w == 32 ? 4 : 0))) != SWPC_OK) goto out;
     b->by = best_by;
     rc = build_tables(b);
     if (kchunk_out) *kchunk_out = best_kc;
     if (by_out) *by_out = best_by;
+    if (width_out) *width_out = best_w;
     if (ms_out) *ms_out = best;
 out:
     (void)hipEventDestroy(e0);

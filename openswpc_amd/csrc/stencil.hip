@@ -129,6 +129,7 @@ struct StepArgs {
     // interior launch shape
     const int *zc;       // first plane of each k chunk (nbz + 1 entries, last = nz + 1)
     int bx0, nbx, nby, nbz, by;
+    int bxw, byi;        // interior tiles: bxw (64 or 32) cells wide, byi rows (bxw * byi = 64 * by threads)
 };
 
 // ---------------------------------------------------------------------------
@@ -297,8 +298,8 @@ template <int NM, bool NT>
 __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
-    const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
-    const int j = g.jb + byy * a.by + threadIdx.y;
+    const int i = 1 + (a.bx0 + bx) * a.bxw + threadIdx.x;
+    const int j = g.jb + byy * a.byi + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
     const int k0 = a.zc[bz];
     const int k1 = a.zc[bz + 1] - 1;
@@ -474,8 +475,8 @@ __global__ void __launch_bounds__(256) k_stress_int_b(const StepArgs *__restrict
 __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
-    const int i = 1 + (a.bx0 + bx) * BX + threadIdx.x;
-    const int j = g.jb + byy * a.by + threadIdx.y;
+    const int i = 1 + (a.bx0 + bx) * a.bxw + threadIdx.x;
+    const int j = g.jb + byy * a.byi + threadIdx.y;
     if (i < g.ib || i > g.ie || j > g.je) return;
     const int k0 = a.zc[bz];
     const int k1 = a.zc[bz + 1] - 1;
@@ -732,9 +733,13 @@ void fill_step_args(swpc_ctx *c, StepArgs &a, int mask)
     a.by = c->by;
     a.zc = c->d_zchunks;
     const Grid &g = c->g;
-    a.bx0 = (g.ib - 1) / BX;
-    a.nbx = (g.ie - 1) / BX - a.bx0 + 1;
-    a.nby = (g.nyk + a.by - 1) / a.by;
+    // variant bit 2: interior tiles 32 cells wide and twice as many rows, so
+    // narrow grids idle fewer lanes at the tile edges
+    a.bxw = (c->variant & 4) ? 32 : BX;
+    a.byi = a.by * (BX / a.bxw);
+    a.bx0 = (g.ib - 1) / a.bxw;
+    a.nbx = (g.ie - 1) / a.bxw - a.bx0 + 1;
+    a.nby = (g.nyk + a.byi - 1) / a.byi;
     a.nbz = c->nbz;
 }
 
@@ -763,7 +768,7 @@ hipError_t launch_stress_cols(swpc_ctx *c, int mask)
     StepArgs a;
     step_args(c, a, mask);
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
-    dim3 block(BX, a.by), grid(a.nbx * a.nby * a.nbz);
+    dim3 block(a.bxw, a.byi), grid(a.nbx * a.nby * a.nbz);
     const bool nt = (c->variant & 1) != 0;
 #define SWPC_STRESS_CASE(NMV)                                                                       \
     case NMV:                                                                                       \
@@ -785,7 +790,7 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
     step_args(c, a, mask);
     a.zc = c->d_zchunks + c->nbz + 1;
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
-    hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), dim3(BX, a.by), 0, c->stream, a);
+    hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), dim3(a.bxw, a.byi), 0, c->stream, a);
     return hipGetLastError();
 }
 
@@ -907,14 +912,14 @@ void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz)
     nbz = a.nbz;
 }
 
-hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, const void *d_args, const int4 *d_cols, int ncols,
-                              const int4 *d_side, int nside, hipStream_t s)
+hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, int bxw, const void *d_args, const int4 *d_cols,
+                              int ncols, const int4 *d_side, int nside, hipStream_t s)
 {
     const StepArgs *mem = (const StepArgs *)d_args;
-    const dim3 block(BX, by);
+    const dim3 side(BX, by), block(bxw, by * (BX / bxw));
     if (nside > 0) {
-        if (phase == 0) hipLaunchKernelGGL(k_stress_pml_b, dim3(nside), block, 0, s, mem, d_side);
-        else hipLaunchKernelGGL(k_vel_pml_b, dim3(nside), block, 0, s, mem, d_side);
+        if (phase == 0) hipLaunchKernelGGL(k_stress_pml_b, dim3(nside), side, 0, s, mem, d_side);
+        else hipLaunchKernelGGL(k_vel_pml_b, dim3(nside), side, 0, s, mem, d_side);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

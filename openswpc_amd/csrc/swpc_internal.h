@@ -223,8 +223,8 @@ hipError_t launch_counter_advance(int *d, hipStream_t s);
 size_t step_args_size();
 void fill_batch_args(swpc_ctx *c, void *out, int phase);
 void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz);
-hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, const void *d_args, const int4 *d_cols, int ncols,
-                              const int4 *d_side, int nside, hipStream_t s);
+hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, int bxw, const void *d_args, const int4 *d_cols,
+                              int ncols, const int4 *d_side, int nside, hipStream_t s);
 hipError_t launch_vmax(swpc_ctx *c);
 hipError_t launch_ref_to_dev(swpc_ctx *c, const float *src_ref, float *dst, hipStream_t s);
 hipError_t launch_dev_to_ref(swpc_ctx *c, const float *src, float *dst_ref, hipStream_t s);

@@ -122,7 +122,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_batch_last_error.restype = C.c_char_p
     L.swpc_batch_step.argtypes = [vp, i, i, i, i]
     L.swpc_batch_synchronize.argtypes = [vp]
-    L.swpc_batch_tune.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    L.swpc_batch_tune.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_float)]
     L.swpc_batch_stream.argtypes = [vp]
     L.swpc_batch_stream.restype = vp
     _lib = L
@@ -408,9 +409,11 @@ class SwpcBatch:
         self._chk(self.L.swpc_batch_synchronize(self.h))
 
     def tune(self, trial_steps=3):
-        kc, by, ms = C.c_int(), C.c_int(), C.c_float()
-        self._chk(self.L.swpc_batch_tune(self.h, int(trial_steps), C.byref(kc), C.byref(by), C.byref(ms)))
-        return kc.value, by.value, ms.value
+        """(k chunk, tile rows, interior tile width, ms per trial step) of the fastest shape"""
+        kc, by, w, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        self._chk(self.L.swpc_batch_tune(self.h, int(trial_steps), C.byref(kc), C.byref(by), C.byref(w),
+                                         C.byref(ms)))
+        return kc.value, by.value, w.value, ms.value
 
     def stream(self):
         return self.L.swpc_batch_stream(self.h)

@@ -284,11 +284,11 @@ module m_swpc_gpu
             import :: c_int, c_ptr
             type(c_ptr), value :: batch
         end function
-        integer(c_int) function swpc_batch_tune(batch, trial_steps, kchunk, by, ms) bind(C, name='swpc_batch_tune')
+        integer(c_int) function swpc_batch_tune(batch, trial_steps, kchunk, by, width, ms) bind(C, name='swpc_batch_tune')
             import :: c_int, c_ptr, c_float
             type(c_ptr), value :: batch
             integer(c_int), value :: trial_steps
-            integer(c_int), intent(out) :: kchunk, by
+            integer(c_int), intent(out) :: kchunk, by, width
             real(c_float), intent(out) :: ms
         end function
         type(c_ptr) function swpc_batch_stream(batch) bind(C, name='swpc_batch_stream')

@@ -32,8 +32,8 @@ def _check_members(gs, runs):
         assert np.any(run.field("Vz") != 0), n
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_batch_members_match_their_oracles(orc, graph):
+@pytest.mark.parametrize("graph,variant", [(False, 1), (True, 1), (True, 5)])
+def test_batch_members_match_their_oracles(orc, graph, variant):
     nt = 18
     cfgs = _members()
     models = [O.build_model(c) for c in cfgs]
@@ -42,7 +42,7 @@ def test_batch_members_match_their_oracles(orc, graph):
         r = O.OracleRun(m, 32)
         r.run(nt)
         runs.append(r)
-    gs = [gpu_from_model(m) for m in models]
+    gs = [gpu_from_model(m, variant=variant) for m in models]
     b = SwpcBatch(gs)
     if graph:
         b.step(1, 12, use_graph=True, block=4)
@@ -65,8 +65,8 @@ def test_tuner_dry_trials_leave_state_untouched(orc):
         runs.append(r)
     gs = [gpu_from_model(m) for m in models]
     b = SwpcBatch(gs)
-    kc, by, ms = b.tune(trial_steps=2)
-    assert kc > 0 and by in (2, 4) and ms > 0
+    kc, by, width, ms = b.tune(trial_steps=2)
+    assert kc > 0 and by in (2, 4) and width in (32, 64) and ms > 0
     for g in gs:
         for name in ("Vx", "Sxx"):
             assert not np.any(g.get_field(name))
