@@ -531,7 +531,7 @@ def cpu_baseline(target_s=12.0):
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     nx, ny, nz = 192, 192, 128
-    cfg = O.Config(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, dt=0.0115, nt=400, na=20, nm=3, zbeg=-1.0,
+    cfg = O.Config(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, dt=0.0115, nt=2000, na=20, nm=3, zbeg=-1.0,
                    vmodel="lgm", layers=[[0.0, 2.3, 3.464, 2.0, 200, 100], [60.0, 3.2, 7.794, 4.5, 450, 225]],
                    sources=[dict(x=0.0, y=0.0, z=10.0, t0=0.0, tr=1.0, mo=1e15, strike=30.0, dip=60.0, rake=90.0)])
     m = O.build_model(cfg)

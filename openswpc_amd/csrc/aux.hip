@@ -6,6 +6,8 @@
 // applies its contributions in source order, exactly the sequence of
 // subtractions the reference performs (m_source.f90:798-841), so no atomics
 // and no order-dependent rounding.
+#include <cstring>
+
 #include "swpc_internal.h"
 
 // --------------------------------------------------------------------------
@@ -27,11 +29,10 @@ struct SrcArgs {
 
 __device__ static inline int step_of(const SrcArgs &a) { return a.d_it ? *a.d_it : a.it; }
 
-__global__ void k_stressglut(SrcArgs a)
+// target t of one context's stress glut at step it
+__device__ static inline void stressglut_body(const SrcArgs &a, int t, int it)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ntarget) return;
-    const int it = step_of(a);
     if (it < 1 || it > a.nt) return;
     const SrcTarget tg = a.tg[t];
     float *f = a.fld[tg.field];
@@ -48,12 +49,12 @@ __global__ void k_stressglut(SrcArgs a)
     f[tg.idx] = v;
 }
 
+__global__ void k_stressglut(SrcArgs a) { stressglut_body(a, blockIdx.x * blockDim.x + threadIdx.x, step_of(a)); }
+
 // source__bodyforce, m_source.f90:850-898
-__global__ void k_bodyforce(SrcArgs a)
+__device__ static inline void bodyforce_body(const SrcArgs &a, int t, int it)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ntarget) return;
-    const int it = step_of(a);
     if (it < 1 || it > a.nt) return;
     const SrcTarget tg = a.tg[t];
     float *f = a.fld[tg.field];
@@ -67,6 +68,18 @@ __global__ void k_bodyforce(SrcArgs a)
         v = v + b * fc * stime * a.dt_dxyz / 2.0f;
     }
     f[tg.idx] = v;
+}
+
+__global__ void k_bodyforce(SrcArgs a) { bodyforce_body(a, blockIdx.x * blockDim.x + threadIdx.x, step_of(a)); }
+
+// batched launches: context m = blockIdx.y, its arguments in HBM
+__global__ void k_stressglut_b(const SrcArgs *__restrict__ args, int it, const int *d_it)
+{
+    stressglut_body(args[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x, d_it ? *d_it : it);
+}
+__global__ void k_bodyforce_b(const SrcArgs *__restrict__ args, int it, const int *d_it)
+{
+    bodyforce_body(args[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x, d_it ? *d_it : it);
 }
 
 static void srcargs_fill(swpc_ctx *c, SrcArgs &a, int it, const int *d_it)
@@ -120,11 +133,10 @@ struct WavArgs {
     const int *d_it;
 };
 
-__global__ void k_wav_store(WavArgs a)
+// station n of one context at step it
+__device__ static inline void wav_store_body(const WavArgs &a, int n, int it)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.nst) return;
-    const int it = a.d_it ? *a.d_it : a.it;
     const Grid &g = a.g;
     const int i = a.ist[n], j = a.jst[n], k = a.kst[n];
     const long long P = g.plane;
@@ -202,10 +214,18 @@ __global__ void k_wav_store(WavArgs a)
     }
 }
 
-hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+__global__ void k_wav_store(WavArgs a)
 {
-    if (c->nst <= 0) return hipSuccess;
-    WavArgs a;
+    wav_store_body(a, blockIdx.x * blockDim.x + threadIdx.x, a.d_it ? *a.d_it : a.it);
+}
+
+__global__ void k_wav_store_b(const WavArgs *__restrict__ args, int it, const int *d_it)
+{
+    wav_store_body(args[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x, d_it ? *d_it : it);
+}
+
+static void wavargs_fill(swpc_ctx *c, WavArgs &a, int it, const int *d_it)
+{
     a.g = c->g;
     a.Vx = c->fields[SWPC_VX]; a.Vy = c->fields[SWPC_VY]; a.Vz = c->fields[SWPC_VZ];
     a.Sxx = c->fields[SWPC_SXX]; a.Syy = c->fields[SWPC_SYY]; a.Szz = c->fields[SWPC_SZZ];
@@ -220,7 +240,47 @@ hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
     a.r41x = 1.0f / 24.0f / dx; a.r41y = 1.0f / 24.0f / dy; a.r41z = 1.0f / 24.0f / dz;
     a.it = it;
     a.d_it = d_it;
+}
+
+hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+{
+    if (c->nst <= 0) return hipSuccess;
+    WavArgs a;
+    wavargs_fill(c, a, it, d_it);
     hipLaunchKernelGGL(k_wav_store, dim3((c->nst + 63) / 64), dim3(64), 0, (s ? s : c->stream), a);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// batched receivers and sources: one launch for every member context
+// --------------------------------------------------------------------------
+size_t batch_aux_size(int kind) { return kind == BATCH_AUX_WAV ? sizeof(WavArgs) : sizeof(SrcArgs); }
+
+int batch_aux_fill(swpc_ctx *c, int kind, void *out)
+{
+    if (kind == BATCH_AUX_WAV) {
+        if (c->nst <= 0) return 0;
+        WavArgs a;
+        wavargs_fill(c, a, 0, nullptr);
+        memcpy(out, &a, sizeof(a));
+        return c->nst;
+    }
+    const bool bf = kind == BATCH_AUX_BODYFORCE;
+    if (c->nsrc <= 0 || c->ntarget <= 0 || (c->bf_mode != 0) != bf) return 0;
+    SrcArgs a;
+    srcargs_fill(c, a, 0, nullptr);
+    memcpy(out, &a, sizeof(a));
+    return c->ntarget;
+}
+
+hipError_t launch_batch_aux(int kind, const void *d_args, int nmem, int nmax, int it, const int *d_it, hipStream_t s)
+{
+    if (nmem <= 0 || nmax <= 0) return hipSuccess;
+    const dim3 grid((nmax + 63) / 64, nmem), block(64);
+    if (kind == BATCH_AUX_WAV) hipLaunchKernelGGL(k_wav_store_b, grid, block, 0, s, (const WavArgs *)d_args, it, d_it);
+    else if (kind == BATCH_AUX_BODYFORCE)
+        hipLaunchKernelGGL(k_bodyforce_b, grid, block, 0, s, (const SrcArgs *)d_args, it, d_it);
+    else hipLaunchKernelGGL(k_stressglut_b, grid, block, 0, s, (const SrcArgs *)d_args, it, d_it);
     return hipGetLastError();
 }
 

@@ -30,6 +30,8 @@ struct swpc_batch {
     void *d_args[2];               // StepArgs of every member: stress, velocity
     int4 *d_cols, *d_side;
     int ncols, nside;
+    void *d_aux[BATCH_AUX_N];      // receiver, stress-glut and body-force arguments of the members that have them
+    int naux[BATCH_AUX_N], maxaux[BATCH_AUX_N];
     int *d_it;
     hipGraphExec_t graph;
     int graph_block;
@@ -74,6 +76,11 @@ static void free_tables(swpc_batch *b)
     bfree(p);
     b->d_cols = b->d_side = nullptr;
     b->ncols = b->nside = 0;
+    for (int k = 0; k < BATCH_AUX_N; ++k) {
+        bfree(b->d_aux[k]);
+        b->d_aux[k] = nullptr;
+        b->naux[k] = b->maxaux[k] = 0;
+    }
 }
 
 // (re)build the argument arrays and tile tables from the members' current state
@@ -125,6 +132,25 @@ static int build_tables(swpc_batch *b)
     }
     b->ncols = (int)cols.size();
     b->nside = (int)side.size();
+    // receivers and sources: one launch per kind for every member that has them
+    for (int k = 0; k < BATCH_AUX_N; ++k) {
+        const size_t sz = batch_aux_size(k);
+        std::vector<char> buf(sz * n);
+        int cnt = 0, mx = 0;
+        for (int m = 0; m < n; ++m) {
+            const int t = batch_aux_fill(b->mem[m], k, buf.data() + sz * cnt);
+            if (t > 0) {
+                ++cnt;
+                mx = std::max(mx, t);
+            }
+        }
+        if (cnt > 0) {
+            BCHK(b, hipMalloc(&b->d_aux[k], sz * cnt));
+            BCHK(b, hipMemcpy(b->d_aux[k], buf.data(), sz * cnt, hipMemcpyHostToDevice));
+        }
+        b->naux[k] = cnt;
+        b->maxaux[k] = mx;
+    }
     for (int m = 0; m < n; ++m) b->gen[m] = b->mem[m]->gen;
     return SWPC_OK;
 }
@@ -146,24 +172,31 @@ static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
     hipStream_t s = b->stream;
     for (swpc_ctx *c : b->mem) {
         if ((e = launch_green_store(c, it, d_it, s)) != hipSuccess) return e;
-        if ((e = launch_wav_store(c, it, d_it, s)) != hipSuccess) return e;
         if ((e = launch_snap(c, it, d_it, s)) != hipSuccess) return e;
     }
+    if ((e = launch_batch_aux(BATCH_AUX_WAV, b->d_aux[BATCH_AUX_WAV], b->naux[BATCH_AUX_WAV],
+                              b->maxaux[BATCH_AUX_WAV], it, d_it, s)) != hipSuccess)
+        return e;
     if ((e = launch_batch_phase(0, b->nm, b->nt, b->by, batch_bxw(b), b->d_args[0], b->d_cols, b->ncols, b->d_side,
                                 b->nside, s)) != hipSuccess)
         return e;
-    for (swpc_ctx *c : b->mem) {
+    for (swpc_ctx *c : b->mem)
         if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_stress(c, s)) != hipSuccess) return e;
-        if ((e = launch_stressglut(c, it, d_it, s)) != hipSuccess) return e;
-    }
+    // each member's sponge before its stress glut, as in its own step
+    if ((e = launch_batch_aux(BATCH_AUX_GLUT, b->d_aux[BATCH_AUX_GLUT], b->naux[BATCH_AUX_GLUT],
+                              b->maxaux[BATCH_AUX_GLUT], it, d_it, s)) != hipSuccess)
+        return e;
     if ((e = launch_batch_phase(1, b->nm, b->nt, b->by, batch_bxw(b), b->d_args[1], b->d_cols, b->ncols, b->d_side,
                                 b->nside, s)) != hipSuccess)
         return e;
-    for (swpc_ctx *c : b->mem) {
+    for (swpc_ctx *c : b->mem)
         if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_vel(c, s)) != hipSuccess) return e;
-        if ((e = launch_bodyforce(c, it, d_it, s)) != hipSuccess) return e;
+    // sponge, body force, then the Green's-function source, per member as in its own step
+    if ((e = launch_batch_aux(BATCH_AUX_BODYFORCE, b->d_aux[BATCH_AUX_BODYFORCE], b->naux[BATCH_AUX_BODYFORCE],
+                              b->maxaux[BATCH_AUX_BODYFORCE], it, d_it, s)) != hipSuccess)
+        return e;
+    for (swpc_ctx *c : b->mem)
         if ((e = launch_green_source(c, it, d_it, s)) != hipSuccess) return e;
-    }
     if (d_it) e = launch_counter_advance(b->d_it, s);
     return e;
 }

@@ -223,6 +223,12 @@ hipError_t launch_counter_advance(int *d, hipStream_t s);
 size_t step_args_size();
 void fill_batch_args(swpc_ctx *c, void *out, int phase);
 void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz);
+// batched receivers and sources (aux.hip): kinds of per-member argument arrays
+enum { BATCH_AUX_WAV = 0, BATCH_AUX_GLUT = 1, BATCH_AUX_BODYFORCE = 2, BATCH_AUX_N = 3 };
+size_t batch_aux_size(int kind);
+// fills one member's arguments of a kind; returns its thread count (0: nothing to launch)
+int batch_aux_fill(swpc_ctx *c, int kind, void *out);
+hipError_t launch_batch_aux(int kind, const void *d_args, int nmem, int nmax, int it, const int *d_it, hipStream_t s);
 hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, int bxw, const void *d_args, const int4 *d_cols,
                               int ncols, const int4 *d_side, int nside, hipStream_t s);
 hipError_t launch_vmax(swpc_ctx *c);
