@@ -31,9 +31,13 @@
  *   orc_absorb_c_update_* - the taper profile by its closed form
  *       (tests/test_oracle_setup.py); the multiply order of its application
  *       is parity unpinned beyond reading.
- *   orc_wav_store stress and strain samples - parity unpinned beyond reading
- *       (checked against the GPU only); the same sampling conventions are
- *       pinned for orc_snap_write.
+ *   orc_wav_store stress and strain samples - linear-field closed forms
+ *       (tests/test_oracle_wav.py): the strain accumulators of a linear
+ *       velocity field equal dt times its symmetric gradient in the order
+ *       xx yy zz yz xz xy, the stress samples of linear stress fields equal
+ *       the field at the cell centre, decimated samples come at the right
+ *       steps; a swapped pairing is detected.  The output convention (no
+ *       sign flip for strain and stress, unlike Vz) is pinned by reading.
  *   orc_snap_write - linear-field closed forms, index sampling and
  *       accumulation checks (tests/test_oracle_snap.py).
  */
