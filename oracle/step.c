@@ -28,9 +28,10 @@
  *       reciprocal run forced at the station, to 1e-11, with a control
  *       that a misplaced shear split is detected; mirror symmetry pins the
  *       shear operand pairing as well.
- *   orc_absorb_c_update_* - the taper profile by its closed form
- *       (tests/test_oracle_setup.py); the multiply order of its application
- *       is parity unpinned beyond reading.
+ *   orc_absorb_c_update_* - the taper profile by its closed form, and
+ *       its application by the taper at each field's own staggered point
+ *       (tests/test_oracle_setup.py, with a control); the FP32 multiply
+ *       order of the application is pinned by reading only.
  *   orc_wav_store stress and strain samples - linear-field closed forms
  *       (tests/test_oracle_wav.py): the strain accumulators of a linear
  *       velocity field equal dt times its symmetric gradient in the order

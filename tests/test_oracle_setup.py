@@ -182,3 +182,48 @@ def test_zener_recursion_coefficients(orc):
     for _ in range(20000):
         R = c1 * R - c2 * 1.0 * dt
     np.testing.assert_allclose(R, -1.0 / nm, rtol=1e-4)
+
+
+def test_cerjan_applied_at_staggered_points(orc):
+    """absorb_c__update_stress/_vel (m_absorb_c.f90:113-198) multiply each
+    field by the taper at that field's own staggered point: a unit field
+    becomes G(x) G(y) G(z) with x at i - 1/2 (cell) or i (edge) cells from
+    the outer face, per axis as the field is staggered, and no taper at the
+    top.  FP64; a control swaps the profile of one axis."""
+    cfg = O.Config(nx=22, ny=20, nz=18, na=5, nm=0, abc="cerjan", dx=0.5, dy=0.4, dz=0.3, zbeg=-1.0, sources=[],
+                   layers=[[0.0, 2.7, 5.5, 3.2, 1e6, 1e6]])
+    run = O.OracleRun(O.build_model(cfg), 64)
+    for name in O.FIELDS_V + O.FIELDS_S:
+        run.arr[name][...] = 1.0
+    run.L.orc_absorb_c_update_stress(run.ptr)
+    run.L.orc_absorb_c_update_vel(run.ptr)
+
+    def taper(n, h, top, edge):
+        """G at cells 1..n: distance from the outer face of the point at
+        i - 1/2 (cell) or i (edge) cells"""
+        Lw, na = cfg.na * h, cfg.na
+        out = np.ones(n)
+        for i in range(1, n + 1):
+            if i <= na and top:
+                d = (i if edge else i - 0.5) * h
+            elif i >= n - na + 1:
+                d = (n - i + (0.5 if edge else 1.0)) * h
+            else:
+                continue
+            out[i - 1] = np.exp(-0.09 * (1 - d / Lw) ** 2)
+        return out
+
+    gx = {e: taper(cfg.nx, cfg.dx, 1, e) for e in (False, True)}
+    gy = {e: taper(cfg.ny, cfg.dy, 1, e) for e in (False, True)}
+    gz = {e: taper(cfg.nz, cfg.dz, 0, e) for e in (False, True)}
+    stag = {"Vx": (1, 0, 0), "Vy": (0, 1, 0), "Vz": (0, 0, 1), "Sxx": (0, 0, 0), "Syy": (0, 0, 0), "Szz": (0, 0, 0),
+            "Syz": (0, 1, 1), "Sxz": (1, 0, 1), "Sxy": (1, 1, 0)}
+    s = (slice(OFF + 1, OFF + 1 + cfg.ny), slice(OFF + 1, OFF + 1 + cfg.nx), slice(OFF + 1, OFF + 1 + cfg.nz))
+    for name, (ex, ey, ez) in stag.items():
+        want = gy[bool(ey)][:, None, None] * gx[bool(ex)][None, :, None] * gz[bool(ez)][None, None, :]
+        got = run.arr[name][s]
+        np.testing.assert_allclose(got, want, rtol=2e-6, err_msg=name)
+        if name == "Syz":
+            # control: the cell-centred y profile in place of the edge one is seen
+            bad = gy[False][:, None, None] * gx[False][None, :, None] * gz[True][None, None, :]
+            assert not np.allclose(got, bad, rtol=1e-4)
