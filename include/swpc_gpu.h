@@ -339,7 +339,8 @@ int64_t swpc_device_bytes(const swpc_ctx *ctx);
    stores for the single-use streams of the stress update, bit 1 = side-column
    PML kernels on a second stream alongside the interior ones, bit 2 =
    interior tiles 32 cells wide with 2 by rows (narrow grids idle fewer
-   lanes).  All variants give bit-identical results. */
+   lanes), bit 4 = the adjoint copies its step records instead of stepping
+   out of place into them.  All variants give bit-identical results. */
 int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a

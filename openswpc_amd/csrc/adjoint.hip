@@ -742,10 +742,15 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     const long long per_state = w.z.state(), sstep = w.z.sstep();
     const long long nck = scheme_ck_slots(nt, c1, c2), nout = (nt + c1 - 1) / c1;
     const long long nobs = (long long)c->ntw * 3 * c->nst;
+    const float *seg_before = w.seg;
     if (!wgrow(c, w.ck, w.ck_n, nck * per_state) || !wgrow(c, w.seg, w.seg_n, scheme_seg_slots(nt, c1, c2) * sstep) ||
         !wgrow(c, w.obs, w.obs_n, nobs))
         return adj_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the checkpoints "
                                         "(swpc_adj_plan finds a scheme that fits)");
+    // the steps recorded in place of the records never write the halo of a
+    // field, which the next step's stencils read as zero: a new store starts
+    // at zero and its halos stay so
+    if (w.seg != seg_before && zero(w.seg, w.seg_n, s) != hipSuccess) return SWPC_EHIP;
     float *outer_ck = w.ck, *inner_ck = w.ck + nout * per_state;
     if (hipMemcpyAsync(w.obs, obs, nobs * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) return SWPC_EHIP;
 
@@ -909,21 +914,68 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     const float sc = 0.5f * c->wav_M0 * c->wav_UC * 1e9f;
     // records of steps n0 .. n1-1 from the current state; per step:
     // V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
+    // Records of steps n0 .. n1-1 from the current state.  Record n holds
+    // V_n (3 fields) | S_{n+1} (6 fields) | aV_n, aS_n (18 x npml, laid out as
+    // the auxiliaries).  The steps run out of place: step n reads V_n, S_n and
+    // the auxiliaries from the records and writes S_{n+1} into record n and
+    // V_{n+1} and the new auxiliaries into record n + 1 (the last step into
+    // the context's own arrays), so no field is copied but those of step n0.
+    // Variant bit 4 keeps the copying form (the same bits; for checks).
     auto record = [&](int n0, int n1) -> hipError_t {
         hipError_t e = hipSuccess;
+        if (c->variant & 16) {
+            for (int n = n0; n < n1 && e == hipSuccess; ++n) {
+                float *st = w.seg + (size_t)(n - n0) * sstep;
+                const int it = n + 1;
+                for (int q = 0; q < 3 && e == hipSuccess; ++q)
+                    e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(st + 9 * nf, c->ade, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = stress_phase(it);
+                for (int q = 3; q < 9 && e == hipSuccess; ++q)
+                    e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(st + 9 * nf + na9, c->ade + na9, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = vel_phase(it);
+            }
+            return e;
+        }
+        auto V = [&](int n) { return w.seg + (size_t)(n - n0) * sstep; };
+        auto S = [&](int n) { return V(n) + 3 * nf; };
+        auto A = [&](int n) { return V(n) + 9 * nf; };
+        float *real[SWPC_NFIELD];
+        for (int q = 0; q < SWPC_NFIELD; ++q) real[q] = c->fields[q];
+        float *const real_ade = c->ade;
+        for (int q = 0; q < 3 && e == hipSuccess; ++q)
+            e = hipMemcpyAsync(V(n0) + q * nf, real[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(A(n0), real_ade, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
         for (int n = n0; n < n1 && e == hipSuccess; ++n) {
-            float *st = w.seg + (size_t)(n - n0) * sstep;
             const int it = n + 1;
-            for (int q = 0; q < 3 && e == hipSuccess; ++q)
-                e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(st + 9 * nf, c->ade, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = stress_phase(it);
-            for (int q = 3; q < 9 && e == hipSuccess; ++q)
-                e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(st + 9 * nf + na9, c->ade + na9, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+            const bool last = n == n1 - 1;
+            float *aux_out = last ? real_ade : A(n + 1);
+            // stress phase: V_n read, S_n -> S_{n+1} in record n, aV_n -> aV_{n+1}
+            for (int q = 0; q < 3; ++q) c->fields_in[q] = c->fields[q] = V(n) + q * nf;
+            for (int q = 3; q < 9; ++q) {
+                c->fields[q] = S(n) + (q - 3) * nf;
+                c->fields_in[q] = n == n0 ? real[q] : S(n - 1) + (q - 3) * nf;
+            }
+            c->ade = aux_out;
+            c->ade_in = A(n);
+            e = stress_phase(it);
+            // velocity phase: S_{n+1} read, V_n -> V_{n+1}, aS_n -> aS_{n+1}
+            for (int q = 0; q < 3; ++q) {
+                c->fields_in[q] = V(n) + q * nf;
+                c->fields[q] = last ? real[q] : V(n + 1) + q * nf;
+            }
+            for (int q = 3; q < 9; ++q) c->fields_in[q] = c->fields[q] = S(n) + (q - 3) * nf;
             if (e == hipSuccess) e = vel_phase(it);
         }
+        for (int q = 0; q < SWPC_NFIELD; ++q) {
+            c->fields[q] = real[q];
+            c->fields_in[q] = nullptr;
+        }
+        c->ade = real_ade;
+        c->ade_in = nullptr;
         return e;
     };
     // the adjoint of steps n1-1 down to n0 from their records

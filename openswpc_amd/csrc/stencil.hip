@@ -112,6 +112,9 @@ __device__ static inline void st1(float *p, float v)
 struct StepArgs {
     Grid g;
     FieldPtrs f;
+    // where the updated fields' old values are read: f itself (in place), or
+    // other buffers, so a step can write its results next to its inputs
+    FieldPtrs fin;
     MediumPtrs m;
     float *R;
     ViscoCoef v;
@@ -119,6 +122,7 @@ struct StepArgs {
     float dt;
     // PML
     float *ade;
+    const float *ade_in;  // old auxiliaries (ade when in place)
     long long npml;
     PmlBox box[5];
     const float4 *gxc, *gxe, *gyc, *gye, *gzc, *gze;
@@ -144,7 +148,7 @@ __device__ static inline PmlBox pick_box(const StepArgs &a, int q)
 // absorb_p__update_stress for one cell, m_absorb_p.f90:455-517.  The caller
 // passes its z-queue values: V at k (and k+1 for Vx, Vy; k-1 for Vz) and mu
 // on planes k, k+1.
-template <bool NT>
+template <bool NT, bool OOP>
 __device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, long long q, int k, float vx_0,
                                                 float vx_p1, float vy_0, float vy_p1, float vz_m1, float vz_0,
                                                 float mu00, float mu00n, float mu10, float mu10n, float mu01,
@@ -158,6 +162,7 @@ __device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, 
     const float *__restrict__ Vz = a.f.Vz;
     const long long n = a.npml;
     float *__restrict__ A = a.ade + q;
+    const float *__restrict__ AI = OOP ? a.ade_in + q : A;
     const float dt = a.dt;
     (void)P;
     const float vx_im1 = Vx[p - 1], vy_jm1 = Vy[p - Y];
@@ -165,16 +170,22 @@ __device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, 
     const float vx_jp1 = Vx[p + Y], vz_jp1 = Vz[p + Y];
     const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
     const float lam0 = ld1<NT>(a.m.lam + p);
-    const float axVx = ld1<NT>(A + A_axVx * n), ayVy = ld1<NT>(A + A_ayVy * n), azVz = ld1<NT>(A + A_azVz * n);
-    const float ayVx = ld1<NT>(A + A_ayVx * n), azVx = ld1<NT>(A + A_azVx * n);
-    const float axVy = ld1<NT>(A + A_axVy * n), azVy = ld1<NT>(A + A_azVy * n);
-    const float axVz = ld1<NT>(A + A_axVz * n), ayVz = ld1<NT>(A + A_ayVz * n);
+    const float axVx = ld1<NT>(AI + A_axVx * n), ayVy = ld1<NT>(AI + A_ayVy * n), azVz = ld1<NT>(AI + A_azVz * n);
+    const float ayVx = ld1<NT>(AI + A_ayVx * n), azVx = ld1<NT>(AI + A_azVx * n);
+    const float axVy = ld1<NT>(AI + A_axVy * n), azVy = ld1<NT>(AI + A_azVy * n);
+    const float axVz = ld1<NT>(AI + A_axVz * n), ayVz = ld1<NT>(AI + A_ayVz * n);
     float *__restrict__ Sxx = a.f.Sxx;
     float *__restrict__ Syy = a.f.Syy;
     float *__restrict__ Szz = a.f.Szz;
     float *__restrict__ Syz = a.f.Syz;
     float *__restrict__ Sxz = a.f.Sxz;
     float *__restrict__ Sxy = a.f.Sxy;
+    const float *__restrict__ SxxI = OOP ? a.fin.Sxx : Sxx;
+    const float *__restrict__ SyyI = OOP ? a.fin.Syy : Syy;
+    const float *__restrict__ SzzI = OOP ? a.fin.Szz : Szz;
+    const float *__restrict__ SyzI = OOP ? a.fin.Syz : Syz;
+    const float *__restrict__ SxzI = OOP ? a.fin.Sxz : Sxz;
+    const float *__restrict__ SxyI = OOP ? a.fin.Sxy : Sxy;
 
     // normal components
     const float dxVx = (vx_0 - vx_im1) * a.r20x;
@@ -185,9 +196,9 @@ __device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, 
     const float dxVx_ade = gxc.x * dxVx + gxc.y * axVx;
     const float dyVy_ade = gyc.x * dyVy + gyc.y * ayVy;
     const float dzVz_ade = gzc.x * dzVz + gzc.y * azVz;
-    st1<NT>(Sxx + p, ld1<NT>(Sxx + p) + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt);
-    st1<NT>(Syy + p, ld1<NT>(Syy + p) + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt);
-    st1<NT>(Szz + p, ld1<NT>(Szz + p) + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt);
+    st1<NT>(Sxx + p, ld1<NT>(SxxI + p) + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt);
+    st1<NT>(Syy + p, ld1<NT>(SyyI + p) + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt);
+    st1<NT>(Szz + p, ld1<NT>(SzzI + p) + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt);
     st1<NT>(A + A_axVx * n, gxc.z * axVx + gxc.w * dxVx * dt);
     st1<NT>(A + A_ayVy * n, gyc.z * ayVy + gyc.w * dyVy * dt);
     st1<NT>(A + A_azVz * n, gzc.z * azVz + gzc.w * dzVz * dt);
@@ -202,9 +213,9 @@ __device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, 
     const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
     const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
     const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
-    st1<NT>(Syz + p, ld1<NT>(Syz + p) + muyz * (gye.x * dyVz + gze.x * dzVy + gye.y * ayVz + gze.y * azVy) * dt);
-    st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + muxz * (gxe.x * dxVz + gze.x * dzVx + gxe.y * axVz + gze.y * azVx) * dt);
-    st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + muxy * (gxe.x * dxVy + gye.x * dyVx + gxe.y * axVy + gye.y * ayVx) * dt);
+    st1<NT>(Syz + p, ld1<NT>(SyzI + p) + muyz * (gye.x * dyVz + gze.x * dzVy + gye.y * ayVz + gze.y * azVy) * dt);
+    st1<NT>(Sxz + p, ld1<NT>(SxzI + p) + muxz * (gxe.x * dxVz + gze.x * dzVx + gxe.y * axVz + gze.y * azVx) * dt);
+    st1<NT>(Sxy + p, ld1<NT>(SxyI + p) + muxy * (gxe.x * dxVy + gye.x * dyVx + gxe.y * axVy + gye.y * ayVx) * dt);
     st1<NT>(A + A_ayVx * n, gye.z * ayVx + gye.w * dyVx * dt);
     st1<NT>(A + A_azVx * n, gze.z * azVx + gze.w * dzVx * dt);
     st1<NT>(A + A_axVy * n, gxe.z * axVy + gxe.w * dxVy * dt);
@@ -215,6 +226,7 @@ __device__ __forceinline__ void pml_stress_cell(const StepArgs &a, long long p, 
 
 // absorb_p__update_vel for one cell, m_absorb_p.f90:267-304.  Queue values:
 // Szz on k, k+1; Syz, Sxz on k-1, k; rho on k, k+1.
+template <bool OOP>
 __device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, long long q, int k, float szz_0,
                                              float szz_p1, float syz_m1, float syz_0, float sxz_m1, float sxz_0,
                                              float rho0, float rho_kp, float4 gxc, float4 gxe, float4 gyc,
@@ -229,6 +241,7 @@ __device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, lon
     const float *__restrict__ rho = a.m.rho;
     const long long n = a.npml;
     float *__restrict__ A = a.ade + q;
+    const float *__restrict__ AI = OOP ? a.ade_in + q : A;
     const float dt = a.dt;
     const float sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1];
     const float syy_0 = Syy[p], syy_jp1 = Syy[p + Y];
@@ -236,9 +249,9 @@ __device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, lon
     const float sxy_0 = Sxy[p], sxy_im1 = Sxy[p - 1], sxy_jm1 = Sxy[p - Y];
     const float rho_ip = rho[p + 1], rho_jp = rho[p + Y];
     const float4 gzc = a.gzc[k - 1], gze = a.gze[k - 1];
-    const float axSxx = ld1<true>(A + A_axSxx * n), aySxy = ld1<true>(A + A_aySxy * n), azSxz = ld1<true>(A + A_azSxz * n);
-    const float axSxy = ld1<true>(A + A_axSxy * n), aySyy = ld1<true>(A + A_aySyy * n), azSyz = ld1<true>(A + A_azSyz * n);
-    const float axSxz = ld1<true>(A + A_axSxz * n), aySyz = ld1<true>(A + A_aySyz * n), azSzz = ld1<true>(A + A_azSzz * n);
+    const float axSxx = ld1<true>(AI + A_axSxx * n), aySxy = ld1<true>(AI + A_aySxy * n), azSxz = ld1<true>(AI + A_azSxz * n);
+    const float axSxy = ld1<true>(AI + A_axSxy * n), aySyy = ld1<true>(AI + A_aySyy * n), azSyz = ld1<true>(AI + A_azSyz * n);
+    const float axSxz = ld1<true>(AI + A_axSxz * n), aySyz = ld1<true>(AI + A_aySyz * n), azSzz = ld1<true>(AI + A_azSzz * n);
 
     const float dxSxx = (sxx_ip1 - sxx_0) * a.r20x;
     const float dySyy = (syy_jp1 - syy_0) * a.r20y;
@@ -256,11 +269,11 @@ __device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, lon
     float *__restrict__ Vx = a.f.Vx;
     float *__restrict__ Vy = a.f.Vy;
     float *__restrict__ Vz = a.f.Vz;
-    st1<true>(Vx + p, ld1<true>(Vx + p) + bx * (gxe.x * dxSxx + gyc.x * dySxy + gzc.x * dzSxz
+    st1<true>(Vx + p, ld1<true>((OOP ? a.fin.Vx : Vx) + p) + bx * (gxe.x * dxSxx + gyc.x * dySxy + gzc.x * dzSxz
                                                 + gxe.y * axSxx + gyc.y * aySxy + gzc.y * azSxz) * dt);
-    st1<true>(Vy + p, ld1<true>(Vy + p) + by * (gxc.x * dxSxy + gye.x * dySyy + gzc.x * dzSyz
+    st1<true>(Vy + p, ld1<true>((OOP ? a.fin.Vy : Vy) + p) + by * (gxc.x * dxSxy + gye.x * dySyy + gzc.x * dzSyz
                                                 + gxc.y * axSxy + gye.y * aySyy + gzc.y * azSyz) * dt);
-    st1<true>(Vz + p, ld1<true>(Vz + p) + bz * (gxc.x * dxSxz + gyc.x * dySyz + gze.x * dzSzz
+    st1<true>(Vz + p, ld1<true>((OOP ? a.fin.Vz : Vz) + p) + bz * (gxc.x * dxSxz + gyc.x * dySyz + gze.x * dzSzz
                                                 + gxc.y * axSxz + gyc.y * aySyz + gze.y * azSzz) * dt);
     st1<true>(A + A_axSxx * n, gxe.z * axSxx + gxe.w * dxSxx * dt);
     st1<true>(A + A_aySxy * n, gyc.z * aySxy + gyc.w * dySxy * dt);
@@ -294,7 +307,7 @@ __device__ static inline void plain_tile(const StepArgs &a, int &bx, int &byy, i
     bz = t / a.nby;
 }
 
-template <int NM, bool NT>
+template <int NM, bool NT, bool OOP>
 __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
@@ -324,6 +337,12 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
     float *__restrict__ Syz = a.f.Syz;
     float *__restrict__ Sxz = a.f.Sxz;
     float *__restrict__ Sxy = a.f.Sxy;
+    const float *__restrict__ SxxI = OOP ? a.fin.Sxx : Sxx;
+    const float *__restrict__ SyyI = OOP ? a.fin.Syy : Syy;
+    const float *__restrict__ SzzI = OOP ? a.fin.Szz : Szz;
+    const float *__restrict__ SyzI = OOP ? a.fin.Syz : Syz;
+    const float *__restrict__ SxzI = OOP ? a.fin.Sxz : Sxz;
+    const float *__restrict__ SxyI = OOP ? a.fin.Sxy : Sxy;
     const float dt = a.dt;
     const float d2 = a.v.d2;
     const long long rn = g.rn;
@@ -391,9 +410,9 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
             }
             const float taup_plus1 = 1.0f + taup1 * (1.0f + d2);
             const float taus_plus1 = 1.0f + taus1 * (1.0f + d2);
-            st1<NT>(Sxx + p, ld1<NT>(Sxx + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
-            st1<NT>(Syy + p, ld1<NT>(Syy + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
-            st1<NT>(Szz + p, ld1<NT>(Szz + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
+            st1<NT>(Sxx + p, ld1<NT>(SxxI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
+            st1<NT>(Syy + p, ld1<NT>(SyyI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
+            st1<NT>(Szz + p, ld1<NT>(SzzI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
 
             // shear components, m_kernel.f90:285-333
             const float dxVy_dyVx = (vy_ip1 - vy_0) * re40x - (vy_ip2 - vy_im1) * re41x
@@ -422,9 +441,9 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
                 Rxy_n = Rxy_n + d1 * nxy;
             }
             R.template store<NT>(a.R, rn, r);
-            st1<NT>(Syz + p, ld1<NT>(Syz + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
-            st1<NT>(Sxz + p, ld1<NT>(Sxz + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
-            st1<NT>(Sxy + p, ld1<NT>(Sxy + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
+            st1<NT>(Syz + p, ld1<NT>(SyzI + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
+            st1<NT>(Sxz + p, ld1<NT>(SxzI + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
+            st1<NT>(Sxy + p, ld1<NT>(SxyI + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
 
             // rotate the z queues
             vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
@@ -444,7 +463,7 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
         for (int k = kb0; k <= kb1; ++k, p += P, q += astride) {
             const float vx_p2 = Vx[p + 2 * P], vy_p2 = Vy[p + 2 * P], vz_p1 = Vz[p + P];
             const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
-            pml_stress_cell<NT>(a, p, q, k, vx_0, vx_p1, vy_0, vy_p1, vz_m1, vz_0, mu00, mu00n, mu10, mu10n, mu01,
+            pml_stress_cell<NT, OOP>(a, p, q, k, vx_0, vx_p1, vy_0, vy_p1, vz_m1, vz_0, mu00, mu00n, mu10, mu10n, mu01,
                                 mu01n, mu11, gxc, gxe, gyc, gye);
             vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
             vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = vy_p2;
@@ -455,12 +474,12 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
     }
 }
 
-template <int NM, bool NT>
+template <int NM, bool NT, bool OOP>
 __global__ void __launch_bounds__(256) k_stress_int(StepArgs a)
 {
     int bx, byy, bz;
     plain_tile(a, bx, byy, bz);
-    stress_int_body<NM, NT>(a, bx, byy, bz);
+    stress_int_body<NM, NT, OOP>(a, bx, byy, bz);
 }
 
 // batched launch: tile = (member, bx, by, bz), member arguments in HBM
@@ -468,10 +487,11 @@ template <int NM, bool NT>
 __global__ void __launch_bounds__(256) k_stress_int_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    stress_int_body<NM, NT>(mem[t.x], t.y, t.z, t.w);
+    stress_int_body<NM, NT, false>(mem[t.x], t.y, t.z, t.w);
 }
 
 // kernel__update_vel, m_kernel.f90:75-140, then the bottom PML band
+template <bool OOP>
 __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
@@ -540,9 +560,9 @@ __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy,
                               + (syz_0 - syz_jm1) * re40y - (syz_jp1 - syz_jm2) * re41y
                               + (szz_p1 - szz_0) * re40z - (szz_p2 - szz_m1) * re41z;
 
-            st1<true>(Vx + p, ld1<true>(Vx + p) + 2.0f / (rho0 + rho[p + 1]) * d3Sx3 * dt);
-            st1<true>(Vy + p, ld1<true>(Vy + p) + 2.0f / (rho0 + rho[p + Y]) * d3Sy3 * dt);
-            st1<true>(Vz + p, ld1<true>(Vz + p) + 2.0f / (rho0 + rho_kp) * d3Sz3 * dt);
+            st1<true>(Vx + p, ld1<true>((OOP ? a.fin.Vx : Vx) + p) + 2.0f / (rho0 + rho[p + 1]) * d3Sx3 * dt);
+            st1<true>(Vy + p, ld1<true>((OOP ? a.fin.Vy : Vy) + p) + 2.0f / (rho0 + rho[p + Y]) * d3Sy3 * dt);
+            st1<true>(Vz + p, ld1<true>((OOP ? a.fin.Vz : Vz) + p) + 2.0f / (rho0 + rho_kp) * d3Sz3 * dt);
 
             sxz_m2 = sxz_m1; sxz_m1 = sxz_0; sxz_0 = sxz_p1;
             syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1;
@@ -559,7 +579,7 @@ __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy,
         for (int k = kb0; k <= kb1; ++k, p += P, q += astride) {
             const float sxz_p1 = Sxz[p + P], syz_p1 = Syz[p + P], szz_p2 = Szz[p + 2 * P];
             const float rho_kp = rho[p + P];
-            pml_vel_cell(a, p, q, k, szz_0, szz_p1, syz_m1, syz_0, sxz_m1, sxz_0, rho0, rho_kp, gxc, gxe, gyc, gye);
+            pml_vel_cell<OOP>(a, p, q, k, szz_0, szz_p1, syz_m1, syz_0, sxz_m1, sxz_0, rho0, rho_kp, gxc, gxe, gyc, gye);
             sxz_m2 = sxz_m1; sxz_m1 = sxz_0; sxz_0 = sxz_p1;
             syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1;
             szz_m1 = szz_0; szz_0 = szz_p1; szz_p1 = szz_p2;
@@ -570,17 +590,18 @@ __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy,
 
 // 4 waves per SIMD: the velocity update fits 128 VGPRs without spilling and
 // runs about 6 % faster than at the compiler's default of 3 waves
+template <bool OOP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vel_int(StepArgs a)
 {
     int bx, byy, bz;
     plain_tile(a, bx, byy, bz);
-    vel_int_body(a, bx, byy, bz);
+    vel_int_body<OOP>(a, bx, byy, bz);
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vel_int_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    vel_int_body(mem[t.x], t.y, t.z, t.w);
+    vel_int_body<false>(mem[t.x], t.y, t.z, t.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,6 +644,7 @@ __device__ static inline bool pml_lane(const StepArgs &a, const int4 tl, PmlLane
 }
 
 // absorb_p__update_stress, m_absorb_p.f90:448-519
+template <bool OOP>
 __device__ __forceinline__ void stress_pml_body(const StepArgs &a, const int4 tl)
 {
     PmlLane L;
@@ -645,7 +667,7 @@ __device__ __forceinline__ void stress_pml_body(const StepArgs &a, const int4 tl
     for (int k = L.kb; k <= L.ke; ++k, p += P, q += L.astride) {
         const float vx_p1 = Vx[p + P], vy_p1 = Vy[p + P], vz_0 = Vz[p];
         const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
-        pml_stress_cell<true>(a, p, q, k, vx_0, vx_p1, vy_0, vy_p1, vz_m1, vz_0, mu00, mu00n, mu10, mu10n, mu01,
+        pml_stress_cell<true, OOP>(a, p, q, k, vx_0, vx_p1, vy_0, vy_p1, vz_m1, vz_0, mu00, mu00n, mu10, mu10n, mu01,
                               mu01n, mu11, gxc, gxe, gyc, gye);
         vx_0 = vx_p1; vy_0 = vy_p1; vz_m1 = vz_0;
         mu00 = mu00n; mu10 = mu10n; mu01 = mu01n;
@@ -654,6 +676,7 @@ __device__ __forceinline__ void stress_pml_body(const StepArgs &a, const int4 tl
 }
 
 // absorb_p__update_vel, m_absorb_p.f90:261-306
+template <bool OOP>
 __device__ __forceinline__ void vel_pml_body(const StepArgs &a, const int4 tl)
 {
     PmlLane L;
@@ -673,19 +696,21 @@ __device__ __forceinline__ void vel_pml_body(const StepArgs &a, const int4 tl)
     float szz_0 = Szz[p], syz_m1 = Syz[p - P], sxz_m1 = Sxz[p - P], rho0 = rho[p];
     for (int k = L.kb; k <= L.ke; ++k, p += P, q += L.astride) {
         const float szz_p1 = Szz[p + P], syz_0 = Syz[p], sxz_0 = Sxz[p], rho_kp = rho[p + P];
-        pml_vel_cell(a, p, q, k, szz_0, szz_p1, syz_m1, syz_0, sxz_m1, sxz_0, rho0, rho_kp, gxc, gxe, gyc, gye);
+        pml_vel_cell<OOP>(a, p, q, k, szz_0, szz_p1, syz_m1, syz_0, sxz_m1, sxz_0, rho0, rho_kp, gxc, gxe, gyc, gye);
         szz_0 = szz_p1; syz_m1 = syz_0; sxz_m1 = sxz_0; rho0 = rho_kp;
     }
 }
 
+template <bool OOP>
 __global__ void __launch_bounds__(256) k_stress_pml(StepArgs a)
 {
-    stress_pml_body(a, a.tiles[xcd_remap(blockIdx.x, gridDim.x)]);
+    stress_pml_body<OOP>(a, a.tiles[xcd_remap(blockIdx.x, gridDim.x)]);
 }
 
+template <bool OOP>
 __global__ void __launch_bounds__(256) k_vel_pml(StepArgs a)
 {
-    vel_pml_body(a, a.tiles[xcd_remap(blockIdx.x, gridDim.x)]);
+    vel_pml_body<OOP>(a, a.tiles[xcd_remap(blockIdx.x, gridDim.x)]);
 }
 
 // batched side-column tiles: (i0 | width << 20 | xslab << 28, j0, k0 | k1 << 16, member)
@@ -694,13 +719,13 @@ __device__ static inline int4 unpack_bside(const int4 t) { return make_int4(t.x,
 __global__ void __launch_bounds__(256) k_stress_pml_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    stress_pml_body(mem[t.w], unpack_bside(t));
+    stress_pml_body<false>(mem[t.w], unpack_bside(t));
 }
 
 __global__ void __launch_bounds__(256) k_vel_pml_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    vel_pml_body(mem[t.w], unpack_bside(t));
+    vel_pml_body<false>(mem[t.w], unpack_bside(t));
 }
 
 void fill_step_args(swpc_ctx *c, StepArgs &a, int mask);
@@ -712,6 +737,12 @@ void fill_step_args(swpc_ctx *c, StepArgs &a, int mask)
     a.f.Vx = c->fields[SWPC_VX]; a.f.Vy = c->fields[SWPC_VY]; a.f.Vz = c->fields[SWPC_VZ];
     a.f.Sxx = c->fields[SWPC_SXX]; a.f.Syy = c->fields[SWPC_SYY]; a.f.Szz = c->fields[SWPC_SZZ];
     a.f.Syz = c->fields[SWPC_SYZ]; a.f.Sxz = c->fields[SWPC_SXZ]; a.f.Sxy = c->fields[SWPC_SXY];
+    {
+        float *const *in = c->fields_in[0] ? c->fields_in : c->fields;
+        a.fin.Vx = in[SWPC_VX]; a.fin.Vy = in[SWPC_VY]; a.fin.Vz = in[SWPC_VZ];
+        a.fin.Sxx = in[SWPC_SXX]; a.fin.Syy = in[SWPC_SYY]; a.fin.Szz = in[SWPC_SZZ];
+        a.fin.Syz = in[SWPC_SYZ]; a.fin.Sxz = in[SWPC_SXZ]; a.fin.Sxy = in[SWPC_SXY];
+    }
     a.m.rho = c->rho; a.m.lam = c->lam; a.m.mu = c->mu; a.m.taup = c->taup; a.m.taus = c->taus;
     a.m.kfs_top = c->kfs_top; a.m.kfs_bot = c->kfs_bot; a.m.kob_top = c->kob_top; a.m.kob_bot = c->kob_bot;
     a.m.kob = c->kob;
@@ -720,6 +751,7 @@ void fill_step_args(swpc_ctx *c, StepArgs &a, int mask)
     a.c = c->coef;
     a.dt = c->desc.dt;
     a.ade = c->ade;
+    a.ade_in = c->ade_in ? c->ade_in : c->ade;
     a.npml = c->npml;
     for (int q = 0; q < 5; ++q) a.box[q] = c->box[q];
     a.gxc = c->gxc; a.gxe = c->gxe; a.gyc = c->gyc; a.gye = c->gye; a.gzc = c->gzc; a.gze = c->gze;
@@ -749,7 +781,8 @@ hipError_t launch_stress_side(swpc_ctx *c, hipStream_t s)
     if (c->desc.abc != SWPC_ABC_PML || c->npml == 0 || c->npml_tiles <= 0) return hipSuccess;
     StepArgs a;
     step_args(c, a, MASK_PML);
-    hipLaunchKernelGGL(k_stress_pml, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
+    if (c->fields_in[0]) hipLaunchKernelGGL(k_stress_pml<true>, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
+    else hipLaunchKernelGGL(k_stress_pml<false>, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
     return hipGetLastError();
 }
 
@@ -758,7 +791,8 @@ hipError_t launch_vel_side(swpc_ctx *c, hipStream_t s)
     if (c->desc.abc != SWPC_ABC_PML || c->npml == 0 || c->npml_tiles <= 0) return hipSuccess;
     StepArgs a;
     step_args(c, a, MASK_PML);
-    hipLaunchKernelGGL(k_vel_pml, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
+    if (c->fields_in[0]) hipLaunchKernelGGL(k_vel_pml<true>, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
+    else hipLaunchKernelGGL(k_vel_pml<false>, dim3(c->npml_tiles), dim3(BX, a.by), 0, s ? s : c->stream, a);
     return hipGetLastError();
 }
 
@@ -770,10 +804,12 @@ hipError_t launch_stress_cols(swpc_ctx *c, int mask)
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
     dim3 block(a.bxw, a.byi), grid(a.nbx * a.nby * a.nbz);
     const bool nt = (c->variant & 1) != 0;
+    const bool oop = c->fields_in[0] != nullptr;   // out of place: non-temporal streams as well
 #define SWPC_STRESS_CASE(NMV)                                                                       \
     case NMV:                                                                                       \
-        if (nt) hipLaunchKernelGGL((k_stress_int<NMV, true>), grid, block, 0, c->stream, a);      \
-        else hipLaunchKernelGGL((k_stress_int<NMV, false>), grid, block, 0, c->stream, a);        \
+        if (oop) hipLaunchKernelGGL((k_stress_int<NMV, true, true>), grid, block, 0, c->stream, a); \
+        else if (nt) hipLaunchKernelGGL((k_stress_int<NMV, true, false>), grid, block, 0, c->stream, a); \
+        else hipLaunchKernelGGL((k_stress_int<NMV, false, false>), grid, block, 0, c->stream, a); \
         break;
     switch (c->g.nm) {
         SWPC_STRESS_CASE(0) SWPC_STRESS_CASE(1) SWPC_STRESS_CASE(2) SWPC_STRESS_CASE(3) SWPC_STRESS_CASE(4)
@@ -790,7 +826,8 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
     step_args(c, a, mask);
     a.zc = c->d_zchunks + c->nbz + 1;
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
-    hipLaunchKernelGGL(k_vel_int, dim3(a.nbx * a.nby * a.nbz), dim3(a.bxw, a.byi), 0, c->stream, a);
+    if (c->fields_in[0]) hipLaunchKernelGGL(k_vel_int<true>, dim3(a.nbx * a.nby * a.nbz), dim3(a.bxw, a.byi), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_vel_int<false>, dim3(a.nbx * a.nby * a.nbz), dim3(a.bxw, a.byi), 0, c->stream, a);
     return hipGetLastError();
 }
 

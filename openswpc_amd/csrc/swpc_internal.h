@@ -117,6 +117,11 @@ struct swpc_ctx {
     char err[512];
 
     float *fields[SWPC_NFIELD];
+    // when set, the step reads the old values of the fields it updates
+    // (and of the PML auxiliaries) here and writes the new ones to fields
+    // (ade): the adjoint records steps next to their inputs this way
+    float *fields_in[SWPC_NFIELD];
+    const float *ade_in;
     float *rho, *lam, *mu, *taup, *taus;
     int *kfs_top, *kfs_bot, *kob_top, *kob_bot, *kob;
     float *R;          // 6*nm*rn

@@ -2,7 +2,8 @@
 swpc_adj_accumulate / swpc_adj_get_gradient) with two-level checkpointing,
 three Zener mechanisms:
 
-  * two-level schemes give the bits of the single-level one;
+  * two-level schemes, and step records taken out of place, give the bits
+    of the single-level scheme with copied records;
   * the device sum over two sources, each with its own source and receivers,
     matches the sum of the FP64 oracle's two gradients, and is bit for bit
     the float32 sum of the two single-source GPU gradients (each source's
@@ -63,8 +64,11 @@ def _switch(g, m):
 
 def test_two_level_schemes_are_bit_identical(two):
     ma, _, (oa, _, _), _ = two
+    # reference: single level, step records by copies (variant bit 4)
+    h = gpu_from_model(ma, variant=1 | 16)
+    J0, *g0 = h.adjoint_gradient(NT, oa, ckp=7)
+    h.close()
     g = gpu_from_model(ma)
-    J0, *g0 = g.adjoint_gradient(NT, oa, ckp=7)
     for c1, c2 in ((20, 6), (50, 7), (13, 13), (16, 1), (1, 0)):
         g.adjoint_reset()
         J = g.adjoint_accumulate(NT, oa, c1, c2)
