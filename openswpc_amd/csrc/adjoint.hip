@@ -136,16 +136,19 @@ __device__ static inline float wt(const float t[4], int c) { return c == 1 ? t[1
 // points p-2s .. p+2s are read once per axis and shared by its gathers.
 struct AxisW {
     float A[5], B[5];
+    unsigned pml;   // bit o: point p + (o - 2) s is a PML cell
 };
 __device__ static inline AxisW axis_w(const unsigned char *cls, long long p, long long s, const float tA[4],
                                       const float tB[4])
 {
     AxisW w;
+    w.pml = 0;
 #pragma unroll
     for (int o = 0; o < 5; ++o) {
         const int c = cls[p + (o - 2) * s];
         w.A[o] = wt(tA, c);
         w.B[o] = wt(tB, c);
+        w.pml |= (c == 3 ? 1u : 0u) << o;
     }
     return w;
 }
@@ -158,6 +161,29 @@ __device__ static inline float gat_fwd(const float *D, long long p, long long s,
 __device__ static inline float gat_bwd(const float *D, long long p, long long s, const AxisW &w)
 {
     return w.A[2] * D[p] - w.A[3] * D[p + s] - w.B[1] * D[p - s] + w.B[4] * D[p + 2 * s];
+}
+
+// The same gathers for a density that interior cells share with another
+// one (the velocity update's three derivatives of one component all carry
+// that component's weight; the stress update's two derivatives of one shear
+// component carry its weight): the density kernels store Dp for PML cells
+// only, and the gather reads point o from Dp where it is a PML cell and from
+// Di, which holds the same value, elsewhere.  Points outside the cells hold 0
+// in both.
+__device__ static inline float ld_sel(const float *Dp, const float *Di, long long p, long long s, int o,
+                                      const AxisW &w)
+{
+    return ((w.pml >> o) & 1u ? Dp : Di)[p + (o - 2) * s];
+}
+__device__ static inline float gat_fwd_s(const float *Dp, const float *Di, long long p, long long s, const AxisW &w)
+{
+    return w.A[1] * ld_sel(Dp, Di, p, s, 1, w) - w.A[2] * ld_sel(Dp, Di, p, s, 2, w)
+         - w.B[0] * ld_sel(Dp, Di, p, s, 0, w) + w.B[3] * ld_sel(Dp, Di, p, s, 3, w);
+}
+__device__ static inline float gat_bwd_s(const float *Dp, const float *Di, long long p, long long s, const AxisW &w)
+{
+    return w.A[2] * ld_sel(Dp, Di, p, s, 2, w) - w.A[3] * ld_sel(Dp, Di, p, s, 3, w)
+         - w.B[1] * ld_sel(Dp, Di, p, s, 1, w) + w.B[4] * ld_sel(Dp, Di, p, s, 4, w);
 }
 
 #define AT(F, di, dj, dk) (F)[gidx(g, i + (di), j + (dj), k + (dk))]
@@ -240,8 +266,13 @@ __global__ void __launch_bounds__(256) k_adj_vel_dens(AdjArgs a)
             for (int c = 0; c < 9; ++c) a.aA[(long long)(A_axSxx + c) * n + q] = gp[c].y * w[c] + gp[c].z * ahat[c];
             fx = f[0]; fy = f[1]; fz = f[2];
         }
+        // interior cells: d[0..2], d[3..5], d[6..8] are one value each, kept in D0, D3, D6
+        if (kind == 1) {
+            a.D[0][p] = d[0]; a.D[3][p] = d[3]; a.D[6][p] = d[6];
+        } else {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) a.D[c][p] = d[c];
+            for (int c = 0; c < 9; ++c) a.D[c][p] = d[c];
+        }
         a.q[0][p] = -2.0f / (sx * sx) * a.dt * fx * vx;
         a.q[1][p] = -2.0f / (sy * sy) * a.dt * fy * vy;
         a.q[2][p] = -2.0f / (sz * sz) * a.dt * fz * vz;
@@ -259,11 +290,11 @@ __global__ void __launch_bounds__(256) k_adj_vel_gather(AdjArgs a)
         const AxisW wy = axis_w(a.cls, p, sy, a.tA[1], a.tB[1]);
         const AxisW wz = axis_w(a.cls, p, sz, a.tA[2], a.tB[2]);
         const float dxx = gat_fwd(a.D[0], p, sx, wx);
-        const float dyy = gat_fwd(a.D[4], p, sy, wy);
-        const float dzz = gat_fwd(a.D[8], p, sz, wz);
-        const float dxy = gat_bwd(a.D[1], p, sy, wy) + gat_bwd(a.D[3], p, sx, wx);
-        const float dxz = gat_bwd(a.D[2], p, sz, wz) + gat_bwd(a.D[6], p, sx, wx);
-        const float dyz = gat_bwd(a.D[5], p, sz, wz) + gat_bwd(a.D[7], p, sy, wy);
+        const float dyy = gat_fwd_s(a.D[4], a.D[3], p, sy, wy);
+        const float dzz = gat_fwd_s(a.D[8], a.D[6], p, sz, wz);
+        const float dxy = gat_bwd_s(a.D[1], a.D[0], p, sy, wy) + gat_bwd(a.D[3], p, sx, wx);
+        const float dxz = gat_bwd_s(a.D[2], a.D[0], p, sz, wz) + gat_bwd(a.D[6], p, sx, wx);
+        const float dyz = gat_bwd_s(a.D[5], a.D[3], p, sz, wz) + gat_bwd_s(a.D[7], a.D[6], p, sy, wy);
         // each cell's density enters the buoyancy of its own point and of the
         // point one cell back along each axis (q is 0 outside the cells)
         float gr = a.q[0][p] + a.q[1][p] + a.q[2][p];
@@ -433,8 +464,14 @@ __global__ void __launch_bounds__(256) k_adj_stress_dens(AdjArgs a)
         hmean4_w(mu, m001, m010, m011, gyz);
         hmean4_w(mu, m001, m100, m101, gxz);
         hmean4_w(mu, m100, m010, m110, gxy);
+        // interior cells: d[3] = d[4], d[5] = d[6], d[7] = d[8], kept in D3, D5, D7
+        if (kind == 1) {
+            a.D[0][p] = d[0]; a.D[1][p] = d[1]; a.D[2][p] = d[2];
+            a.D[3][p] = d[3]; a.D[5][p] = d[5]; a.D[7][p] = d[7];
+        } else {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) a.D[c][p] = d[c];
+            for (int c = 0; c < 9; ++c) a.D[c][p] = d[c];
+        }
         a.g_lam[p] = glo + dgl;
         a.g_mu[p] = (gmo + dgm) + (qyz * gyz[0] + qxz * gxz[0] + qxy * gxy[0]);
         a.q[0][p] = qyz * gyz[1] + qxz * gxz[1];
@@ -456,8 +493,10 @@ __global__ void __launch_bounds__(256) k_adj_stress_gather(AdjArgs a)
         const AxisW wx = axis_w(a.cls, p, sx, a.tA[0], a.tB[0]);
         const AxisW wy = axis_w(a.cls, p, sy, a.tA[1], a.tB[1]);
         const AxisW wz = axis_w(a.cls, p, sz, a.tA[2], a.tB[2]);
-        const float vx = gat_bwd(a.D[0], p, sx, wx) + gat_fwd(a.D[8], p, sy, wy) + gat_fwd(a.D[6], p, sz, wz);
-        const float vy = gat_bwd(a.D[1], p, sy, wy) + gat_fwd(a.D[7], p, sx, wx) + gat_fwd(a.D[4], p, sz, wz);
+        const float vx = gat_bwd(a.D[0], p, sx, wx) + gat_fwd_s(a.D[8], a.D[7], p, sy, wy) +
+                         gat_fwd_s(a.D[6], a.D[5], p, sz, wz);
+        const float vy = gat_bwd(a.D[1], p, sy, wy) + gat_fwd(a.D[7], p, sx, wx) +
+                         gat_fwd_s(a.D[4], a.D[3], p, sz, wz);
         const float vz = gat_bwd(a.D[2], p, sz, wz) + gat_fwd(a.D[5], p, sx, wx) + gat_fwd(a.D[3], p, sy, wy);
         const float gm = a.q[0][p - sz] + a.q[1][p - sy] + a.q[2][p - sx] + a.q[3][p - sy - sz] + a.q[4][p - sx - sz] +
                          a.q[5][p - sx - sy];
