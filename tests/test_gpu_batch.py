@@ -74,3 +74,41 @@ def test_tuner_dry_trials_leave_state_untouched(orc):
     b.synchronize()
     _check_members(gs, runs)
     b.close()
+
+
+def test_full_greens_batch_matches_members_stepped_alone():
+    """The benchmark's 64-member Green's-function batch at full size
+    (302.6 M cells, grids 96..256 x 96..256 x 96..192, NM = 2, 10-cell PML,
+    body forces, 8 receivers each), autotuned, as graph replays: every
+    member's fields and traces equal, bit for bit, those of the same member
+    stepped alone (the single-context path the tests above pin to the
+    oracle).  20 steps carry each force's field into its free surface and,
+    on the narrow members, into the side PML."""
+    import bench
+    from openswpc_amd.host import build_inputs, make_simulation
+
+    nt = 20
+    specs = bench.batch_specs(nt=nt + 1)
+    batched = [make_simulation(build_inputs(sp)) for sp in specs]
+    alone = [make_simulation(build_inputs(sp)) for sp in specs]
+    b = SwpcBatch(batched)
+    try:
+        kc, by, width, ms = b.tune(trial_steps=1)
+        print("tuned", kc, by, width, ms, flush=True)
+        b.step(1, nt, use_graph=True, block=5)
+        b.synchronize()
+        for s in alone:
+            s.step(1, nt, use_graph=True, block=5)
+        for n, (x, y) in enumerate(zip(batched, alone)):
+            y.synchronize()
+            for name in O.FIELDS_V + O.FIELDS_S:
+                fx, fy = x.get_field(name), y.get_field(name)
+                assert ulp_diff(fx, fy) == 0, (n, name)
+                assert np.array_equal(np.signbit(fx), np.signbit(fy)), (n, name)
+            assert np.any(x.get_field("Vz") != 0), n
+            tx, ty = x.seismograms("vel"), y.seismograms("vel")
+            assert tx.shape == ty.shape and np.array_equal(tx, ty), n
+    finally:
+        b.close()
+        for s in batched + alone:
+            s.close()
