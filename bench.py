@@ -585,8 +585,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the single-GPU path is the whole "
-                                   "domain on one device; multi-GPU sharding is not built (see DESIGN.md)",
+        print(json.dumps({"error": "this library runs on one GPU only: the whole domain lives on one "
+                                   "device; sharding across GPUs (x/y slabs with halo exchange) is not built",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
