@@ -1,7 +1,7 @@
 """The GPU discrete adjoint (swpc_adj_gradient) against the FP64 oracle
 adjoint (itself pinned by the dot-product test and finite differences):
 misfit and lam/mu/rho gradients on the same model and data, elastic and with
-three Zener mechanisms, and checkpoint spacing that leaves the result
+one, three and eight Zener mechanisms, and checkpoint spacing that leaves the result
 bit-identical."""
 import numpy as np
 import pytest
@@ -15,10 +15,10 @@ pytestmark = pytest.mark.gpu
 OFF = O.OFF
 
 
-@pytest.fixture(scope="module", params=[0, 3], ids=["elastic", "nm3"])
+@pytest.fixture(scope="module", params=[0, 3, 1, 8], ids=["elastic", "nm3", "nm1", "nm8"])
 def case(orc, request):
     nt = 50
-    cfg = _cfg(nt=nt, wav_flags=1) if request.param == 0 else _visco(nt=nt, wav_flags=1)
+    cfg = _cfg(nt=nt, wav_flags=1) if request.param == 0 else _visco(nt=nt, wav_flags=1, nm=request.param)
     m = O.build_model(cfg)
     rng = np.random.default_rng(5)
     adj = O.OracleAdjoint(m, 64)

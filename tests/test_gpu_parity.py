@@ -37,7 +37,7 @@ def _run_both(cfg, nsteps, graph=False):
     return m, run, g
 
 
-@pytest.mark.parametrize("nm", [3, 2, 0])
+@pytest.mark.parametrize("nm", [3, 2, 0, 1, 5, 8])
 def test_pml_moment_tensor_steps(orc, nm):
     cfg = small_config(nm=nm)
     m, run, g = _run_both(cfg, 40)

@@ -40,9 +40,9 @@ def _visco(**kw):
     return _cfg(**base)
 
 
-@pytest.mark.parametrize("nm", [0, 3])
+@pytest.mark.parametrize("nm", [0, 3, 1, 8])
 def test_dot_product_one_step(orc, nm):
-    cfg = _cfg(sources=[]) if nm == 0 else _visco(sources=[])
+    cfg = _cfg(sources=[]) if nm == 0 else _visco(sources=[], nm=nm)
     m = O.build_model(cfg)
     adj = O.OracleAdjoint(m, 64)
     run = adj.run
