@@ -579,6 +579,9 @@ AdjSizes sizes_of(const swpc_ctx *c)
 // forward pass; each outer segment is recomputed once to place inner
 // checkpoints every c2 steps, and each inner segment once more into the
 // per-step records the adjoint reads.  c2 == c1 is the single-level scheme.
+// The first pass itself places the last outer segment's inner checkpoints and
+// records its last inner segment, so that segment is not run again but for
+// its other inner segments' records.
 long long scheme_fwd_steps(int nt, int c1, int c2)
 {
     long long f = nt;  // the recording pass
@@ -587,7 +590,7 @@ long long scheme_fwd_steps(int nt, int c1, int c2)
         f += L;
         if (c2 < L) f += (long long)((L - 1) / c2) * c2;
     }
-    return f;
+    return f - (nt - (long long)((nt - 1) / c1) * c1);   // the last outer segment's length
 }
 
 long long scheme_ck_slots(int nt, int c1, int c2)
@@ -825,6 +828,79 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
         return e;
     };
 
+    // Records of steps n0 .. n1-1 from the current state.  Record n holds
+    // V_n (3 fields) | S_{n+1} (6 fields) | aV_n, aS_n (18 x npml, laid out as
+    // the auxiliaries).  The steps run out of place: step n reads V_n, S_n and
+    // the auxiliaries from the records and writes S_{n+1} into record n and
+    // V_{n+1} and the new auxiliaries into record n + 1 (the last step into
+    // the context's own arrays), so no field is copied but those of step n0.
+    // Variant bit 4 keeps the copying form (the same bits; for checks).
+    // sample: the receivers sample every step first (the first forward pass).
+    auto record = [&](int n0, int n1, bool sample) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (c->variant & 16) {
+            for (int n = n0; n < n1 && e == hipSuccess; ++n) {
+                float *st = w.seg + (size_t)(n - n0) * sstep;
+                const int it = n + 1;
+                if (sample) e = launch_wav_store(c, it, nullptr);
+                for (int q = 0; q < 3 && e == hipSuccess; ++q)
+                    e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(st + 9 * nf, c->ade, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = stress_phase(it);
+                for (int q = 3; q < 9 && e == hipSuccess; ++q)
+                    e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(st + 9 * nf + na9, c->ade + na9, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = vel_phase(it);
+            }
+            return e;
+        }
+        auto V = [&](int n) { return w.seg + (size_t)(n - n0) * sstep; };
+        auto S = [&](int n) { return V(n) + 3 * nf; };
+        auto A = [&](int n) { return V(n) + 9 * nf; };
+        float *real[SWPC_NFIELD];
+        for (int q = 0; q < SWPC_NFIELD; ++q) real[q] = c->fields[q];
+        float *const real_ade = c->ade;
+        for (int q = 0; q < 3 && e == hipSuccess; ++q)
+            e = hipMemcpyAsync(V(n0) + q * nf, real[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(A(n0), real_ade, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        for (int n = n0; n < n1 && e == hipSuccess; ++n) {
+            const int it = n + 1;
+            const bool last = n == n1 - 1;
+            float *aux_out = last ? real_ade : A(n + 1);
+            if (sample) {
+                // the receivers read V_n and S_n where this step finds them
+                for (int q = 0; q < 3; ++q) c->fields[q] = V(n) + q * nf;
+                for (int q = 3; q < 9; ++q) c->fields[q] = n == n0 ? real[q] : S(n - 1) + (q - 3) * nf;
+                e = launch_wav_store(c, it, nullptr);
+                if (e != hipSuccess) break;
+            }
+            // stress phase: V_n read, S_n -> S_{n+1} in record n, aV_n -> aV_{n+1}
+            for (int q = 0; q < 3; ++q) c->fields_in[q] = c->fields[q] = V(n) + q * nf;
+            for (int q = 3; q < 9; ++q) {
+                c->fields[q] = S(n) + (q - 3) * nf;
+                c->fields_in[q] = n == n0 ? real[q] : S(n - 1) + (q - 3) * nf;
+            }
+            c->ade = aux_out;
+            c->ade_in = A(n);
+            e = stress_phase(it);
+            // velocity phase: S_{n+1} read, V_n -> V_{n+1}, aS_n -> aS_{n+1}
+            for (int q = 0; q < 3; ++q) {
+                c->fields_in[q] = V(n) + q * nf;
+                c->fields[q] = last ? real[q] : V(n + 1) + q * nf;
+            }
+            for (int q = 3; q < 9; ++q) c->fields_in[q] = c->fields[q] = S(n) + (q - 3) * nf;
+            if (e == hipSuccess) e = vel_phase(it);
+        }
+        for (int q = 0; q < SWPC_NFIELD; ++q) {
+            c->fields[q] = real[q];
+            c->fields_in[q] = nullptr;
+        }
+        c->ade = real_ade;
+        c->ade_in = nullptr;
+        return e;
+    };
     // ---- forward pass from rest, outer checkpoints every c1 steps ----
     hipError_t e = hipSuccess;
     for (int q = 0; q < SWPC_NFIELD && e == hipSuccess; ++q) e = zero(c->fields[q], nf, s);
@@ -832,13 +908,22 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     if (e == hipSuccess && nR) e = zero(c->R, nR, s);
     if (e == hipSuccess && c->d_uacc) e = zero(c->d_uacc, 3LL * c->nst, s);
     if (e == hipSuccess && c->d_eacc) e = zero(c->d_eacc, 6LL * c->nst, s);
-    for (int n = 0; n < nt && e == hipSuccess; ++n) {
+    // The backward walk starts with the last outer segment: this pass also
+    // lays its inner checkpoints and records its last inner segment, which
+    // the walk would otherwise recompute (scheme_fwd_steps counts it so).
+    const int N0L = (int)(nout - 1) * c1;
+    const int ninL = c2 < nt - N0L ? (nt - N0L - 1) / c2 : 0;
+    const int n0L = N0L + ninL * c2;
+    for (int n = 0; n < n0L && e == hipSuccess; ++n) {
         const int it = n + 1;
         if (n % c1 == 0) e = save_state(outer_ck + (size_t)(n / c1) * per_state);
+        if (e == hipSuccess && n > N0L && (n - N0L) % c2 == 0)
+            e = save_state(inner_ck + (size_t)((n - N0L) / c2 - 1) * per_state);
         if (e == hipSuccess) e = launch_wav_store(c, it, nullptr);
         if (e == hipSuccess) e = stress_phase(it);
         if (e == hipSuccess) e = vel_phase(it);
     }
+    if (e == hipSuccess) e = record(n0L, nt, true);
     if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
 
     // ---- misfit on the host (double) ----
@@ -951,72 +1036,6 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
             return SWPC_EHIP;
     }
     const float sc = 0.5f * c->wav_M0 * c->wav_UC * 1e9f;
-    // records of steps n0 .. n1-1 from the current state; per step:
-    // V_n (3 fields) | S_{n+1} (6 fields) | aV_n (9 x npml) | aS_n (9 x npml)
-    // Records of steps n0 .. n1-1 from the current state.  Record n holds
-    // V_n (3 fields) | S_{n+1} (6 fields) | aV_n, aS_n (18 x npml, laid out as
-    // the auxiliaries).  The steps run out of place: step n reads V_n, S_n and
-    // the auxiliaries from the records and writes S_{n+1} into record n and
-    // V_{n+1} and the new auxiliaries into record n + 1 (the last step into
-    // the context's own arrays), so no field is copied but those of step n0.
-    // Variant bit 4 keeps the copying form (the same bits; for checks).
-    auto record = [&](int n0, int n1) -> hipError_t {
-        hipError_t e = hipSuccess;
-        if (c->variant & 16) {
-            for (int n = n0; n < n1 && e == hipSuccess; ++n) {
-                float *st = w.seg + (size_t)(n - n0) * sstep;
-                const int it = n + 1;
-                for (int q = 0; q < 3 && e == hipSuccess; ++q)
-                    e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(st + 9 * nf, c->ade, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
-                if (e == hipSuccess) e = stress_phase(it);
-                for (int q = 3; q < 9 && e == hipSuccess; ++q)
-                    e = hipMemcpyAsync(st + q * nf, c->fields[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(st + 9 * nf + na9, c->ade + na9, na9 * sizeof(float), hipMemcpyDeviceToDevice, s);
-                if (e == hipSuccess) e = vel_phase(it);
-            }
-            return e;
-        }
-        auto V = [&](int n) { return w.seg + (size_t)(n - n0) * sstep; };
-        auto S = [&](int n) { return V(n) + 3 * nf; };
-        auto A = [&](int n) { return V(n) + 9 * nf; };
-        float *real[SWPC_NFIELD];
-        for (int q = 0; q < SWPC_NFIELD; ++q) real[q] = c->fields[q];
-        float *const real_ade = c->ade;
-        for (int q = 0; q < 3 && e == hipSuccess; ++q)
-            e = hipMemcpyAsync(V(n0) + q * nf, real[q], nf * sizeof(float), hipMemcpyDeviceToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(A(n0), real_ade, na18 * sizeof(float), hipMemcpyDeviceToDevice, s);
-        for (int n = n0; n < n1 && e == hipSuccess; ++n) {
-            const int it = n + 1;
-            const bool last = n == n1 - 1;
-            float *aux_out = last ? real_ade : A(n + 1);
-            // stress phase: V_n read, S_n -> S_{n+1} in record n, aV_n -> aV_{n+1}
-            for (int q = 0; q < 3; ++q) c->fields_in[q] = c->fields[q] = V(n) + q * nf;
-            for (int q = 3; q < 9; ++q) {
-                c->fields[q] = S(n) + (q - 3) * nf;
-                c->fields_in[q] = n == n0 ? real[q] : S(n - 1) + (q - 3) * nf;
-            }
-            c->ade = aux_out;
-            c->ade_in = A(n);
-            e = stress_phase(it);
-            // velocity phase: S_{n+1} read, V_n -> V_{n+1}, aS_n -> aS_{n+1}
-            for (int q = 0; q < 3; ++q) {
-                c->fields_in[q] = V(n) + q * nf;
-                c->fields[q] = last ? real[q] : V(n + 1) + q * nf;
-            }
-            for (int q = 3; q < 9; ++q) c->fields_in[q] = c->fields[q] = S(n) + (q - 3) * nf;
-            if (e == hipSuccess) e = vel_phase(it);
-        }
-        for (int q = 0; q < SWPC_NFIELD; ++q) {
-            c->fields[q] = real[q];
-            c->fields_in[q] = nullptr;
-        }
-        c->ade = real_ade;
-        c->ade_in = nullptr;
-        return e;
-    };
     // the adjoint of steps n1-1 down to n0 from their records
     auto backward = [&](int n0, int n1) -> hipError_t {
         hipError_t e = hipSuccess;
@@ -1046,17 +1065,20 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     for (int sg = (int)nout - 1; sg >= 0 && e == hipSuccess; --sg) {
         const int N0 = sg * c1, N1 = (N0 + c1 < nt) ? N0 + c1 : nt;
         const int nin = c2 < N1 - N0 ? (N1 - N0 - 1) / c2 : 0;  // inner segments after the first
-        e = load_state(outer_ck + (size_t)sg * per_state);
-        // inner checkpoints: the fields end at the start of the last inner segment
-        for (int n = N0; n < N0 + nin * c2 && e == hipSuccess; ++n) {
-            if (n > N0 && (n - N0) % c2 == 0) e = save_state(inner_ck + (size_t)((n - N0) / c2 - 1) * per_state);
-            if (e == hipSuccess) e = stress_phase(n + 1);
-            if (e == hipSuccess) e = vel_phase(n + 1);
+        const bool laid = sg == (int)nout - 1;   // by the forward pass: inner checkpoints, last records
+        if (!laid) {
+            e = load_state(outer_ck + (size_t)sg * per_state);
+            // inner checkpoints: the fields end at the start of the last inner segment
+            for (int n = N0; n < N0 + nin * c2 && e == hipSuccess; ++n) {
+                if (n > N0 && (n - N0) % c2 == 0) e = save_state(inner_ck + (size_t)((n - N0) / c2 - 1) * per_state);
+                if (e == hipSuccess) e = stress_phase(n + 1);
+                if (e == hipSuccess) e = vel_phase(n + 1);
+            }
         }
         for (int is = nin; is >= 0 && e == hipSuccess; --is) {
             const int n0 = N0 + is * c2, n1 = (n0 + c2 < N1) ? n0 + c2 : N1;
             if (is < nin) e = load_state(is == 0 ? outer_ck + (size_t)sg * per_state : inner_ck + (size_t)(is - 1) * per_state);
-            if (e == hipSuccess) e = record(n0, n1);
+            if (e == hipSuccess && !(laid && is == nin)) e = record(n0, n1, false);
             if (e == hipSuccess) e = backward(n0, n1);
         }
     }

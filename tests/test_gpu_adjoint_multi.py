@@ -128,11 +128,12 @@ def test_planner_fits_its_budget_and_matches_the_allocations(two):
     g = gpu_from_model(ma)
     big = g.adjoint_plan(NT, 1 << 40)
     c1, c2, nbytes, nfwd = big
-    assert c1 == c2 and nfwd == 2 * NT
+    # room for every step's record: the one forward pass records them all
+    assert c1 == c2 == NT and nfwd == NT
     prev = nfwd
-    budgets = [nbytes - 1, nbytes * 3 // 4, nbytes // 2]
     plans = []
-    for b in budgets:
+    b = nbytes - 1
+    for _ in range(64):
         try:
             p = g.adjoint_plan(NT, b)
         except SwpcError:
@@ -141,7 +142,10 @@ def test_planner_fits_its_budget_and_matches_the_allocations(two):
         assert p[3] >= prev
         prev = p[3]
         plans.append(p)
-    assert plans and plans[0][1] < plans[0][0] and plans[0][3] > 2 * NT
+        b = p[2] * 9 // 10
+    print(plans, flush=True)
+    assert plans and plans[0][3] > NT
+    assert any(p[1] < p[0] for p in plans), "no two-level scheme before the budget ran out"
     with pytest.raises(SwpcError):
         g.adjoint_plan(NT, 1 << 20)
     # what the library allocates for a scheme is what the planner counted
