@@ -334,7 +334,8 @@ int swpc_synchronize(swpc_ctx *ctx);
 int64_t swpc_device_bytes(const swpc_ctx *ctx);
 
 /* Launch-shape tuning knobs.  kchunk: k-planes marched per thread block
-   (0 = keep); by: rows per 64-wide tile, 1, 2 or 4 (0 = keep); variant:
+   (0 = keep; default 16); by: rows per 64-wide tile, 1, 2 or 4 (0 = keep;
+   default 4); variant:
    kernel variant flags (-1 = keep; default 1): bit 0 = non-temporal loads and
    stores for the single-use streams of the stress update, bit 1 = side-column
    PML kernels on a second stream alongside the interior ones, bit 2 =
