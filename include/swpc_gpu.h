@@ -334,9 +334,9 @@ int swpc_synchronize(swpc_ctx *ctx);
 int64_t swpc_device_bytes(const swpc_ctx *ctx);
 
 /* Launch-shape tuning knobs.  kchunk: k-planes marched per thread block
-   (0 = keep; default 16); by: rows per 64-wide tile, 1, 2 or 4 (0 = keep;
-   default 4); variant:
-   kernel variant flags (-1 = keep; default 1): bit 0 = non-temporal loads and
+   (0 = keep; default 16, 32 when nm = 0); by: rows per 64-wide tile, 1, 2
+   or 4 (0 = keep; default 4); variant: kernel variant flags (-1 = keep;
+   default 1): bit 0 = non-temporal loads and
    stores for the single-use streams of the stress update, bit 1 = side-column
    PML kernels on a second stream alongside the interior ones, bit 2 =
    interior tiles 32 cells wide with 2 by rows (narrow grids idle fewer

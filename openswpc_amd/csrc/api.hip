@@ -158,7 +158,11 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     memset(c, 0, sizeof(*c));
     c->desc = *d;
     c->device = d->device;
-    c->kchunk = 16;   // fastest on 1024x1024x512 among 8..128 (the stress and side-column launches favour short chunks)
+    // k chunk: 16 planes was fastest on 1024x1024x512 with NM = 3 among 8..128
+    // (the stress and side-column launches favour short chunks); an elastic
+    // update streams far fewer arrays, so each chunk's priming loads weigh
+    // more and 32 planes is faster there (256x256x192, NM = 0)
+    c->kchunk = d->nm > 0 ? 16 : 32;
     c->by = 4;
     c->variant = 1;
     if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
