@@ -58,6 +58,7 @@ static void reset_graph(swpc_ctx *c)
 // over the full depth, interior columns over the bottom na planes.
 static int build_pml_tiles(swpc_ctx *c)
 {
+    if (c->d_pml_tiles) c->bytes -= (long long)c->npml_tiles * 4 * (long long)sizeof(int);
     dfree(c->d_pml_tiles);
     c->d_pml_tiles = nullptr;
     c->npml_tiles = 0;
@@ -126,6 +127,7 @@ static int build_zchunks(swpc_ctx *c)
         }
         z[nbz] = nz + 1;
     }
+    if (c->d_zchunks) c->bytes -= 2LL * (c->nbz + 1) * (long long)sizeof(int);
     dfree(c->d_zchunks);
     c->d_zchunks = nullptr;
     c->nbz = nbz;
@@ -1065,7 +1067,9 @@ extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by, int variant)
     if (kchunk < 0 || by < 0 || by == 3 || by > 4)
         return fail(c, SWPC_EINVAL, "swpc_set_tuning: kchunk >= 0, by in {1, 2, 4} (launch bound 256)");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the tables rebuilt below may be read by work on any stream of the
+    // device (the context's own, a batch's that steps this member)
+    HIPCHK(c, hipDeviceSynchronize());
     if (kchunk > 0) c->kchunk = kchunk;
     if (by > 0) c->by = by;
     if (variant >= 0) c->variant = variant;

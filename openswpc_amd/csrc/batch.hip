@@ -86,6 +86,8 @@ static void free_tables(swpc_batch *b)
 // (re)build the argument arrays and tile tables from the members' current state
 static int build_tables(swpc_batch *b)
 {
+    // the batch stream may still run replays that read the old tables
+    BCHK(b, hipStreamSynchronize(b->stream));
     breset_graph(b);
     free_tables(b);
     const int n = (int)b->mem.size();

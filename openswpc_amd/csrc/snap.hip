@@ -69,10 +69,18 @@ __global__ void __launch_bounds__(256) k_snap(SnapArgs a)
         float rot_z = (a.Vy[q + 1] - a.Vy[q]) * a.r20x - (a.Vx[q + Y] - a.Vx[q]) * a.r20y;
         const float lam = a.lam[q];
         div = div * lam / fabsf(lam + EPSL);
-        const float syz = fabsf(a.Syz[q]), sxz = fabsf(a.Sxz[q]), sxy = fabsf(a.Sxy[q]);
-        rot_x = rot_x * syz / (syz + EPSL);
-        rot_y = rot_y * sxz / (sxz + EPSL);
-        rot_z = rot_z * sxy / (sxy + EPSL);
+        const float syz = a.Syz[q], sxz = a.Sxz[q], sxy = a.Sxy[q];
+        if (T.plane == SWPC_SNP_YZ) {
+            // the yz section masks with abs(S + eps) (m_snap.f90:1028-1030)
+            rot_x = rot_x * fabsf(syz) / fabsf(syz + EPSL);
+            rot_y = rot_y * fabsf(sxz) / fabsf(sxz + EPSL);
+            rot_z = rot_z * fabsf(sxy) / fabsf(sxy + EPSL);
+        } else {
+            // xz, xy, fs and ob mask with abs(S) + eps (m_snap.f90:1121-1123)
+            rot_x = rot_x * fabsf(syz) / (fabsf(syz) + EPSL);
+            rot_y = rot_y * fabsf(sxz) / (fabsf(sxz) + EPSL);
+            rot_z = rot_z * fabsf(sxy) / (fabsf(sxy) + EPSL);
+        }
         fr[p] = div * a.UC * a.M0 * 1e-3f;
         fr[p + npt] = rot_x * a.UC * a.M0 * 1e-3f;
         fr[p + 2 * npt] = rot_y * a.UC * a.M0 * 1e-3f;

@@ -45,7 +45,9 @@ def host_lib() -> C.CDLL:
     L.swpch_x2i.argtypes = [f, f, f]
     L.swpch_stable_dt.restype = f
     L.swpch_stable_dt.argtypes = [f, f, f, f]
-    L.swpch_vmodel_layers.argtypes = [i, i, i, i, f, f, f, i, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_int)]
+    L.swpch_vmodel_layers.argtypes = [i, i, i, i, f, f, f, i, i, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_int)]
+    L.swpch_seawater_vel.restype = f
+    L.swpch_seawater_vel.argtypes = [f, i]
     L.swpch_perturb.argtypes = [i, i, i, fp, f, f, f, fp, fp, fp]
     L.swpch_medium_setup.argtypes = [i, i, i, i, i, f, f, f, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_float),
                                      ip, ip, ip, ip, ip, ip, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -127,6 +129,7 @@ class ModelSpec:
     vmodel: str = "lhm"          # 'lhm' (homogeneous layers) | 'lgm' (linear gradients)
     layers: list = field(default_factory=list)   # rows: depth rho vp vs qp qs
     vcut: float = 0.0
+    munk: bool = False            # seawater sound speed: Munk's profile instead of 1.5 km/s
     xi: np.ndarray | None = None  # optional relative perturbation, (k,i,j) halo layout
     sources: list = field(default_factory=list)
     bf_mode: bool = False
@@ -222,7 +225,7 @@ def build_inputs(spec: ModelSpec) -> Inputs:
     rc = C.c_int()
     kind = {"lhm": 0, "lgm": 1}[spec.vmodel]
     f32 = np.float32
-    L.swpch_vmodel_layers(kind, nx, ny, nz, spec.zbeg, spec.dz, spec.vcut, len(spec.layers), layers,
+    L.swpch_vmodel_layers(kind, nx, ny, nz, spec.zbeg, spec.dz, spec.vcut, int(spec.munk), len(spec.layers), layers,
                           rho, lam, mu, taup, taus, C.byref(rc))
     if rc.value != 0:
         raise ValueError(f"velocity model rejected (rc={rc.value})")

@@ -49,7 +49,7 @@ program swpc_fdrive
     allocate (kfs_bot(-2:nx + 3, -2:ny + 3), kob_top(-2:nx + 3, -2:ny + 3), kob_bot(-2:nx + 3, -2:ny + 3))
     layers(:, 1) = [0.0, 2.7, 5.5, 3.2, 320.0, 160.0]
     layers(:, 2) = [5.0, 2.9, 6.5, 3.7, 370.0, 185.0]
-    call swpch_vmodel_layers(0, nx, ny, nz, ZBEG, DX, 0.0, NL, layers, rho, lam, mu, taup, taus, rc)
+    call swpch_vmodel_layers(0, nx, ny, nz, ZBEG, DX, 0.0, 0, NL, layers, rho, lam, mu, taup, taus, rc)
     call check(rc, 'velocity model')
     call swpch_medium_setup(nx, ny, nz, NA, NM, 0.05, 5.0, 1.0, rho, lam, mu, taup, taus, ts, zeta, &
                             kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot, vmin, vmax)

@@ -20,7 +20,7 @@ module swpc_host
     real(DP), parameter :: PI = 3.141592653589793238462643383279502884_DP
     real(SP), parameter :: EPSL = epsilon(1.0_SP)
 
-    public :: swpch_i2x, swpch_x2i, swpch_vmodel_layers, swpch_perturb, swpch_medium_setup
+    public :: swpch_i2x, swpch_x2i, swpch_seawater_vel, swpch_vmodel_layers, swpch_perturb, swpch_medium_setup
     public :: swpch_kernel_coef, swpch_pml_profile, swpch_cerjan_profile, swpch_sdr2moment
     public :: swpch_momentrate, swpch_stf_table, swpch_station_index, swpch_stable_dt
 
@@ -47,14 +47,28 @@ contains
         swpch_stable_dt = (6.0_SP / 7.0_SP) * hh / vmax
     end function swpch_stable_dt
 
+    !! Sound speed of seawater at depth z [km] (src/shared/m_seawater.f90:34-47):
+    !! 1.5 km/s, or Munk's canonical profile (epsilon 0.00737, sound axis at
+    !! 1300 m, scale depth 650 m) when munk is nonzero.
+    real(c_float) function swpch_seawater_vel(z, munk) bind(C, name='swpch_seawater_vel')
+        real(c_float), value :: z
+        integer(c_int), value :: munk
+        real(DP) :: e, zb
+        e = 0.0_DP
+        if (munk /= 0) e = 0.00737_DP
+        zb = 2 * (real(z, DP) * 1000.0_DP - 1300.0_DP) / 1300.0_DP
+        swpch_seawater_vel = real(1.5_DP * (1.0_DP + e * (zb - 1.0_DP + exp(-zb))), SP)
+    end function swpch_seawater_vel
+
     !! Layered medium on the halo grid: kind 0 = homogeneous layers
     !! (vmodel_lhm, src/swpc_3d/m_vmodel_lhm.f90:20-158), kind 1 = linear
     !! gradient between layer tops (vmodel_lgm, m_vmodel_lgm.f90:20-174).
-    !! layers(6,nl) = depth, rho, vp, vs, qp, qs.  Cells above depth(1) with
-    !! z < 0 are air; the ocean column is not supported (rc = -2).
-    subroutine swpch_vmodel_layers(kind, nx, ny, nz, zbeg, dz, vcut, nl, layers, rho, lam, mu, qp, qs, rc) &
+    !! layers(6,nl) = depth, rho, vp, vs, qp, qs.  Above depth(1): air where
+    !! z < 0, seawater (rho 1, mu 0, Q 1e6, sound speed swpch_seawater_vel)
+    !! where 0 <= z (m_vmodel_lhm.f90:111-133).
+    subroutine swpch_vmodel_layers(kind, nx, ny, nz, zbeg, dz, vcut, munk, nl, layers, rho, lam, mu, qp, qs, rc) &
         bind(C, name='swpch_vmodel_layers')
-        integer(c_int), value :: kind, nx, ny, nz, nl
+        integer(c_int), value :: kind, nx, ny, nz, nl, munk
         real(c_float), value :: zbeg, dz, vcut
         real(c_float), intent(in) :: layers(6, nl)
         real(c_float), intent(out) :: rho(-2:nz + 3, -2:nx + 3, -2:ny + 3), lam(-2:nz + 3, -2:nx + 3, -2:ny + 3)
@@ -82,11 +96,12 @@ contains
         do k = -2, nz + 3
             z = swpch_i2x(k, zbeg, dz)
             if (z < dep(1)) then
-                if (z >= 0.0_SP) then
-                    rc = -2
-                    return
+                if (z < 0.0_SP) then
+                    rho1 = 0.001_SP; vp1 = 0.0_SP; vs1 = 0.0_SP; qp1 = 10.0_SP; qs1 = 10.0_SP
+                else
+                    rho1 = 1.0_SP; vp1 = 1.0_SP * swpch_seawater_vel(z, munk); vs1 = 0.0_SP
+                    qp1 = 1000000.0_SP; qs1 = 1000000.0_SP
                 end if
-                rho1 = 0.001_SP; vp1 = 0.0_SP; vs1 = 0.0_SP; qp1 = 10.0_SP; qs1 = 10.0_SP
             else if (kind == 0) then
                 do l = 1, nl
                     if (z >= dep(l)) then

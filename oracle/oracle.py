@@ -68,7 +68,9 @@ def lib(prec: int = 32) -> C.CDLL:
     L.orc_seismic_moment.argtypes = [f]
     for nm in ("orc_vmodel_lhm", "orc_vmodel_lgm"):
         getattr(L, nm).restype = i
-        getattr(L, nm).argtypes = [i, i, i, f, f, f, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+        getattr(L, nm).argtypes = [i, i, i, f, f, f, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_seawater_vel.restype = f
+    L.orc_seawater_vel.argtypes = [f, i]
     L.orc_homogenize_absorber.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p]
     L.orc_q_to_tau.argtypes = [C.c_int64, i, f, _f32p, _f32p]
     L.orc_relaxed_medium.argtypes = [C.c_int64, i, _f32p, f, _f32p, _f32p, _f32p, _f32p]
@@ -124,6 +126,7 @@ class Config:
     layers: list = field(default_factory=lambda: [[0.0, 2.7, 5.5, 3.2, 320.0, 160.0],
                                                   [5.0, 2.9, 6.5, 3.7, 370.0, 185.0]])
     vcut: float = 0.0
+    munk: bool = False               # seawater: Munk's profile instead of 1.5 km/s (m_seawater.f90)
     # sources: list of dict(x,y,z,t0,tr,mo, mxx..mxy) or body force dict(x,y,z,t0,tr,fx,fy,fz)
     sources: list = field(default_factory=list)
     bf_mode: bool = False
@@ -220,9 +223,11 @@ def build_model(cfg: Config) -> Model:
     layers = np.ascontiguousarray(np.asarray(cfg.layers, np.float32).reshape(-1))
     nl = len(cfg.layers)
     if cfg.vmodel == "lhm":
-        rc = L.orc_vmodel_lhm(nx, ny, nz, cfg.zbeg, cfg.dz, cfg.vcut, nl, layers, rho, lam, mu, qp, qs)
+        rc = L.orc_vmodel_lhm(nx, ny, nz, cfg.zbeg, cfg.dz, cfg.vcut, int(cfg.munk), nl, layers, rho, lam, mu,
+                              qp, qs)
     elif cfg.vmodel == "lgm":
-        rc = L.orc_vmodel_lgm(nx, ny, nz, cfg.zbeg, cfg.dz, cfg.vcut, nl, layers, rho, lam, mu, qp, qs)
+        rc = L.orc_vmodel_lgm(nx, ny, nz, cfg.zbeg, cfg.dz, cfg.vcut, int(cfg.munk), nl, layers, rho, lam, mu,
+                              qp, qs)
     else:
         raise ValueError(cfg.vmodel)
     if rc != 0:

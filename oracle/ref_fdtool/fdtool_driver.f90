@@ -1,11 +1,14 @@
-! fdtool_driver.f90 -- drives the reference's own src/shared/m_fdtool.f90
-! (compiled unmodified from /root/reference by oracle/ref_fdtool/Makefile)
+! fdtool_driver.f90 -- drives the reference's own src/shared/m_fdtool.f90 and
+! src/shared/m_seawater.f90 (compiled unmodified from /root/reference by
+! oracle/ref_fdtool/Makefile)
 ! and prints input/output bit patterns used as golden vectors for the oracle.
 ! Test infrastructure only.
 program fdtool_driver
     use m_std
     use m_fdtool
+    use m_seawater
     implicit none
+    integer :: munk
     integer :: nm, im, k, it, ityp
     real(SP) :: ts(8), zeta, fmin, fmax, t, v, m6(6), dt
     real(SP) :: fmins(3) = [0.05, 0.02, 0.1], fmaxs(3) = [5.0, 2.0, 10.0]
@@ -73,5 +76,14 @@ program fdtool_driver
     do k = 1, 8
         t = 2.0 + k * 0.75
         write (*, '(A,2(1X,Z8.8))') 'mw', transfer(t, 0), transfer(seismic_moment(t), 0)
+    end do
+
+    ! seawater sound speed, constant and Munk's profile (m_seawater.f90:20-47)
+    do munk = 0, 1
+        call seawater__init(munk == 1)
+        do k = 0, 40
+            t = k * 0.173
+            write (*, '(A,1X,I0,2(1X,Z8.8))') 'sea', munk, transfer(t, 0), transfer(seawater__vel(t), 0)
+        end do
     end do
 end program fdtool_driver

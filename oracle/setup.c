@@ -198,9 +198,21 @@ void orc_sdr2moment(float strike, float dip, float rake, float *m6 /* xx yy zz y
 /* seismic_moment, src/shared/m_fdtool.f90:296-304 */
 float orc_seismic_moment(float mw) { return powf(10.0f, 1.5f * mw + 9.05f); }
 
-/* ---- vmodel_lhm (air column only), src/swpc_3d/m_vmodel_lhm.f90:20-158 ----
-   layers: nl rows of (depth, rho, vp, vs, qp, qs).  Fills every halo cell. */
-int orc_vmodel_lhm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int nl, const float *layers,
+/* ---- seawater__vel, src/shared/m_seawater.f90:34-47: 1.5 km/s, or Munk's
+   profile (epsilon 0.00737, axis at 1300 m) when munk != 0; z in km ---- */
+float orc_seawater_vel(float z, int munk)
+{
+    const double epsil = munk ? 0.00737 : 0.0;
+    const double zc = 1300.0;
+    const double zb = 2 * ((double)z * 1000.0 - zc) / zc;
+    return (float)(1.5 * (1.0 + epsil * (zb - 1.0 + exp(-zb))));
+}
+
+/* ---- vmodel_lhm, src/swpc_3d/m_vmodel_lhm.f90:20-158 ----
+   layers: nl rows of (depth, rho, vp, vs, qp, qs).  Fills every halo cell.
+   Above depth(1): air where z < 0, seawater (rho 1, mu 0, Q 1e6) where
+   0 <= z < depth(1) (:111-133). */
+int orc_vmodel_lhm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int munk, int nl, const float *layers,
                    float *rho, float *lam, float *mu, float *qp, float *qs)
 {
     if (nl < 1 || nl > 64) return -1;
@@ -218,9 +230,12 @@ int orc_vmodel_lhm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int
         float zc = orc_i2x(k, zbeg, dz);
         float vr, vl, vm, vqp, vqs;
         if (zc < dep[0]) {
-            /* air (z < 0); the seawater branch (0 <= z < depth(1)) is not restated */
-            if (zc >= 0.0f) return -2;
-            vr = 0.001f; vm = 0.0f; vl = 0.0f; vqp = 10.0f; vqs = 10.0f;
+            if (zc < 0.0f) {
+                vr = 0.001f; vm = 0.0f; vl = 0.0f; vqp = 10.0f; vqs = 10.0f;
+            } else {
+                const float vp1 = 1.0f * orc_seawater_vel(zc, munk);
+                vr = 1.0f; vm = 0.0f; vl = 1.0f * vp1 * vp1; vqp = 1000000.0f; vqs = 1000000.0f;
+            }
         } else {
             float rho1 = 0, vp1 = 0, vs1 = 0, qp1 = 0, qs1 = 0;
             for (int l = 0; l < nl; l++)
@@ -236,9 +251,9 @@ int orc_vmodel_lhm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int
     return 0;
 }
 
-/* ---- vmodel_lgm (linear-gradient layers, air column only),
-   src/swpc_3d/m_vmodel_lgm.f90:20-174 ---- */
-int orc_vmodel_lgm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int nl, const float *layers,
+/* ---- vmodel_lgm (linear-gradient layers), src/swpc_3d/m_vmodel_lgm.f90:20-174;
+   air and seawater above depth(1) as in vmodel_lhm (:110-128) ---- */
+int orc_vmodel_lgm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int munk, int nl, const float *layers,
                    float *rho, float *lam, float *mu, float *qp, float *qs)
 {
     if (nl < 1 || nl > 64) return -1;
@@ -256,8 +271,12 @@ int orc_vmodel_lgm(int nx, int ny, int nz, float zbeg, float dz, float vcut, int
         float zs = orc_i2x(k, zbeg, dz);
         float rho1, vp1, vs1, qp1, qs1;
         if (zs < dep[0]) {
-            if (zs >= 0.0f) return -2; /* seawater branch not restated */
-            rho1 = 0.001f; vp1 = 0.0f; vs1 = 0.0f; qp1 = 10.0f; qs1 = 10.0f;
+            if (zs < 0.0f) {
+                rho1 = 0.001f; vp1 = 0.0f; vs1 = 0.0f; qp1 = 10.0f; qs1 = 10.0f;
+            } else {
+                rho1 = 1.0f; vp1 = 1.0f * orc_seawater_vel(zs, munk); vs1 = 0.0f;
+                qp1 = 1000000.0f; qs1 = 1000000.0f;
+            }
         } else {
             rho1 = r0[nl - 1]; vp1 = vp0[nl - 1]; vs1 = vs0[nl - 1]; qp1 = q0p[nl - 1]; qs1 = q0s[nl - 1];
             for (int l = 0; l < nl - 1; l++) {

@@ -717,9 +717,17 @@ void orc_snap_write(orc_state *s, int it, const int32_t *kfs, const int32_t *sw,
                         const sp_t lam = V3(s->lam, k, i, j);
                         div = div * lam / (sp_t)fabs((double)(lam + epsl));
                         const mp_t syz = V3(s->Syz, k, i, j), sxz = V3(s->Sxz, k, i, j), sxy = V3(s->Sxy, k, i, j);
-                        rot_x = (sp_t)((mp_t)rot_x * (mp_t)fabs((double)syz) / ((mp_t)fabs((double)syz) + (mp_t)epsl));
-                        rot_y = (sp_t)((mp_t)rot_y * (mp_t)fabs((double)sxz) / ((mp_t)fabs((double)sxz) + (mp_t)epsl));
-                        rot_z = (sp_t)((mp_t)rot_z * (mp_t)fabs((double)sxy) / ((mp_t)fabs((double)sxy) + (mp_t)epsl));
+                        if (plane == 1) {
+                            /* wbuf_yz_ps masks with abs(S + eps) (m_snap.f90:1028-1030); every
+                               other plane with abs(S) + eps (:1121-1123) */
+                            rot_x = (sp_t)((mp_t)rot_x * (mp_t)fabs((double)syz) / (mp_t)fabs((double)(syz + (mp_t)epsl)));
+                            rot_y = (sp_t)((mp_t)rot_y * (mp_t)fabs((double)sxz) / (mp_t)fabs((double)(sxz + (mp_t)epsl)));
+                            rot_z = (sp_t)((mp_t)rot_z * (mp_t)fabs((double)sxy) / (mp_t)fabs((double)(sxy + (mp_t)epsl)));
+                        } else {
+                            rot_x = (sp_t)((mp_t)rot_x * (mp_t)fabs((double)syz) / ((mp_t)fabs((double)syz) + (mp_t)epsl));
+                            rot_y = (sp_t)((mp_t)rot_y * (mp_t)fabs((double)sxz) / ((mp_t)fabs((double)sxz) + (mp_t)epsl));
+                            rot_z = (sp_t)((mp_t)rot_z * (mp_t)fabs((double)sxy) / ((mp_t)fabs((double)sxy) + (mp_t)epsl));
+                        }
                         fr[p + 0 * npt] = div * uc * m0 * (sp_t)1e-3f;
                         fr[p + 1 * npt] = rot_x * uc * m0 * (sp_t)1e-3f;
                         fr[p + 2 * npt] = rot_y * uc * m0 * (sp_t)1e-3f;

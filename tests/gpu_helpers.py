@@ -47,3 +47,14 @@ def small_config(**kw):
                 wav_flags=15)
     base.update(kw)
     return O.Config(**base)
+
+
+def oracle_config(spec, **kw):
+    """the oracle's Config of a host ModelSpec (same grid, medium, sources,
+    stations; wav_flags = spec.wav_what)"""
+    import dataclasses
+    names = {f.name for f in dataclasses.fields(O.Config)}
+    base = {k: getattr(spec, k) for k in names if hasattr(spec, k)}
+    base["wav_flags"] = spec.wav_what
+    base.update(kw)
+    return O.Config(**base)

@@ -18,7 +18,7 @@ NT = 600
 
 
 def test_full_grid_two_sources_same_bits_under_two_checkpoint_schemes():
-    import bench
+    from openswpc_amd import workloads as bench
     from openswpc_amd.host import build_inputs, make_simulation, set_acquisition
 
     geo = bench.adjoint_multi_geometry()

@@ -84,7 +84,7 @@ def test_full_greens_batch_matches_members_stepped_alone():
     stepped alone (the single-context path the tests above pin to the
     oracle).  20 steps carry each force's field into its free surface and,
     on the narrow members, into the side PML."""
-    import bench
+    from openswpc_amd import workloads as bench
     from openswpc_amd.host import build_inputs, make_simulation
 
     nt = 20

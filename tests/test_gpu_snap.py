@@ -81,3 +81,33 @@ def test_section_files_from_the_device(orc, tmp_path):
         assert r["frames"].shape[0] == nframe
         for f in range(nframe):
             assert ulp_diff(r["frames"][f], run.snap_frames[name][f]) == 0, (name, f)
+
+
+@pytest.mark.parametrize("shear", [-0.5, -2.0, 0.25])
+def test_ps_masks_near_epsilon(orc, shear):
+    """Shear stresses a few epsilon from zero, of either sign: the yz section
+    masks the rotation with |S|/|S + eps| (m_snap.f90:1028-1030), the other
+    planes with |S|/(|S| + eps); the device equals the oracle bit for bit."""
+    cfg = small_config(nt=1)
+    m = O.build_model(cfg)
+    par = dict(k0_xy=9, i0_yz=17, j0_xz=15, idec=2, jdec=3, kdec=2, ntdec_s=1, nframe=1)
+    run = O.OracleRun(m, 32)
+    rng = np.random.default_rng(11)
+    eps = np.float32(1.1920928955078125e-07)
+    g = gpu_from_model(m)
+    for n in ("Vx", "Vy", "Vz"):
+        run.arr[n][...] = rng.standard_normal(run.arr[n].shape).astype(np.float32)
+    for n in ("Syz", "Sxz", "Sxy"):
+        run.arr[n][...] = np.float32(shear) * eps * rng.uniform(0.5, 1.5, run.arr[n].shape).astype(np.float32)
+    for n in ("Vx", "Vy", "Vz", "Syz", "Sxz", "Sxy"):
+        g.set_field(n, run.arr[n])
+    run.snap_setup(ALL, **par)
+    run.snap_write(1)
+    g.set_snapshots(ALL, m.kfs, M0=m.M0, UC=m.UC, **par)
+    g.snap_write(1)
+    g.synchronize()
+    for pl in ("xy", "yz", "xz", "fs", "ob"):
+        got, want = g.snapshot(f"{pl}_ps", 0), run.snap_frames[f"{pl}_ps"][0]
+        assert np.any(want[1:] != 0), pl
+        assert ulp_diff(got, want) == 0, pl
+    g.close()

@@ -36,7 +36,15 @@ def _pair(orc, hostmod, **kw):
 
 @pytest.mark.parametrize("kw", [dict(), dict(nm=2, vmodel="lgm", layers=[[0.0, 2.3, 3.5, 2.0, 200, 100],
                                                                           [12.0, 3.0, 7.8, 4.5, 450, 225]]),
-                                dict(abc="cerjan"), dict(nm=0)])
+                                dict(abc="cerjan"), dict(nm=0),
+                                # an ocean column: 1.5 km of air and 3 km of seawater over the solid
+                                dict(zbeg=-1.5, layers=[[3.0, 2.7, 5.5, 3.2, 320.0, 160.0],
+                                                        [6.0, 2.9, 6.5, 3.7, 370.0, 185.0]],
+                                     stations=[(2.1, 1.3, 0.0, 'obb'), (-3.2, 2.7, 0.0, 'fsb'),
+                                               (0.7, -2.9, 6.0, 'dep')]),
+                                dict(zbeg=-0.5, munk=True, vmodel="lgm",
+                                     layers=[[2.5, 2.3, 3.5, 2.0, 200, 100], [12.0, 3.0, 7.8, 4.5, 450, 225]],
+                                     stations=[(2.1, 1.3, 0.0, 'obb'), (0.7, -2.9, 0.0, 'obb')])])
 def test_host_matches_oracle(orc, hostmod, kw):
     m, inp = _pair(orc, hostmod, **kw)
     a = inp.arrays
@@ -59,3 +67,12 @@ def test_host_matches_oracle(orc, hostmod, kw):
     else:
         for n in ("cgx_c", "cgx_b", "cgy_c", "cgy_b", "cgz_c", "cgz_b"):
             assert ulp_diff(a[n], getattr(m, n)) == 0, n
+
+
+def test_host_seawater_velocity(orc, hostmod):
+    """both restatements of seawater__vel agree bit for bit (the oracle's is
+    pinned by the reference's own m_seawater.f90)"""
+    L = hostmod.host_lib()
+    for munk in (0, 1):
+        for z in np.linspace(0.0, 7.0, 57, dtype=np.float32):
+            assert np.float32(L.swpch_seawater_vel(z, munk)) == np.float32(O.lib(32).orc_seawater_vel(z, munk))

@@ -40,10 +40,18 @@ def _visco(**kw):
     return _cfg(**base)
 
 
-@pytest.mark.parametrize("nm", [0, 3, 1, 8])
-def test_dot_product_one_step(orc, nm):
-    cfg = _cfg(sources=[]) if nm == 0 else _visco(sources=[], nm=nm)
+OCEAN = dict(zbeg=-1.0, layers=[[1.5, 2.5, 5.0, 2.9, 150.0, 75.0], [4.0, 2.8, 6.2, 3.6, 250.0, 125.0]])
+
+
+@pytest.mark.parametrize("nm,ocean", [(0, False), (3, False), (1, False), (8, False), (0, True), (3, True)])
+def test_dot_product_one_step(orc, nm, ocean):
+    """ocean: a seawater layer puts the ocean-bottom band (kob_top..kob_bot)
+    below the free-surface band, so its 2nd-order stencils are transposed too"""
+    kw = OCEAN if ocean else {}
+    cfg = _cfg(sources=[], **kw) if nm == 0 else _visco(sources=[], nm=nm, **kw)
     m = O.build_model(cfg)
+    if ocean:
+        assert np.all(m.kob[OFF + 1:-OFF - 3, OFF + 1:-OFF - 3] > m.kfs[OFF + 1:-OFF - 3, OFF + 1:-OFF - 3])
     adj = O.OracleAdjoint(m, 64)
     run = adj.run
     rng = np.random.default_rng(21)

@@ -76,3 +76,14 @@ def test_index_maps_and_cfl(orc):
         assert f2h(L.orc_fdm_stable_dt(h2f(r[0]), h2f(r[1]), h2f(r[2]), h2f(r[3]))) == r[4]
     for r in rows("mw"):
         assert f2h(L.orc_seismic_moment(h2f(r[0]))) == r[1]
+
+
+def test_seawater_velocity(orc):
+    """seawater__vel of the reference's own m_seawater.f90: 1.5 km/s, and
+    Munk's profile when munk_profile is on, bit for bit"""
+    L = orc.lib(32)
+    n = 0
+    for r in rows("sea"):
+        assert f2h(L.orc_seawater_vel(h2f(r[1]), int(r[0]))) == r[2], r
+        n += 1
+    assert n == 82

@@ -89,7 +89,7 @@ def test_layered_models(orc):
     out = {}
     for name, fn in (("lhm", L.orc_vmodel_lhm), ("lgm", L.orc_vmodel_lgm)):
         a = [_halo3(ny, nx, nz) for _ in range(5)]
-        assert fn(nx, ny, nz, zbeg, dz, 1.5, 3, np.ascontiguousarray(layers.reshape(-1)), *a) == 0
+        assert fn(nx, ny, nz, zbeg, dz, 1.5, 0, 3, np.ascontiguousarray(layers.reshape(-1)), *a) == 0
         out[name] = a
     k = np.arange(nz + 6) - OFF
     z = zbeg + (k - 0.5) * dz
@@ -109,6 +109,60 @@ def test_layered_models(orc):
     r = 2.5 + 0.4 * f
     vs = 2.8 + 0.9 * f
     np.testing.assert_allclose(mu_g[kk], r * vs * vs, rtol=1e-5)
+
+
+def test_ocean_column(orc):
+    """Between sea level and depth(1) the column is seawater: rho 1, mu 0,
+    lam = vp^2 with vp = 1.5 km/s, Q = 1e6 (m_vmodel_lhm.f90:111-133,
+    m_vmodel_lgm.f90:110-128); Munk's profile (m_seawater.f90:34-47) gives
+    1.5 km/s at its sound axis z = 1.3 km and more above and below it."""
+    L = O.lib(32)
+    nx, ny, nz, dz, zbeg = 4, 4, 30, 0.5, -2.0
+    layers = np.array([[3.0, 2.5, 5.0, 2.8, 200, 100], [9.0, 2.9, 6.5, 3.7, 300, 150]], np.float32)
+    k = np.arange(nz + 6) - OFF
+    z = (zbeg + (k - 0.5) * dz).astype(np.float32)
+    for fn in (L.orc_vmodel_lhm, L.orc_vmodel_lgm):
+        for munk in (0, 1):
+            a = [_halo3(ny, nx, nz) for _ in range(5)]
+            assert fn(nx, ny, nz, zbeg, dz, 0.0, munk, 2, np.ascontiguousarray(layers.reshape(-1)), *a) == 0
+            rho, lam, mu, qp, qs = (x[2 + OFF, 2 + OFF] for x in a)
+            sea = (z >= 0) & (z < 3.0)
+            assert sea.sum() == 6
+            assert np.all(rho[sea] == 1.0) and np.all(mu[sea] == 0) and np.all(qp[sea] == 1e6)
+            assert np.all(qs[sea] == 1e6)
+            vp = np.sqrt(lam[sea].astype(np.float64))
+            if munk:
+                zb = 2 * (z[sea].astype(np.float64) * 1000 - 1300) / 1300
+                np.testing.assert_allclose(vp, 1.5 * (1 + 0.00737 * (zb - 1 + np.exp(-zb))), rtol=1e-6)
+                assert vp.min() > 1.5
+            else:
+                assert np.all(lam[sea] == np.float32(2.25))
+            air = z < 0
+            assert np.all(lam[air] == 0) and np.all(rho[air] == np.float32(0.001))
+            assert np.all(mu[z >= 3.0] > 0)
+    assert L.orc_seawater_vel(1.3, 1) == np.float32(1.5)
+    assert abs(L.orc_seawater_vel(0.0, 1) - 1.5 * (1 + 0.00737 * (-2 - 1 + np.exp(2)))) < 1e-6
+
+
+def test_ocean_bottom_detection(orc):
+    """With a water layer kfs is the last air cell and kob the last water
+    cell (m_medium.f90:339-394); the two bands are disjoint and 'obb'
+    stations sit on kob + 1 (m_wav.f90:209)."""
+    cfg = O.Config(nx=24, ny=22, nz=30, na=6, nm=3, zbeg=-1.0, dz=0.5, dx=0.5, dy=0.5,
+                   layers=[[2.0, 2.5, 5.0, 2.8, 200, 100], [6.0, 2.9, 6.5, 3.7, 300, 150]],
+                   stations=[(0.4, 0.3, 0.0, "obb"), (0.4, 0.3, 0.0, "fsb")])
+    m = O.build_model(cfg)
+    cells = (slice(OFF + 1, OFF + cfg.ny + 1), slice(OFF + 1, OFF + cfg.nx + 1))
+    # band windows (i-2..i+3, j-2..j+3) inside the detected part 0..n+2
+    inner = (slice(OFF + 2, OFF + cfg.ny), slice(OFF + 2, OFF + cfg.nx))
+    # z(k) = -1 + (k - 0.5) dz: air for k <= 2, water for k = 3..6, solid below
+    assert np.all(m.kfs[cells] == 2) and np.all(m.kob[cells] == 6)
+    assert np.all(m.kfs_top[cells] == 1) and np.all(m.kfs_bot[cells] == 4)
+    assert np.all(m.kob_top[inner] == 4) and np.all(m.kob_bot[cells] == 8)
+    # at the grid edges the window reaches the undetected value kbeg - 1 = 0
+    assert m.kob_top[OFF + 1, OFF + 5] == 1 and m.kob_top[OFF + 5, OFF + cfg.nx] == 1
+    assert list(m.kst) == [7, 3]
+    assert m.vmin > 2.7 and m.vmax > 6.0
 
 
 def test_cerjan_taper_closed_form(orc):

@@ -118,3 +118,36 @@ def test_u_integrates_and_maps_keep_maxima(orc):
     np.testing.assert_allclose(vmax[2], np.hypot(vmax[0], vmax[1]), rtol=1e-12)
     umax = run.snap_umax["fs"]
     np.testing.assert_allclose(umax[0], np.abs(u).max(), rtol=1e-10)
+
+
+@pytest.mark.parametrize("shear", [-0.5, -2.0, 0.25])
+def test_ps_masks_near_epsilon(orc, shear):
+    """Shear stresses of a few epsilon, negative or positive, weight the
+    rotation by |S|/|S + eps| on the yz section (m_snap.f90:1028-1030) and by
+    |S|/(|S| + eps) on every other plane (:1121-1123): closed forms that tell
+    the two masks apart (for S = -eps/2 they are 1 and 1/3)."""
+    run = _run(prec=32)
+    cfg, m = run.m.cfg, run.m
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((3, 3))
+    (px, py, pz) = _coords(cfg)
+    for n, (x, y, z) in zip(("Vx", "Vy", "Vz"), (px, py, pz)):
+        comp = "xyz".index(n[1])
+        run.arr[n][...] = A[comp, 0] * x + A[comp, 1] * y + A[comp, 2] * z
+    eps = float(np.float32(1.1920928955078125e-07))
+    s = np.float32(shear * eps)
+    for n in ("Sxy", "Sxz", "Syz"):
+        run.arr[n][...] = s
+    run.snap_setup(ALL, k0_xy=7, i0_yz=9, j0_xz=5, idec=2, jdec=3, kdec=2, ntdec_s=1, nframe=1)
+    run.snap_write(1)
+    scale = float(np.float32(m.UC)) * float(np.float32(m.M0)) * 1e-3
+    rot = np.array([A[2, 1] - A[1, 2], A[0, 2] - A[2, 0], A[1, 0] - A[0, 1]])
+    w_yz = abs(float(s)) / abs(float(s) + eps)
+    w_other = abs(float(s)) / (abs(float(s)) + eps)
+    # the two forms differ only for negative shear stresses
+    assert (abs(w_yz - w_other) > 0.1) == (shear < 0)
+    for pl in ("xy", "yz", "xz", "fs", "ob"):
+        fr = run.snap_frames[f"{pl}_ps"][0]
+        w = w_yz if pl == "yz" else w_other
+        for c in range(1, 4):
+            np.testing.assert_allclose(fr[c], rot[c - 1] * w * scale, rtol=2e-5, err_msg=f"{pl} rot {c}")
