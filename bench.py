@@ -15,7 +15,13 @@ of whole steps.  Inputs are resident in HBM before the timed region; the
 timed region is bracketed by HIP events on the library's stream.
 
 Prints one JSON line (metric, value = Gcell-updates/s, roofline of the
-dominant kernel, step roofline, CPU baseline).
+dominant kernel, step roofline, CPU baseline).  After the production grid,
+the default run measures the other forms on the same GPU and adds them under
+"forms", each with its rate over the plain form's: the 64-member batched
+Green's-function workload (autotuned, 100 graph-replayed steps, and the same
+members stepped one after another), the elastic 256x256x192 adjoint gradient
+(2000 steps, checkpoints every 45) and one source of the 384x384x256
+multi-source adjoint (2500 steps, NM = 3).
 """
 from __future__ import annotations
 
@@ -46,9 +52,9 @@ def log(*a):
 # ---------------------------------------------------------------------------
 # runs of the workloads (openswpc_amd/workloads.py)
 # ---------------------------------------------------------------------------
-def run_batch(args):
-    """--workload batch: the batched launch against the same members stepped
-    one after another, each as graph replays."""
+def measure_batch(steps, warmup, block_req):
+    """The batched launch against the same members stepped one after
+    another, each as graph replays.  Returns the measured figures."""
     from openswpc_amd.gpu import SwpcBatch
     from openswpc_amd.host import build_inputs, make_simulation
     t0 = time.perf_counter()
@@ -66,24 +72,24 @@ def run_batch(args):
     log(f"[bench] autotuned kchunk={kc} by={by} width={width} ({tune_ms:.2f} ms/step in trials, "
         f"{time.perf_counter() - t0:.1f}s)")
     ev = HipEvents()
-    block = min(args.block, max(args.steps, 1))
+    block = min(block_req, max(steps, 1))
     it = 1
-    b.step(it, args.warmup, use_graph=True, block=min(block, max(args.warmup, 1)))
-    it += args.warmup
+    b.step(it, warmup, use_graph=True, block=min(block, max(warmup, 1)))
+    it += warmup
     b.synchronize()
     e0, e1 = ev.create(), ev.create()
     ev.record(e0, b.stream())
-    b.step(it, args.steps, use_graph=True, block=block)
+    b.step(it, steps, use_graph=True, block=block)
     ev.record(e1, b.stream())
-    ms_batch = ev.elapsed_ms(e0, e1) / args.steps
-    it += args.steps
+    ms_batch = ev.elapsed_ms(e0, e1) / steps
+    it += steps
     # the same members one after another (each its own graph of `block` steps)
     for s in sims:
         s.step(it, block, use_graph=True, block=block)
     for s in sims:
         s.synchronize()
     it += block
-    nseq = max(block, (args.steps // block) * block)
+    nseq = max(block, (steps // block) * block)
     t_seq = 0.0
     for s in sims:
         a0, a1 = ev.create(), ev.create()
@@ -97,6 +103,31 @@ def run_batch(args):
     n_pml = cells - n_int
     b_si, b_vi, b_sp, b_vp = bytes_model(nm)
     bytes_step = n_int * (b_si + b_vi) + n_pml * (b_sp + b_vp)
+    b.close()
+    for s in sims:
+        s.close()
+    return dict(members=len(specs), cells=cells, block=block, kc=kc, by=by, width=width, ms_batch=ms_batch,
+                ms_seq=ms_seq, bytes_step=bytes_step)
+
+
+def batch_summary(r):
+    """the batched form as one object of the default benchmark line"""
+    cells, ms_b, ms_s = r["cells"], r["ms_batch"], r["ms_seq"]
+    return {"workload": f"{r['members']} uneven grids 96..256 x 96..256 x 96..192 (seed 6), NM=2, 10-cell PML, "
+                        f"unit single forces, 8 receivers each",
+            "cells": cells, "ms_per_step": round(ms_b, 4), "value": round(cells / (ms_b * 1e-3) / 1e9, 4),
+            "unit": "Gcell-updates/s",
+            "step_roofline_frac": round(r["bytes_step"] / (ms_b * 1e-3) / HBM_PEAK, 4),
+            "sequential_ms_per_step": round(ms_s, 4), "batched_over_sequential": round(ms_s / ms_b, 3),
+            "autotuned": {"kchunk": r["kc"], "by": r["by"], "tile_width": r["width"]}}
+
+
+def run_batch(args):
+    """--workload batch: the batched launch against the same members stepped
+    one after another, each as graph replays."""
+    r = measure_batch(args.steps, args.warmup, args.block)
+    cells, ms_batch, ms_seq, bytes_step = r["cells"], r["ms_batch"], r["ms_seq"], r["bytes_step"]
+    block, kc, by, width = r["block"], r["kc"], r["by"], r["width"]
     value = cells / (ms_batch * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -111,9 +142,9 @@ def run_batch(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"batched Green's functions: {len(specs)} uneven grids 96..256 x 96..256 x 96..192 "
-                               f"(seed 6), NM=2, 10-cell PML, unit single forces, 8 receivers each, 4 layered "
-                               f"profiles", "members": len(specs), "cells": cells, "graph_block": block,
+        "config": {"workload": f"batched Green's functions: {r['members']} uneven grids 96..256 x 96..256 x "
+                               f"96..192 (seed 6), NM=2, 10-cell PML, unit single forces, 8 receivers each, 4 "
+                               f"layered profiles", "members": r["members"], "cells": cells, "graph_block": block,
                    "autotuned": {"kchunk": kc, "by": by, "tile_width": width}},
         "step_roofline": {"bytes_per_step": bytes_step, "achieved": round(bytes_step / (ms_batch * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "frac": round(bytes_step / (ms_batch * 1e-3) / HBM_PEAK, 4)},
@@ -121,24 +152,20 @@ def run_batch(args):
                        "speedup_batched": round(ms_seq / ms_batch, 3)},
     }
     print(json.dumps(out), flush=True)
-    b.close()
-    for s in sims:
-        s.close()
 
 
 
-def run_adjoint(args):
-    """--workload adjoint: misfit gradient with checkpoints every 45 steps,
-    against the plain forward loop on the same grid"""
+def measure_adjoint(nt, block):
+    """Misfit gradient of the elastic 256 x 256 x 192 workload with
+    checkpoints every 45 steps, and the plain forward loop on the same grid."""
     from openswpc_amd.host import build_inputs, make_simulation
-    nt = args.adj_steps
     t0 = time.perf_counter()
     true_spec, init_spec, info = adjoint_specs(nt=nt)
     sim_true = make_simulation(build_inputs(true_spec))
     ev = HipEvents()
     e0, e1 = ev.create(), ev.create()
     ev.record(e0, sim_true.stream())
-    sim_true.step(1, nt, use_graph=True, block=min(args.block, nt))
+    sim_true.step(1, nt, use_graph=True, block=min(block, nt))
     ev.record(e1, sim_true.stream())
     ms_fwd = ev.elapsed_ms(e0, e1) / nt
     obs = sim_true.seismograms("vel")
@@ -149,10 +176,31 @@ def run_adjoint(args):
     J, gl, gm, gr = sim.adjoint_gradient(nt, obs, ckp=45)
     wall = time.perf_counter() - t
     sim.close()
-    nx, ny, nz = init_spec.nx, init_spec.ny, init_spec.nz
-    cells = nx * ny * nz
-    value = cells * nt / wall / 1e9
     inner = (slice(3, -3),) * 3
+    return dict(cells=init_spec.nx * init_spec.ny * init_spec.nz, nt=nt, wall=wall, ms_fwd=ms_fwd, misfit=J,
+                sign=info["sign"], grad_norms={"lam": float(np.linalg.norm(gl[inner])),
+                                               "mu": float(np.linalg.norm(gm[inner])),
+                                               "rho": float(np.linalg.norm(gr[inner]))})
+
+
+def adjoint_summary(r):
+    """the adjoint-gradient form as one object of the default benchmark line"""
+    cells, nt, wall, ms_fwd = r["cells"], r["nt"], r["wall"], r["ms_fwd"]
+    return {"workload": "elastic adjoint gradient 256x256x192, dx=0.2 km, 20-cell PML, 1 moment tensor, "
+                        f"64 receivers, {nt} steps, checkpoints every 45",
+            "ms_per_step": round(wall * 1e3 / nt, 4), "value": round(cells * nt / wall / 1e9, 4),
+            "unit": "Gcell-updates/s of gradient (forward, recomputation and adjoint of every step)",
+            "forward_ms_per_step": round(ms_fwd, 4), "forward_value": round(cells / (ms_fwd * 1e-3) / 1e9, 4),
+            "gradient_over_forward_time": round(wall * 1e3 / nt / ms_fwd, 3)}
+
+
+def run_adjoint(args):
+    """--workload adjoint: misfit gradient with checkpoints every 45 steps,
+    against the plain forward loop on the same grid"""
+    nt = args.adj_steps
+    r = measure_adjoint(nt, args.block)
+    cells, wall, ms_fwd = r["cells"], r["wall"], r["ms_fwd"]
+    value = cells * nt / wall / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -161,26 +209,22 @@ def run_adjoint(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "adjoint gradient, elastic 256x256x192, dx=0.2 km, 20-cell PML, layered model; "
                                "data from layered + Gaussian anomaly (8 %, radius 3 km, seed 7); 1 moment tensor, "
-                               "64 receivers; checkpoints every 45 steps", "anomaly_sign": info["sign"]},
+                               "64 receivers; checkpoints every 45 steps", "anomaly_sign": r["sign"]},
         "forward_plain": {"ms_per_step": round(ms_fwd, 4), "value": round(cells / (ms_fwd * 1e-3) / 1e9, 4)},
         "gradient_over_forward_time": round(wall * 1e3 / nt / ms_fwd, 3),
-        "misfit": J,
-        "grad_norms": {"lam": float(np.linalg.norm(gl[inner])), "mu": float(np.linalg.norm(gm[inner])),
-                       "rho": float(np.linalg.norm(gr[inner]))},
+        "misfit": r["misfit"],
+        "grad_norms": r["grad_norms"],
     }
     print(json.dumps(out), flush=True)
 
 
-def run_adjoint_multi(args):
-    """--workload adjoint_multi: gradients of seeded moment-tensor sources on a
-    shared 384 x 384 x 256 grid (dx 0.2 km, NM = 3, 20-cell PML, 2500 steps),
-    each with its own 128 receivers, summed on the device under the two-level
-    checkpoint scheme the planner fits into the free device memory.  The
-    first --adj-sources of the 16 seeded sources are timed; the gradient
-    rate is per source-step, the same for every source."""
+def measure_adjoint_multi(nt, ns, block):
+    """Gradients of the first ns seeded moment-tensor sources on the shared
+    384 x 384 x 256 grid (dx 0.2 km, NM = 3, 20-cell PML), each with its own
+    128 receivers, summed on the device under the two-level checkpoint
+    scheme the planner fits into the free device memory."""
     import dataclasses
     from openswpc_amd.host import build_inputs, make_simulation, set_acquisition
-    nt, ns = args.adj_steps, args.adj_sources
     geo = adjoint_multi_geometry()
     t0 = time.perf_counter()
     spec_true = adjoint_multi_spec(geo, nt=nt, true=True)
@@ -193,7 +237,7 @@ def run_adjoint_multi(args):
         sp = dataclasses.replace(spec_true, sources=[geo["sources"][s]], stations=geo["stations"][s], xi=None)
         set_acquisition(sim, inp_true, sp)
         ev.record(e0, sim.stream())
-        sim.step(1, nt, use_graph=True, block=min(args.block, nt))
+        sim.step(1, nt, use_graph=True, block=min(block, nt))
         ev.record(e1, sim.stream())
         ms = ev.elapsed_ms(e0, e1) / nt
         ms_fwd = ms if ms_fwd is None else min(ms_fwd, ms)
@@ -220,9 +264,39 @@ def run_adjoint_multi(args):
     wall = time.perf_counter() - t
     held = sim.device_bytes()
     sim.close()
-    cells = spec.nx * spec.ny * spec.nz
-    value = cells * nt * ns / wall / 1e9
     inner = (slice(3, -3),) * 3
+    return dict(geo=geo, cells=spec.nx * spec.ny * spec.nz, nt=nt, ns=ns, wall=wall, ms_fwd=ms_fwd, c1=c1, c2=c2,
+                nbytes=nbytes, nfwd=nfwd, held=held, misfits=misfits,
+                grad_norms={"lam": float(np.linalg.norm(gl[inner])), "mu": float(np.linalg.norm(gm[inner])),
+                            "rho": float(np.linalg.norm(gr[inner]))})
+
+
+def adjoint_multi_summary(r):
+    """the multi-source adjoint form as one object of the default benchmark line"""
+    cells, nt, ns, wall, ms_fwd = r["cells"], r["nt"], r["ns"], r["wall"], r["ms_fwd"]
+    return {"workload": f"multi-source adjoint 384x384x256, dx=0.2 km, NM=3, 20-cell PML, {nt} steps, {ns} of 16 "
+                        "seeded moment tensors timed, 128 receivers each, gradients summed on the device",
+            "ms_per_step": round(wall * 1e3 / (nt * ns), 4), "value": round(cells * nt * ns / wall / 1e9, 4),
+            "unit": "Gcell-updates/s of gradient per source-step",
+            "checkpoints": {"outer": r["c1"], "inner": r["c2"], "device_bytes": r["nbytes"],
+                            "forward_steps": r["nfwd"]},
+            "forward_ms_per_step": round(ms_fwd, 4), "forward_value": round(cells / (ms_fwd * 1e-3) / 1e9, 4),
+            "gradient_over_forward_time": round(wall * 1e3 / (nt * ns) / ms_fwd, 3)}
+
+
+def run_adjoint_multi(args):
+    """--workload adjoint_multi: gradients of seeded moment-tensor sources on a
+    shared 384 x 384 x 256 grid (dx 0.2 km, NM = 3, 20-cell PML, 2500 steps),
+    each with its own 128 receivers, summed on the device under the two-level
+    checkpoint scheme the planner fits into the free device memory.  The
+    first --adj-sources of the 16 seeded sources are timed; the gradient
+    rate is per source-step, the same for every source."""
+    nt, ns = args.adj_steps, args.adj_sources
+    r = measure_adjoint_multi(nt, ns, args.block)
+    geo, wall, ms_fwd, held = r["geo"], r["wall"], r["ms_fwd"], r["held"]
+    c1, c2, nbytes, nfwd, misfits = r["c1"], r["c2"], r["nbytes"], r["nfwd"], r["misfits"]
+    cells = r["cells"]
+    value = cells * nt * ns / wall / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -237,8 +311,7 @@ def run_adjoint_multi(args):
         "forward_plain": {"ms_per_step": round(ms_fwd, 4), "value": round(cells / (ms_fwd * 1e-3) / 1e9, 4)},
         "gradient_over_forward_time": round(wall * 1e3 / (nt * ns) / ms_fwd, 3),
         "misfits": misfits,
-        "grad_norms": {"lam": float(np.linalg.norm(gl[inner])), "mu": float(np.linalg.norm(gm[inner])),
-                       "rho": float(np.linalg.norm(gr[inner]))},
+        "grad_norms": r["grad_norms"],
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_adjoint(geo)
@@ -382,14 +455,18 @@ class HipEvents:
 # CPU baseline: the oracle (restated reference scheme, FP32, OpenMP)
 # ---------------------------------------------------------------------------
 def cpu_baseline(target_s=12.0):
+    """The FP32 oracle, vectorised for AVX2 and threaded with OpenMP, on a
+    384 x 384 x 256 sub-grid of the production workload: its cell size, time
+    step, layered gradient, NM = 3 and 20-cell PML (the random perturbation
+    does not change the work per cell); timed for about target_s seconds."""
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    nx, ny, nz = 192, 192, 128
-    cfg = O.Config(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, dt=0.0115, nt=2000, na=20, nm=3, zbeg=-1.0,
+    nx, ny, nz = 384, 384, 256
+    cfg = O.Config(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, dt=0.0117, nt=2000, na=20, nm=3, zbeg=-1.0,
                    vmodel="lgm", layers=[[0.0, 2.3, 3.464, 2.0, 200, 100], [60.0, 3.2, 7.794, 4.5, 450, 225]],
                    sources=[dict(x=0.0, y=0.0, z=10.0, t0=0.0, tr=1.0, mo=1e15, strike=30.0, dip=60.0, rake=90.0)])
     m = O.build_model(cfg)
-    run = O.OracleRun(m, 32)
+    run = O.OracleRun(m, 32, vectorised=True)
     run.run(2)                                   # warm-up (thread pool, page faults)
     t = time.perf_counter()
     run.run(2)
@@ -400,8 +477,9 @@ def cpu_baseline(target_s=12.0):
     dt = time.perf_counter() - t
     val = nx * ny * nz * steps / dt / 1e9
     return {"value": round(val, 5), "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP32 restatement (gcc -O2 OpenMP, {cores} threads), {nx}x{ny}x{nz} NM=3 PML, "
-                      f"{steps} steps, {dt:.1f} s"}
+            "sample": f"oracle FP32 restatement of the reference scheme (gcc -O3 -march=x86-64-v3, no FMA "
+                      f"contraction, OpenMP {cores} threads), the production scheme on a {nx}x{ny}x{nz} sub-grid "
+                      f"(NM=3, 20-cell PML, layered gradient), {steps} steps, {dt:.1f} s"}
 
 
 def read_traffic(workload):
@@ -432,6 +510,11 @@ def main():
     ap.add_argument("--block", type=int, default=50, help="steps per captured HIP graph")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="production workload: skip the batched and adjoint forms measured after it")
+    ap.add_argument("--form-steps", type=int, default=100, help="timed steps of the batched form")
+    ap.add_argument("--host-medium", action="store_true",
+                    help="build the medium with the host library and upload it, instead of on the device")
     ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per kernel with events")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--by", type=int, default=0)
@@ -459,20 +542,25 @@ def main():
         run_rmedia(args)
         return
 
-    from openswpc_amd.host import build_inputs, make_simulation
+    from openswpc_amd.host import build_inputs, make_simulation, make_simulation_device
 
     t0 = time.perf_counter()
-    spec = production_spec() if args.workload == "production" else plain_spec()
-    inp = build_inputs(spec)
-    spec.xi = None
-    log(f"[bench] host set-up {time.perf_counter() - t0:.1f}s dt={spec.dt} vmin={inp.scalars['vmin']:.3f} "
-        f"vmax={inp.scalars['vmax']:.3f}")
-    t0 = time.perf_counter()
-    sim = make_simulation(inp)
+    if args.host_medium:
+        spec = production_spec() if args.workload == "production" else plain_spec()
+        inp = build_inputs(spec)
+        log(f"[bench] host set-up {time.perf_counter() - t0:.1f}s")
+        sim = make_simulation(inp)
+    else:
+        # the medium is built on the device (swpc_medium_layered), from a
+        # random medium generated into the context's staging buffer
+        spec = production_spec() if args.workload == "production" else plain_spec()
+        sim, inp = make_simulation_device(spec)
+    setup_s = time.perf_counter() - t0
+    log(f"[bench] set-up {setup_s:.2f}s ({'host' if args.host_medium else 'device'} medium) dt={spec.dt} "
+        f"vmin={inp.scalars['vmin']:.3f} vmax={inp.scalars['vmax']:.3f}, {sim.device_bytes() / 2**30:.1f} GiB on device")
     if args.kchunk or args.by or args.variant >= 0:
         sim.set_tuning(args.kchunk, args.by, args.variant)
     del inp
-    log(f"[bench] device upload {time.perf_counter() - t0:.1f}s, {sim.device_bytes() / 2**30:.1f} GiB on device")
 
     ev = HipEvents()
     stream = sim.stream()
@@ -560,11 +648,35 @@ def main():
             "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         },
         "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
         "wall_s_timed": round(wall, 3),
         "vmax_end": [float(v) for v in vmax],
     }
-    print(json.dumps(out), flush=True)
     sim.close()
+    if workload == "production" and not args.no_forms:
+        out["forms"] = measure_forms(args, value)
+    print(json.dumps(out), flush=True)
+
+
+def measure_forms(args, plain_value):
+    """The batched Green's-function, adjoint-gradient and multi-source adjoint
+    forms, each against the plain form's rate measured above (the production
+    grid's memory is released first).  A form that fails reports its error
+    and leaves the plain figures standing."""
+    forms = {}
+    legs = [("batched_greens", lambda: batch_summary(measure_batch(args.form_steps, 5, args.form_steps))),
+            ("adjoint_gradient", lambda: adjoint_summary(measure_adjoint(2000, args.block))),
+            ("adjoint_multi", lambda: adjoint_multi_summary(measure_adjoint_multi(2500, 1, args.block)))]
+    for name, leg in legs:
+        t = time.perf_counter()
+        try:
+            r = leg()
+            r["over_plain_form"] = round(r["value"] / plain_value, 4)
+        except Exception as e:  # one form must not hide the others
+            r = {"error": f"{type(e).__name__}: {e}"}
+        log(f"[bench] form {name}: {time.perf_counter() - t:.1f}s {r}")
+        forms[name] = r
+    return forms
 
 
 if __name__ == "__main__":

@@ -211,6 +211,32 @@ int swpc_kernel_vmax(swpc_ctx *ctx, float out[3]);
 int swpc_get_field(swpc_ctx *ctx, int field, float *out);
 int swpc_set_field(swpc_ctx *ctx, int field, const float *in);
 
+/* Reads of the medium the kernels use (the module arrays rho, lam, mu, taup,
+   taus and the band maps of src/swpc_3d/m_medium.f90:31-44,339-394):
+   which = SWPC_MED_RHO..SWPC_MED_TAUS give (k,i,j) halo-3 float arrays,
+   SWPC_MED_KFS_TOP..SWPC_MED_KOB give (i,j) halo-3 int32 maps. */
+enum { SWPC_MED_RHO = 0, SWPC_MED_LAM, SWPC_MED_MU, SWPC_MED_TAUP, SWPC_MED_TAUS,
+       SWPC_MED_KFS_TOP, SWPC_MED_KFS_BOT, SWPC_MED_KOB_TOP, SWPC_MED_KOB_BOT, SWPC_MED_KOB, SWPC_NMEDIUM };
+int swpc_get_medium(swpc_ctx *ctx, int which, void *out);
+
+/* medium__setup of a layered model, built on the device
+   (src/swpc_3d/m_medium.f90:124-427 after vmodel_lhm / vmodel_lgm,
+   src/swpc_3d/m_vmodel_lhm.f90:20-158, m_vmodel_lgm.f90:20-174, with the
+   per-cell random perturbation of m_vmodel_lgm_rmed.f90:195-203): the same
+   kernel inputs swpc_set_medium takes, without the 3D arrays crossing from
+   the host.  prof: (5, nz+6) floats, rows rho, lam, mu, Qp, Qs of each plane
+   k = -2..nz+3 of the layered model.  xi: the relative perturbation on the
+   (k,i,j) halo grid, in host or device memory, or NULL.  clip: vmin, vmax,
+   rhomin of the perturbation (vcheck, src/shared/m_fdtool.f90:15-48).
+   tau: zeta and the real and imaginary parts of the mean
+   cc = <i w ts / (1 - i w ts)> at the reference frequency.  Absorber cells
+   copy the nearest interior cell; the free surface, ocean bottom and their
+   2nd-order bands are detected on the device.  Returns the kfs and kob maps
+   ((i,j) halo-3 int32, for stations and sections) and vrange = vmin (of
+   vs >= epsilon), vmax (of vp) below the free surface.  Synchronises. */
+int swpc_medium_layered(swpc_ctx *ctx, const float *prof, const float *xi, const float clip[3], const float tau[3],
+                        int32_t *kfs, int32_t *kob, float vrange[2]);
+
 /* Seismograms (ntw, ncmp, nst) like wav_vel/wav_disp/wav_stress/wav_strain
    (src/swpc_3d/m_wav.f90:37-38).  kind = one SWPC_WAV_* flag. */
 int swpc_get_seismograms(swpc_ctx *ctx, int kind, float *out);
@@ -241,7 +267,8 @@ int swpc_get_green(swpc_ctx *ctx, float *out);
 
 /* ---- random media (src/shared/m_rmedia.f90:40-173, rmedia__3dgen_ani) ----
    A realisation of a 3D random medium on an n1 x n2 x n3 grid, axis 1
-   fastest in out (n1*n2*n3 floats in host memory): uniform noise, forward
+   fastest in out (n1*n2*n3 floats in host memory, or in device memory of
+   `device`, which is written directly): uniform noise, forward
    FFT, whitening, filtering by the square root of the PSDF of type ptype
    (1 Gaussian, 2 exponential, 3 von Karman with Hurst number kappa;
    correlation lengths a1..a3, spacings d1..d3), zero mean, rms scaled to
@@ -273,12 +300,19 @@ const char *swpc_batch_last_error(const swpc_batch *batch);
    order), as replays of a captured graph of `block` steps when use_graph. */
 int swpc_batch_step(swpc_batch *batch, int it0, int nsteps, int use_graph, int block);
 int swpc_batch_synchronize(swpc_batch *batch);
-/* Autotuner: times candidate k-chunk depths, tile rows and interior tile
-   widths (64 or 32 cells) on trial_steps dry steps (step index 0: no
-   source, no recording) and keeps the fastest.  Call it after set-up and
-   before the first step, while every field is zero, so the trials leave the
-   state untouched.  Outputs may be NULL. */
+/* Autotuner: times candidate k-chunk depths, tile rows, interior tile
+   widths (64 or 32 cells) and member packings on trial_steps dry steps
+   (step index 0: no source, no recording) and keeps the fastest.  Call it
+   after set-up and before the first step, while every field is zero, so the
+   trials leave the state untouched.  Outputs may be NULL; the packing chosen
+   is read with swpc_batch_get_packing. */
 int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, int *width, float *ms_per_step);
+/* Order of the members' tiles in the batched launches: 0 member-major,
+   members largest first (the default); 1 chunk-major, every member's first
+   k chunk, then every member's second, and so on.  Results do not depend
+   on it. */
+int swpc_batch_set_packing(swpc_batch *batch, int packing);
+int swpc_batch_get_packing(const swpc_batch *batch);
 void *swpc_batch_stream(swpc_batch *batch);
 
 /* ---- discrete adjoint (ADE-CFS PML; elastic or with Zener memory variables) ----
@@ -336,17 +370,25 @@ int64_t swpc_device_bytes(const swpc_ctx *ctx);
 /* Launch-shape tuning knobs.  kchunk: k-planes marched per thread block
    (0 = keep; default 16, 32 when nm = 0); by: rows per 64-wide tile, 1, 2
    or 4 (0 = keep; default 4); variant: kernel variant flags (-1 = keep;
-   default 1): bit 0 = non-temporal loads and
+   default 9): bit 0 = non-temporal loads and
    stores for the single-use streams of the stress update, bit 1 = side-column
    PML kernels on a second stream alongside the interior ones, bit 2 =
    interior tiles 32 cells wide with 2 by rows (narrow grids idle fewer
-   lanes), bit 4 = the adjoint copies its step records instead of stepping
-   out of place into them.  All variants give bit-identical results. */
+   lanes), bit 3 = the interior velocity update loads plane k+1 while it
+   computes plane k, bit 4 = the adjoint copies its step records instead of
+   stepping out of place into them.  All variants give bit-identical results. */
 int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a
    caller can time or order work against it. */
 void *swpc_stream(swpc_ctx *ctx);
+
+/* The context's staging buffer: device memory of (nz+6)(nx+6)(ny+6) floats
+   (*nfloats), the size of one (k,i,j) halo array, which the library uses for
+   layout transposes.  A caller may fill it between calls, e.g. with a random
+   medium from swpc_rmedia_3d, and pass it as the xi of swpc_medium_layered;
+   the next transfer call overwrites it.  NULL for a NULL context. */
+void *swpc_staging_buffer(swpc_ctx *ctx, int64_t *nfloats);
 
 #ifdef __cplusplus
 }

@@ -630,7 +630,6 @@ struct AdjWorkspace {
     int nsum = 0;  // sources summed since the last reset
 };
 
-int adj_fail(swpc_ctx *c, int code, const char *msg);
 
 namespace {
 
@@ -681,7 +680,7 @@ int ws_open(swpc_ctx *c)
         !walloc(c, w.qq, 6 * nf) || !walloc(c, w.gsrc, 3 * nf) || !walloc(c, w.grad, 3 * nf) ||
         !walloc(c, w.cls, w.z.cls()) || !walloc(c, w.adjR, w.z.nR)) {
         swpc_adj_release(c);
-        return adj_fail(c, SWPC_ENOMEM, "adjoint: device memory for the adjoint fields");
+        return ctx_fail(c, SWPC_ENOMEM, "adjoint: device memory for the adjoint fields");
     }
     // the density kernels write every cell and never the halo, where the
     // gathers then read zeros
@@ -694,12 +693,12 @@ int ws_open(swpc_ctx *c)
 int check_ready(swpc_ctx *c)
 {
     if (c->desc.abc != SWPC_ABC_PML || c->npml == 0)
-        return adj_fail(c, SWPC_ESTATE, "adjoint: the adjoint covers the PML scheme");
+        return ctx_fail(c, SWPC_ESTATE, "adjoint: the adjoint covers the PML scheme");
     if (!c->has_medium || !c->has_pml || (c->desc.nm > 0 && !c->has_visco))
-        return adj_fail(c, SWPC_ESTATE, "adjoint: medium, visco coefficients or PML not set");
+        return ctx_fail(c, SWPC_ESTATE, "adjoint: medium, visco coefficients or PML not set");
     if (c->nst <= 0 || !(c->wav_what & SWPC_WAV_V) || !c->d_wav[0])
-        return adj_fail(c, SWPC_ESTATE, "adjoint: velocity receivers not set");
-    if (c->snap_ntask > 0) return adj_fail(c, SWPC_ESTATE, "adjoint: disable sections first");
+        return ctx_fail(c, SWPC_ESTATE, "adjoint: velocity receivers not set");
+    if (c->snap_ntask > 0) return ctx_fail(c, SWPC_ESTATE, "adjoint: disable sections first");
     return SWPC_OK;
 }
 
@@ -726,7 +725,7 @@ extern "C" int swpc_adj_plan(swpc_ctx *c, int nt, int64_t budget, int *ckp_outer
                              int64_t *fwd_steps)
 {
     if (!c) return SWPC_EINVAL;
-    if (nt < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_plan: nt < 1");
+    if (nt < 1) return ctx_fail(c, SWPC_EINVAL, "swpc_adj_plan: nt < 1");
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
         if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) return SWPC_EHIP;
@@ -746,7 +745,7 @@ extern "C" int swpc_adj_plan(swpc_ctx *c, int nt, int64_t budget, int *ckp_outer
             break;  // a smaller c2 only recomputes more
         }
     }
-    if (!b1) return adj_fail(c, SWPC_ENOMEM, "swpc_adj_plan: no checkpoint scheme fits the budget");
+    if (!b1) return ctx_fail(c, SWPC_ENOMEM, "swpc_adj_plan: no checkpoint scheme fits the budget");
     if (ckp_outer) *ckp_outer = b1;
     if (ckp_inner) *ckp_inner = b2;
     if (bytes) *bytes = bm * (int64_t)sizeof(float);
@@ -770,7 +769,7 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
 {
     if (!c) return SWPC_EINVAL;
     if (!obs || nt < 1 || ckp_outer < 1 || ckp_inner < 0)
-        return adj_fail(c, SWPC_EINVAL, "swpc_adj_accumulate: bad arguments");
+        return ctx_fail(c, SWPC_EINVAL, "swpc_adj_accumulate: bad arguments");
     int rc = check_ready(c);
     if (rc != SWPC_OK) return rc;
     if (hipSetDevice(c->device) != hipSuccess) return SWPC_EHIP;
@@ -787,7 +786,7 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     const float *seg_before = w.seg;
     if (!wgrow(c, w.ck, w.ck_n, nck * per_state) || !wgrow(c, w.seg, w.seg_n, scheme_seg_slots(nt, c1, c2) * sstep) ||
         !wgrow(c, w.obs, w.obs_n, nobs))
-        return adj_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the checkpoints "
+        return ctx_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the checkpoints "
                                         "(swpc_adj_plan finds a scheme that fits)");
     // the steps recorded in place of the records never write the halo of a
     // field, which the next step's stencils read as zero: a new store starts
@@ -924,7 +923,7 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
         if (e == hipSuccess) e = vel_phase(it);
     }
     if (e == hipSuccess) e = record(n0L, nt, true);
-    if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
+    if (e != hipSuccess) return ctx_fail(c, SWPC_EHIP, hipGetErrorString(e));
 
     // ---- misfit on the host (double) ----
     std::vector<float> syn(nobs);
@@ -990,7 +989,7 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     // trace adjoints, each with its station-components in station order
     const int ntgt_max = 6 * c->nst;
     if (!wgrow(c, w.rec, w.rec_n, 4LL * ntgt_max + 1))
-        return adj_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the record table");
+        return ctx_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the record table");
     long long *rec_tgt = (long long *)w.rec;
     int *rec_ptr = (int *)(w.rec + 2LL * ntgt_max);
     int *rec_src = rec_ptr + ntgt_max + 1;
@@ -1086,7 +1085,7 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
         hipLaunchKernelGGL(k_add, dim3(1024), dim3(256), 0, s, w.grad, w.gsrc, 3 * nf);
         e = hipGetLastError();
     }
-    if (e != hipSuccess) return adj_fail(c, SWPC_EHIP, hipGetErrorString(e));
+    if (e != hipSuccess) return ctx_fail(c, SWPC_EHIP, hipGetErrorString(e));
     if (hipStreamSynchronize(s) != hipSuccess) return SWPC_EHIP;
     w.nsum += 1;
     return SWPC_OK;
@@ -1096,7 +1095,7 @@ extern "C" int swpc_adj_get_gradient(swpc_ctx *c, float *g_lam, float *g_mu, flo
 {
     if (!c) return SWPC_EINVAL;
     if (!c->adj_ws || c->adj_ws->nsum == 0)
-        return adj_fail(c, SWPC_ESTATE, "swpc_adj_get_gradient: no gradient summed since the last reset");
+        return ctx_fail(c, SWPC_ESTATE, "swpc_adj_get_gradient: no gradient summed since the last reset");
     if (hipSetDevice(c->device) != hipSuccess) return SWPC_EHIP;
     hipStream_t s = c->stream;
     const long long nf = c->adj_ws->z.nf;
@@ -1117,7 +1116,7 @@ extern "C" int swpc_adj_gradient(swpc_ctx *c, int nt, const float *obs, int ckp,
                                  float *g_mu, float *g_rho)
 {
     if (!c) return SWPC_EINVAL;
-    if (!obs || nt < 1 || ckp < 1) return adj_fail(c, SWPC_EINVAL, "swpc_adj_gradient: bad arguments");
+    if (!obs || nt < 1 || ckp < 1) return ctx_fail(c, SWPC_EINVAL, "swpc_adj_gradient: bad arguments");
     int rc = check_ready(c);
     if (rc == SWPC_OK) rc = swpc_adj_reset(c);
     if (rc == SWPC_OK) rc = swpc_adj_accumulate(c, nt, obs, ckp, ckp, misfit);

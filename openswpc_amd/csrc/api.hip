@@ -138,13 +138,21 @@ static int build_zchunks(swpc_ctx *c)
     return SWPC_OK;
 }
 
-int adj_fail(swpc_ctx *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
+int ctx_fail(swpc_ctx *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
+
+void ctx_reset_graph(swpc_ctx *c) { reset_graph(c); }
 
 extern "C" const char *swpc_last_error(const swpc_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
 extern "C" int64_t swpc_device_bytes(const swpc_ctx *ctx) { return ctx ? ctx->bytes : 0; }
 
 extern "C" void *swpc_stream(swpc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" void *swpc_staging_buffer(swpc_ctx *ctx, int64_t *nfloats)
+{
+    if (nfloats) *nfloats = ctx ? ctx->staging_n : 0;
+    return ctx ? (void *)ctx->staging : nullptr;
+}
 
 extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
 {
@@ -166,7 +174,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     // more and 32 planes is faster there (256x256x192, NM = 0)
     c->kchunk = d->nm > 0 ? 16 : 32;
     c->by = 4;
-    c->variant = 1;
+    c->variant = 1 | 8;
     if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipStreamCreate(&c->stream2) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1026,6 +1034,27 @@ extern "C" int swpc_get_field(swpc_ctx *c, int field, float *out)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, launch_dev_to_ref(c, c->fields[field], c->staging, c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->staging, (size_t)c->staging_n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+extern "C" int swpc_get_medium(swpc_ctx *c, int which, void *out)
+{
+    if (!c || !out || which < 0 || which >= SWPC_NMEDIUM) return SWPC_EINVAL;
+    if (!c->has_medium) return fail(c, SWPC_ESTATE, "swpc_get_medium: medium not set");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (which <= SWPC_MED_TAUS) {
+        float *src[5] = {c->rho, c->lam, c->mu, c->taup, c->taus};
+        HIPCHK(c, launch_dev_to_ref(c, src[which], c->staging, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out, c->staging, (size_t)c->staging_n * sizeof(float), hipMemcpyDeviceToHost,
+                                 c->stream));
+    } else {
+        const int *src[5] = {c->kfs_top, c->kfs_bot, c->kob_top, c->kob_bot, c->kob};
+        const Grid &g = c->g;
+        const size_t w = (size_t)(g.nx + 6) * sizeof(int), sp = (size_t)g.pitch * sizeof(int);
+        HIPCHK(c, hipMemcpy2DAsync(out, w, src[which - SWPC_MED_KFS_TOP] + g2idx(g, -2, -2), sp, w, g.ny + 6,
+                                   hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SWPC_OK;
 }

@@ -254,10 +254,15 @@ hipError_t launch_wav_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
 // --------------------------------------------------------------------------
 // batched receivers and sources: one launch for every member context
 // --------------------------------------------------------------------------
-size_t batch_aux_size(int kind) { return kind == BATCH_AUX_WAV ? sizeof(WavArgs) : sizeof(SrcArgs); }
+size_t batch_aux_size(int kind)
+{
+    if (kind == BATCH_AUX_GREEN_STORE || kind == BATCH_AUX_GREEN_SOURCE) return batch_green_size(kind);
+    return kind == BATCH_AUX_WAV ? sizeof(WavArgs) : sizeof(SrcArgs);
+}
 
 int batch_aux_fill(swpc_ctx *c, int kind, void *out)
 {
+    if (kind == BATCH_AUX_GREEN_STORE || kind == BATCH_AUX_GREEN_SOURCE) return batch_green_fill(c, kind, out);
     if (kind == BATCH_AUX_WAV) {
         if (c->nst <= 0) return 0;
         WavArgs a;
@@ -276,6 +281,8 @@ int batch_aux_fill(swpc_ctx *c, int kind, void *out)
 hipError_t launch_batch_aux(int kind, const void *d_args, int nmem, int nmax, int it, const int *d_it, hipStream_t s)
 {
     if (nmem <= 0 || nmax <= 0) return hipSuccess;
+    if (kind == BATCH_AUX_GREEN_STORE || kind == BATCH_AUX_GREEN_SOURCE)
+        return launch_batch_green(kind, d_args, nmem, nmax, it, d_it, s);
     const dim3 grid((nmax + 63) / 64, nmem), block(64);
     if (kind == BATCH_AUX_WAV) hipLaunchKernelGGL(k_wav_store_b, grid, block, 0, s, (const WavArgs *)d_args, it, d_it);
     else if (kind == BATCH_AUX_BODYFORCE)

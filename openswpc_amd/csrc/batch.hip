@@ -4,14 +4,20 @@
 // (swpc_create, swpc_set_*).  Each phase of a step is one launch for all
 // members: the launch walks a tile table whose entries carry the member
 // index, and each member's launch arguments sit in HBM.  Uneven grids pack
-// into one grid of blocks with no per-member launch tail.  Sources,
-// receivers and sections stay per-member launches (a few threads each).
+// into one grid of blocks with no per-member launch tail.  Receivers,
+// sources, sections, Green's-function stores and Cerjan sponges are
+// member-tagged launches too: one per kind and step for every member that
+// has them.
 // Steps replay from a HIP graph of `block` steps driven by one device step
 // counter shared by all members.
 //
-// Tile packing: member-major (each member's tiles contiguous, so a member's
-// neighbouring tiles share an XCD's L2), members ordered by size, largest
-// first, so the smallest members fill the end of the launch.
+// Tile packing, chosen by the autotuner among:
+//   0: member-major (each member's tiles contiguous, so a member's
+//      neighbouring tiles share an XCD's L2), members ordered by size,
+//      largest first, so the smallest members fill the end of the launch;
+//   1: chunk-major: the tiles of every member's first k chunk, then of every
+//      member's second chunk, and so on (members largest first within a
+//      chunk), so each stretch of the launch streams many members at once.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -26,12 +32,17 @@ struct swpc_batch {
     int device;
     hipStream_t stream;
     int nm, by;
-    bool nt;
+    int packing;                   // tile order of the tables (see above)
     void *d_args[2];               // StepArgs of every member: stress, velocity
     int4 *d_cols, *d_side;
     int ncols, nside;
-    void *d_aux[BATCH_AUX_N];      // receiver, stress-glut and body-force arguments of the members that have them
+    void *d_aux[BATCH_AUX_N];      // receiver, source and Green's-function arguments of the members that have them
     int naux[BATCH_AUX_N], maxaux[BATCH_AUX_N];
+    void *d_snap;                  // section arguments of the members with sections
+    int nsnap, maxsnap;
+    void *d_cerj[2];               // sponge arguments (stress, velocity phase) of the Cerjan members
+    int *d_cerj_blk0[2];           // their first blocks in the flat sponge launch (ncerj + 1 entries)
+    int ncerj, ncerj_blocks[2];
     int *d_it;
     hipGraphExec_t graph;
     int graph_block;
@@ -81,6 +92,16 @@ static void free_tables(swpc_batch *b)
         b->d_aux[k] = nullptr;
         b->naux[k] = b->maxaux[k] = 0;
     }
+    bfree(b->d_snap);
+    b->nsnap = b->maxsnap = 0;
+    for (int ph = 0; ph < 2; ++ph) {
+        bfree(b->d_cerj[ph]);
+        p = b->d_cerj_blk0[ph];
+        bfree(p);
+        b->d_cerj_blk0[ph] = nullptr;
+        b->ncerj_blocks[ph] = 0;
+    }
+    b->ncerj = 0;
 }
 
 // (re)build the argument arrays and tile tables from the members' current state
@@ -99,7 +120,9 @@ static int build_tables(swpc_batch *b)
     });
     const size_t asz = step_args_size();
     std::vector<char> args[2] = {std::vector<char>(asz * n), std::vector<char>(asz * n)};
-    std::vector<int4> cols, side;
+    // per member, its interior tiles and side tiles grouped by k chunk
+    std::vector<std::vector<std::vector<int4>>> mcols(n), mside(n);
+    int maxchunks = 0;
     for (int m : order) {
         swpc_ctx *c = b->mem[m];
         if (c->by != b->by) return bfail(b, SWPC_ESTATE, "member %d has tile rows %d, batch %d", m, c->by, b->by);
@@ -107,18 +130,37 @@ static int build_tables(swpc_batch *b)
         fill_batch_args(c, args[1].data() + asz * m, 1);
         int nbx, nby, nbz;
         member_tiles(c, nbx, nby, nbz);
+        const int nch = (c->g.nz + c->kchunk - 1) / c->kchunk;
+        mcols[m].assign(std::max(nbz, nch), {});
+        mside[m].assign(std::max(nbz, nch), {});
+        maxchunks = std::max(maxchunks, (int)mcols[m].size());
         if (c->g.nxk > 0 && c->g.nyk > 0)
             for (int bz = 0; bz < nbz; ++bz)
                 for (int by = 0; by < nby; ++by)
-                    for (int bx = 0; bx < nbx; ++bx) cols.push_back(make_int4(m, bx, by, bz));
+                    for (int bx = 0; bx < nbx; ++bx) mcols[m][bz].push_back(make_int4(m, bx, by, bz));
         if (c->desc.abc == SWPC_ABC_PML && c->npml_tiles > 0) {
             std::vector<int4> t(c->npml_tiles);
             BCHK(b, hipMemcpy(t.data(), c->d_pml_tiles, t.size() * sizeof(int4), hipMemcpyDeviceToHost));
             for (const int4 &x : t) {
                 if (x.z > 0xFFFF || x.w > 0x7FFF) return bfail(b, SWPC_EINVAL, "member %d: nz too large", m);
-                side.push_back(make_int4(x.x, x.y, x.z | (x.w << 16), m));
+                const int ch = std::min((x.z - 1) / c->kchunk, (int)mside[m].size() - 1);
+                mside[m][ch].push_back(make_int4(x.x, x.y, x.z | (x.w << 16), m));
             }
         }
+    }
+    std::vector<int4> cols, side;
+    if (b->packing == 1) {
+        for (int ch = 0; ch < maxchunks; ++ch)
+            for (int m : order) {
+                if (ch < (int)mcols[m].size()) cols.insert(cols.end(), mcols[m][ch].begin(), mcols[m][ch].end());
+                if (ch < (int)mside[m].size()) side.insert(side.end(), mside[m][ch].begin(), mside[m][ch].end());
+            }
+    } else {
+        for (int m : order)
+            for (size_t ch = 0; ch < mcols[m].size(); ++ch) {
+                cols.insert(cols.end(), mcols[m][ch].begin(), mcols[m][ch].end());
+                side.insert(side.end(), mside[m][ch].begin(), mside[m][ch].end());
+            }
     }
     for (int ph = 0; ph < 2; ++ph) {
         BCHK(b, hipMalloc(&b->d_args[ph], asz * n));
@@ -153,6 +195,50 @@ static int build_tables(swpc_batch *b)
         b->naux[k] = cnt;
         b->maxaux[k] = mx;
     }
+    // sections: one launch for every member that has them
+    {
+        const size_t sz = batch_snap_size();
+        std::vector<char> buf(sz * n);
+        int cnt = 0, mx = 0;
+        for (int m = 0; m < n; ++m) {
+            const int t = batch_snap_fill(b->mem[m], buf.data() + sz * cnt);
+            if (t > 0) {
+                ++cnt;
+                mx = std::max(mx, t);
+            }
+        }
+        if (cnt > 0) {
+            BCHK(b, hipMalloc(&b->d_snap, sz * cnt));
+            BCHK(b, hipMemcpy(b->d_snap, buf.data(), sz * cnt, hipMemcpyHostToDevice));
+        }
+        b->nsnap = cnt;
+        b->maxsnap = mx;
+    }
+    // Cerjan sponges: one flat launch per phase over the sponge members' blocks
+    for (int ph = 0; ph < 2; ++ph) {
+        const size_t sz = batch_cerjan_size();
+        std::vector<char> buf(sz * n);
+        std::vector<int> blk0(1, 0);
+        long long total = 0;
+        int cnt = 0;
+        for (int m = 0; m < n; ++m) {
+            const int t = batch_cerjan_fill(b->mem[m], ph, buf.data() + sz * cnt);
+            if (t > 0) {
+                ++cnt;
+                total += t;
+                if (total > 0x7FFFFFFFLL) return bfail(b, SWPC_EINVAL, "sponge members too large for one launch");
+                blk0.push_back((int)total);
+            }
+        }
+        if (cnt > 0) {
+            BCHK(b, hipMalloc(&b->d_cerj[ph], sz * cnt));
+            BCHK(b, hipMemcpy(b->d_cerj[ph], buf.data(), sz * cnt, hipMemcpyHostToDevice));
+            BCHK(b, hipMalloc((void **)&b->d_cerj_blk0[ph], blk0.size() * sizeof(int)));
+            BCHK(b, hipMemcpy(b->d_cerj_blk0[ph], blk0.data(), blk0.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        b->ncerj = cnt;
+        b->ncerj_blocks[ph] = (int)total;
+    }
     for (int m = 0; m < n; ++m) b->gen[m] = b->mem[m]->gen;
     return SWPC_OK;
 }
@@ -172,33 +258,34 @@ static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
 {
     hipError_t e;
     hipStream_t s = b->stream;
-    for (swpc_ctx *c : b->mem) {
-        if ((e = launch_green_store(c, it, d_it, s)) != hipSuccess) return e;
-        if ((e = launch_snap(c, it, d_it, s)) != hipSuccess) return e;
-    }
+    if ((e = launch_batch_aux(BATCH_AUX_GREEN_STORE, b->d_aux[BATCH_AUX_GREEN_STORE], b->naux[BATCH_AUX_GREEN_STORE],
+                              b->maxaux[BATCH_AUX_GREEN_STORE], it, d_it, s)) != hipSuccess)
+        return e;
+    if ((e = launch_batch_snap(b->d_snap, b->nsnap, b->maxsnap, it, d_it, s)) != hipSuccess) return e;
     if ((e = launch_batch_aux(BATCH_AUX_WAV, b->d_aux[BATCH_AUX_WAV], b->naux[BATCH_AUX_WAV],
                               b->maxaux[BATCH_AUX_WAV], it, d_it, s)) != hipSuccess)
         return e;
-    if ((e = launch_batch_phase(0, b->nm, b->nt, b->by, batch_bxw(b), b->d_args[0], b->d_cols, b->ncols, b->d_side,
+    if ((e = launch_batch_phase(0, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[0], b->d_cols, b->ncols, b->d_side,
                                 b->nside, s)) != hipSuccess)
         return e;
-    for (swpc_ctx *c : b->mem)
-        if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_stress(c, s)) != hipSuccess) return e;
+    if ((e = launch_batch_cerjan(0, b->d_cerj[0], b->d_cerj_blk0[0], b->ncerj, b->ncerj_blocks[0], s)) != hipSuccess)
+        return e;
     // each member's sponge before its stress glut, as in its own step
     if ((e = launch_batch_aux(BATCH_AUX_GLUT, b->d_aux[BATCH_AUX_GLUT], b->naux[BATCH_AUX_GLUT],
                               b->maxaux[BATCH_AUX_GLUT], it, d_it, s)) != hipSuccess)
         return e;
-    if ((e = launch_batch_phase(1, b->nm, b->nt, b->by, batch_bxw(b), b->d_args[1], b->d_cols, b->ncols, b->d_side,
+    if ((e = launch_batch_phase(1, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[1], b->d_cols, b->ncols, b->d_side,
                                 b->nside, s)) != hipSuccess)
         return e;
-    for (swpc_ctx *c : b->mem)
-        if (c->desc.abc == SWPC_ABC_CERJAN && (e = launch_cerjan_vel(c, s)) != hipSuccess) return e;
+    if ((e = launch_batch_cerjan(1, b->d_cerj[1], b->d_cerj_blk0[1], b->ncerj, b->ncerj_blocks[1], s)) != hipSuccess)
+        return e;
     // sponge, body force, then the Green's-function source, per member as in its own step
     if ((e = launch_batch_aux(BATCH_AUX_BODYFORCE, b->d_aux[BATCH_AUX_BODYFORCE], b->naux[BATCH_AUX_BODYFORCE],
                               b->maxaux[BATCH_AUX_BODYFORCE], it, d_it, s)) != hipSuccess)
         return e;
-    for (swpc_ctx *c : b->mem)
-        if ((e = launch_green_source(c, it, d_it, s)) != hipSuccess) return e;
+    if ((e = launch_batch_aux(BATCH_AUX_GREEN_SOURCE, b->d_aux[BATCH_AUX_GREEN_SOURCE],
+                              b->naux[BATCH_AUX_GREEN_SOURCE], b->maxaux[BATCH_AUX_GREEN_SOURCE], it, d_it, s)) != hipSuccess)
+        return e;
     if (d_it) e = launch_counter_advance(b->d_it, s);
     return e;
 }
@@ -241,7 +328,6 @@ extern "C" int swpc_batch_create(swpc_ctx *const *members, int n, swpc_batch **o
     b->device = members[0]->desc.device;
     b->nm = members[0]->desc.nm;
     b->by = members[0]->by;
-    b->nt = (members[0]->variant & 1) != 0;
     int rc = SWPC_OK;
     if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&b->d_it, sizeof(int)) != hipSuccess) {
@@ -309,6 +395,18 @@ extern "C" int swpc_batch_step(swpc_batch *b, int it0, int nsteps, int use_graph
     return SWPC_OK;
 }
 
+extern "C" int swpc_batch_set_packing(swpc_batch *b, int packing)
+{
+    if (!b) return SWPC_EINVAL;
+    if (packing < 0 || packing > 1) return bfail(b, SWPC_EINVAL, "swpc_batch_set_packing: packing 0 or 1");
+    BCHK(b, hipSetDevice(b->device));
+    if (packing == b->packing) return SWPC_OK;
+    b->packing = packing;
+    return build_tables(b);
+}
+
+extern "C" int swpc_batch_get_packing(const swpc_batch *b) { return b ? b->packing : SWPC_EINVAL; }
+
 extern "C" int swpc_batch_synchronize(swpc_batch *b)
 {
     if (!b) return SWPC_EINVAL;
@@ -330,12 +428,13 @@ extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, 
     const int kc_cand[] = {16, 24, 32, 48, 64};
     const int by_cand[] = {2, 4};
     const int w_cand[] = {64, 32};
+    const int pk_cand[] = {0, 1};
     const int base = b->mem[0]->variant & ~4;
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
     float best = 1e30f;
-    int best_kc = b->mem[0]->kchunk, best_by = b->by, best_w = batch_bxw(b);
+    int best_kc = b->mem[0]->kchunk, best_by = b->by, best_w = batch_bxw(b), best_pk = b->packing;
     for (int w : w_cand)
         for (int byv : by_cand)
             for (int kc : kc_cand) {
@@ -343,31 +442,36 @@ extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, 
                 for (swpc_ctx *c : b->mem)
                     if ((rc = swpc_set_tuning(c, kc, byv, variant)) != SWPC_OK) goto out;
                 b->by = byv;
-                if ((rc = build_tables(b)) != SWPC_OK) goto out;
-                {
-                    hipError_t e = enqueue_batch_step(b, 0, nullptr);  // warm
-                    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-                    for (int s = 0; s < trial_steps && e == hipSuccess; ++s) e = enqueue_batch_step(b, 0, nullptr);
-                    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-                    if (e == hipSuccess) e = hipEventSynchronize(e1);
-                    float ms = 0.0f;
-                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                    if (e != hipSuccess) {
-                        rc = bfail(b, SWPC_EHIP, "tuning trial: %s", hipGetErrorString(e));
-                        goto out;
-                    }
-                    ms /= trial_steps;
-                    if (ms < best) {
-                        best = ms;
-                        best_kc = kc;
-                        best_by = byv;
-                        best_w = w;
+                for (int pk : pk_cand) {
+                    b->packing = pk;
+                    if ((rc = build_tables(b)) != SWPC_OK) goto out;
+                    {
+                        hipError_t e = enqueue_batch_step(b, 0, nullptr);  // warm
+                        if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+                        for (int s = 0; s < trial_steps && e == hipSuccess; ++s) e = enqueue_batch_step(b, 0, nullptr);
+                        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+                        if (e == hipSuccess) e = hipEventSynchronize(e1);
+                        float ms = 0.0f;
+                        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                        if (e != hipSuccess) {
+                            rc = bfail(b, SWPC_EHIP, "tuning trial: %s", hipGetErrorString(e));
+                            goto out;
+                        }
+                        ms /= trial_steps;
+                        if (ms < best) {
+                            best = ms;
+                            best_kc = kc;
+                            best_by = byv;
+                            best_w = w;
+                            best_pk = pk;
+                        }
                     }
                 }
             }
     for (swpc_ctx *c : b->mem)
         if ((rc = swpc_set_tuning(c, best_kc, best_by, base | (best_w == 32 ? 4 : 0))) != SWPC_OK) goto out;
     b->by = best_by;
+    b->packing = best_pk;
     rc = build_tables(b);
     if (kchunk_out) *kchunk_out = best_kc;
     if (by_out) *by_out = best_by;

@@ -7,6 +7,8 @@
 // scattered cells: one thread per GF cell, accumulators component-major so
 // that each of the twelve read-modify-writes is coalesced.  Every expression
 // keeps the reference's FP32 evaluation order (the FP32 build, MP = SP).
+#include <cstring>
+
 #include "swpc_internal.h"
 
 // UC_DERIV = 10.0**(-15) and UC_BF = 10.0**(-12) as the Fortran compiler
@@ -35,11 +37,10 @@ __device__ static inline float dstag(const float *F, long long c, long long s, f
 }
 
 // green__store, m_green.f90:406-545
-__global__ void __launch_bounds__(64) k_green_store(GreenArgs a)
+// GF point n of one context at step it
+__device__ static inline void green_store_body(const GreenArgs &a, int n, int it)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.ng) return;
-    const int it = a.d_it ? *a.d_it : a.it;
     const Grid &g = a.g;
     const long long p = gidx(g, a.ig[n], a.jg[n], a.kg[n]);
     const long long X = 1, Y = g.pitch, Z = g.plane;
@@ -105,10 +106,19 @@ __global__ void __launch_bounds__(64) k_green_store(GreenArgs a)
     }
 }
 
-hipError_t launch_green_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+__global__ void __launch_bounds__(64) k_green_store(GreenArgs a)
 {
-    if (!c->green_on || c->green_ng <= 0) return hipSuccess;
-    GreenArgs a;
+    green_store_body(a, blockIdx.x * blockDim.x + threadIdx.x, a.d_it ? *a.d_it : a.it);
+}
+
+// batched: context m = blockIdx.y, its arguments in HBM
+__global__ void __launch_bounds__(64) k_green_store_b(const GreenArgs *__restrict__ args, int it, const int *d_it)
+{
+    green_store_body(args[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x, d_it ? *d_it : it);
+}
+
+static void green_store_fill(swpc_ctx *c, GreenArgs &a, int it, const int *d_it)
+{
     a.Vx = c->fields[SWPC_VX]; a.Vy = c->fields[SWPC_VY]; a.Vz = c->fields[SWPC_VZ];
     a.ig = c->d_green_ijk;
     a.jg = c->d_green_ijk + c->green_ng;
@@ -125,6 +135,13 @@ hipError_t launch_green_store(swpc_ctx *c, int it, const int *d_it, hipStream_t 
     a.r41x = C41 / dx; a.r41y = C41 / dy; a.r41z = C41 / dz;
     a.it = it;
     a.d_it = d_it;
+}
+
+hipError_t launch_green_store(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+{
+    if (!c->green_on || c->green_ng <= 0) return hipSuccess;
+    GreenArgs a;
+    green_store_fill(c, a, it, d_it);
     hipLaunchKernelGGL(k_green_store, dim3((c->green_ng + 63) / 64), dim3(64), 0, (s ? s : c->stream), a);
     return hipGetLastError();
 }
@@ -143,10 +160,8 @@ struct GreenSrcArgs {
 // green__source, m_green.f90:614-649: the force split onto the two staggered
 // velocity nodes of each axis with the averaged buoyancy; all six updates
 // are made, the zero ones included
-__global__ void k_green_source(GreenSrcArgs a)
+__device__ static inline void green_source_body(const GreenSrcArgs &a, int it)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const int it = a.d_it ? *a.d_it : a.it;
     if (it < 1 || it > a.nt) return;
     const Grid &g = a.g;
     const long long p = gidx(g, a.i, a.j, a.k);
@@ -165,10 +180,21 @@ __global__ void k_green_source(GreenSrcArgs a)
     a.Vz[p - Z] = a.Vz[p - Z] + (2.0f / (r0 + rho[p - Z])) * fz / 2.0f;
 }
 
-hipError_t launch_green_source(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+__global__ void k_green_source(GreenSrcArgs a)
 {
-    if (!c->green_on || c->green_nt <= 0) return hipSuccess;
-    GreenSrcArgs a;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    green_source_body(a, a.d_it ? *a.d_it : a.it);
+}
+
+// batched: context m = blockIdx.y
+__global__ void k_green_source_b(const GreenSrcArgs *__restrict__ args, int it, const int *d_it)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    green_source_body(args[blockIdx.y], d_it ? *d_it : it);
+}
+
+static void green_source_fill(swpc_ctx *c, GreenSrcArgs &a, int it, const int *d_it)
+{
     a.Vx = c->fields[SWPC_VX]; a.Vy = c->fields[SWPC_VY]; a.Vz = c->fields[SWPC_VZ];
     a.rho = c->rho;
     a.stf = c->d_green_stf;
@@ -179,6 +205,46 @@ hipError_t launch_green_source(swpc_ctx *c, int it, const int *d_it, hipStream_t
     a.dt_dxyz = c->green_dt_dxyz;
     a.it = it;
     a.d_it = d_it;
+}
+
+hipError_t launch_green_source(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+{
+    if (!c->green_on || c->green_nt <= 0) return hipSuccess;
+    GreenSrcArgs a;
+    green_source_fill(c, a, it, d_it);
     hipLaunchKernelGGL(k_green_source, dim3(1), dim3(64), 0, (s ? s : c->stream), a);
+    return hipGetLastError();
+}
+
+// batched Green's-function store and source (swpc_batch): per-member
+// arguments, one launch for every member that has them
+size_t batch_green_size(int kind) { return kind == BATCH_AUX_GREEN_STORE ? sizeof(GreenArgs) : sizeof(GreenSrcArgs); }
+
+int batch_green_fill(swpc_ctx *c, int kind, void *out)
+{
+    if (!c->green_on) return 0;
+    if (kind == BATCH_AUX_GREEN_STORE) {
+        if (c->green_ng <= 0) return 0;
+        GreenArgs a;
+        green_store_fill(c, a, 0, nullptr);
+        memcpy(out, &a, sizeof(a));
+        return c->green_ng;
+    }
+    if (c->green_nt <= 0) return 0;
+    GreenSrcArgs a;
+    green_source_fill(c, a, 0, nullptr);
+    memcpy(out, &a, sizeof(a));
+    return 1;
+}
+
+hipError_t launch_batch_green(int kind, const void *d_args, int nmem, int nmax, int it, const int *d_it,
+                              hipStream_t s)
+{
+    if (nmem <= 0 || nmax <= 0) return hipSuccess;
+    if (kind == BATCH_AUX_GREEN_STORE)
+        hipLaunchKernelGGL(k_green_store_b, dim3((nmax + 63) / 64, nmem), dim3(64), 0, s, (const GreenArgs *)d_args,
+                           it, d_it);
+    else
+        hipLaunchKernelGGL(k_green_source_b, dim3(1, nmem), dim3(64), 0, s, (const GreenSrcArgs *)d_args, it, d_it);
     return hipGetLastError();
 }

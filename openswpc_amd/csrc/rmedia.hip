@@ -163,6 +163,11 @@ extern "C" int swpc_rmedia_3d(int device, int ptype, int n1, int n2, int n3, flo
         return rfail(SWPC_EINVAL, "swpc_rmedia_3d: bad arguments");
     RCHK(hipSetDevice(device));
     const long long n = (long long)n1 * n2 * n3;
+    // out in device memory of this GPU: the field is written there directly
+    hipPointerAttribute_t at;
+    const bool out_dev = hipPointerGetAttributes(&at, out) == hipSuccess && at.type == hipMemoryTypeDevice &&
+                         at.device == device;
+    (void)hipGetLastError();
     const int n1h = n1 / 2 + 1;
     const long long nc = (long long)n1h * n2 * n3;
     hipStream_t s;
@@ -174,7 +179,7 @@ extern "C" int swpc_rmedia_3d(int device, int ptype, int n1, int n2, int n3, flo
     RCHK(hipMalloc(&dc.p, (size_t)nc * sizeof(double2)));
     RCHK(hipMalloc(&dpart.p, (size_t)nblk * sizeof(double)));
     RCHK(hipMalloc(&dstats.p, 2 * sizeof(double)));
-    RCHK(hipMalloc(&dout.p, (size_t)n * sizeof(float)));
+    if (!out_dev) RCHK(hipMalloc(&dout.p, (size_t)n * sizeof(float)));
     double *r = (double *)dr.p;
     double2 *c = (double2 *)dc.p;
     double *part = (double *)dpart.p, *stats = (double *)dstats.p;
@@ -230,9 +235,10 @@ extern "C" int swpc_rmedia_3d(int device, int ptype, int n1, int n2, int n3, flo
     RCHK(hipStreamSynchronize(s));
     const double st[2] = {mean, std::sqrt(ss / (double)n)};
     RCHK(hipMemcpyAsync(stats, st, sizeof st, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_finish, dim3(nblk), dim3(256), 0, s, r, n, stats, (double)epsil, (float *)dout.p);
+    hipLaunchKernelGGL(k_finish, dim3(nblk), dim3(256), 0, s, r, n, stats, (double)epsil,
+                       out_dev ? out : (float *)dout.p);
     RCHK(hipGetLastError());
-    RCHK(hipMemcpyAsync(out, dout.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (!out_dev) RCHK(hipMemcpyAsync(out, dout.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
     RCHK(hipStreamSynchronize(s));
     return SWPC_OK;
 }

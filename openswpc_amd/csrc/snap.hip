@@ -7,6 +7,8 @@
 // device (frame f = (it-1)/ntdec_s), so graph-replayed step blocks need no
 // host round trip.  u sections integrate V dt every step and the fs/ob
 // sections keep running maximum maps, as the reference does.
+#include <cstring>
+
 #include "swpc_internal.h"
 
 struct SnapArgs {
@@ -28,15 +30,15 @@ struct SnapArgs {
 // (53 >= 2*24 + 2 bits), which the single-precision hardware root is not
 __device__ static inline float sqrt_rn(float x) { return (float)__dsqrt_rn((double)x); }
 
-__global__ void __launch_bounds__(256) k_snap(SnapArgs a)
+// block blk of one context's section launch at step it
+__device__ static inline void snap_body(const SnapArgs &a, int blk, int it)
 {
     int t = 0;
-    while (t + 1 < a.ntask && (int)blockIdx.x >= a.task[t + 1].blk0) ++t;
+    while (t + 1 < a.ntask && blk >= a.task[t + 1].blk0) ++t;
     const SnapTask T = a.task[t];
     const long long npt = (long long)T.ns1 * T.ns2;
-    const long long p = (long long)(blockIdx.x - T.blk0) * blockDim.x + threadIdx.x;
+    const long long p = (long long)(blk - T.blk0) * blockDim.x + threadIdx.x;
     if (p >= npt) return;
-    const int it = a.d_it ? *a.d_it : a.it;
     if (it < 1) return;
     const bool out = ((it - 1) % a.ntdec_s) == 0;
     const int f = (it - 1) / a.ntdec_s;
@@ -130,10 +132,19 @@ __global__ void __launch_bounds__(256) k_snap(SnapArgs a)
     }
 }
 
-hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+__global__ void __launch_bounds__(256) k_snap(SnapArgs a) { snap_body(a, blockIdx.x, a.d_it ? *a.d_it : a.it); }
+
+// batched: context m = blockIdx.y, its arguments in HBM; blocks beyond the
+// context's own count idle
+__global__ void __launch_bounds__(256) k_snap_b(const SnapArgs *__restrict__ args, int it, const int *d_it)
 {
-    if (c->snap_ntask <= 0) return hipSuccess;
-    SnapArgs a;
+    const SnapArgs &a = args[blockIdx.y];
+    if ((int)blockIdx.x >= a.nblk) return;
+    snap_body(a, blockIdx.x, d_it ? *d_it : it);
+}
+
+static void snap_fill(swpc_ctx *c, SnapArgs &a, int it, const int *d_it)
+{
     a.g = c->g;
     a.Vx = c->fields[SWPC_VX]; a.Vy = c->fields[SWPC_VY]; a.Vz = c->fields[SWPC_VZ];
     a.Syz = c->fields[SWPC_SYZ]; a.Sxz = c->fields[SWPC_SXZ]; a.Sxy = c->fields[SWPC_SXY];
@@ -151,6 +162,33 @@ hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
     a.M0 = d.M0; a.UC = d.UC; a.dt = c->desc.dt;
     a.it = it;
     a.d_it = d_it;
+}
+
+hipError_t launch_snap(swpc_ctx *c, int it, const int *d_it, hipStream_t s)
+{
+    if (c->snap_ntask <= 0) return hipSuccess;
+    SnapArgs a;
+    snap_fill(c, a, it, d_it);
     hipLaunchKernelGGL(k_snap, dim3(c->snap_nblk), dim3(256), 0, (s ? s : c->stream), a);
+    return hipGetLastError();
+}
+
+// batched sections (swpc_batch): one launch for every member with sections,
+// as many blocks as the member with the most section points
+size_t batch_snap_size() { return sizeof(SnapArgs); }
+
+int batch_snap_fill(swpc_ctx *c, void *out)
+{
+    if (c->snap_ntask <= 0) return 0;
+    SnapArgs a;
+    snap_fill(c, a, 0, nullptr);
+    memcpy(out, &a, sizeof(a));
+    return c->snap_nblk;
+}
+
+hipError_t launch_batch_snap(const void *d_args, int nmem, int maxblk, int it, const int *d_it, hipStream_t s)
+{
+    if (nmem <= 0 || maxblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_snap_b, dim3(maxblk, nmem), dim3(256), 0, s, (const SnapArgs *)d_args, it, d_it);
     return hipGetLastError();
 }

@@ -490,8 +490,50 @@ __global__ void __launch_bounds__(256) k_stress_int_b(const StepArgs *__restrict
     stress_int_body<NM, NT, false>(mem[t.x], t.y, t.z, t.w);
 }
 
-// kernel__update_vel, m_kernel.f90:75-140, then the bottom PML band
+// Every load of one plane of the interior velocity update: the z-queue
+// entries the plane adds, the x/y neighbours, the buoyancy densities and the
+// old velocities.  Loading the next plane's set before computing this one's
+// keeps two planes of loads in flight per wave.
+struct VelLoads {
+    float sxz_p1, syz_p1, szz_p2, rho_kp;
+    float sxx_im1, sxx_0, sxx_ip1, sxx_ip2;
+    float sxy_0, sxy_jm2, sxy_jm1, sxy_jp1, sxy_im2, sxy_im1, sxy_ip1;
+    float syy_jm1, syy_0, syy_jp1, syy_jp2;
+    float syz_jm2, syz_jm1, syz_jp1;
+    float sxz_im2, sxz_im1, sxz_ip1;
+    float rho_ip, rho_jp;
+    float vx, vy, vz;
+};
+
 template <bool OOP>
+__device__ __forceinline__ void vel_loads(const StepArgs &a, long long p, VelLoads &L)
+{
+    const long long P = a.g.plane;
+    const int Y = a.g.pitch;
+    const float *__restrict__ Sxx = a.f.Sxx;
+    const float *__restrict__ Syy = a.f.Syy;
+    const float *__restrict__ Szz = a.f.Szz;
+    const float *__restrict__ Syz = a.f.Syz;
+    const float *__restrict__ Sxz = a.f.Sxz;
+    const float *__restrict__ Sxy = a.f.Sxy;
+    const float *__restrict__ rho = a.m.rho;
+    L.sxz_p1 = Sxz[p + P]; L.syz_p1 = Syz[p + P]; L.szz_p2 = Szz[p + 2 * P]; L.rho_kp = rho[p + P];
+    L.sxx_im1 = Sxx[p - 1]; L.sxx_0 = Sxx[p]; L.sxx_ip1 = Sxx[p + 1]; L.sxx_ip2 = Sxx[p + 2];
+    L.sxy_0 = Sxy[p];
+    L.sxy_jm2 = Sxy[p - 2 * Y]; L.sxy_jm1 = Sxy[p - Y]; L.sxy_jp1 = Sxy[p + Y];
+    L.sxy_im2 = Sxy[p - 2]; L.sxy_im1 = Sxy[p - 1]; L.sxy_ip1 = Sxy[p + 1];
+    L.syy_jm1 = Syy[p - Y]; L.syy_0 = Syy[p]; L.syy_jp1 = Syy[p + Y]; L.syy_jp2 = Syy[p + 2 * Y];
+    L.syz_jm2 = Syz[p - 2 * Y]; L.syz_jm1 = Syz[p - Y]; L.syz_jp1 = Syz[p + Y];
+    L.sxz_im2 = Sxz[p - 2]; L.sxz_im1 = Sxz[p - 1]; L.sxz_ip1 = Sxz[p + 1];
+    L.rho_ip = rho[p + 1]; L.rho_jp = rho[p + Y];
+    L.vx = ld1<true>((OOP ? a.fin.Vx : a.f.Vx) + p);
+    L.vy = ld1<true>((OOP ? a.fin.Vy : a.f.Vy) + p);
+    L.vz = ld1<true>((OOP ? a.fin.Vz : a.f.Vz) + p);
+}
+
+// kernel__update_vel, m_kernel.f90:75-140, then the bottom PML band.
+// PF: the interior loop loads plane k+1 while it computes plane k.
+template <bool OOP, bool PF>
 __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy, int bz)
 {
     const Grid &g = a.g;
@@ -526,7 +568,46 @@ __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy,
     float szz_m1 = Szz[p - P], szz_0 = Szz[p], szz_p1 = Szz[p + P];
     float rho0 = rho[p];
 
-    if (ka1 >= k0) {
+    if (PF && ka1 >= k0) {
+        const long long c2 = g2idx(g, i, j);
+        const int kft = a.m.kfs_top[c2], kfb = a.m.kfs_bot[c2];
+        const int kot = a.m.kob_top[c2], kobb = a.m.kob_bot[c2];
+        VelLoads L;
+        vel_loads<OOP>(a, p, L);
+        for (int k = k0; k <= ka1; ++k, p += P) {
+            // plane k+1 <= nz + 1: its loads stay within the -2..nz+3 halo
+            VelLoads N;
+            if (k < ka1) vel_loads<OOP>(a, p + P, N);
+
+            const bool in = in_band(k, kft, kfb, kot, kobb);
+            const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
+            const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
+            const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
+            const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
+            const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
+            const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
+
+            const float d3Sx3 = (L.sxx_ip1 - L.sxx_0) * re40x - (L.sxx_ip2 - L.sxx_im1) * re41x
+                              + (L.sxy_0 - L.sxy_jm1) * re40y - (L.sxy_jp1 - L.sxy_jm2) * re41y
+                              + (sxz_0 - sxz_m1) * re40z - (L.sxz_p1 - sxz_m2) * re41z;
+            const float d3Sy3 = (L.sxy_0 - L.sxy_im1) * re40x - (L.sxy_ip1 - L.sxy_im2) * re41x
+                              + (L.syy_jp1 - L.syy_0) * re40y - (L.syy_jp2 - L.syy_jm1) * re41y
+                              + (syz_0 - syz_m1) * re40z - (L.syz_p1 - syz_m2) * re41z;
+            const float d3Sz3 = (sxz_0 - L.sxz_im1) * re40x - (L.sxz_ip1 - L.sxz_im2) * re41x
+                              + (syz_0 - L.syz_jm1) * re40y - (L.syz_jp1 - L.syz_jm2) * re41y
+                              + (szz_p1 - szz_0) * re40z - (L.szz_p2 - szz_m1) * re41z;
+
+            st1<true>(Vx + p, L.vx + 2.0f / (rho0 + L.rho_ip) * d3Sx3 * dt);
+            st1<true>(Vy + p, L.vy + 2.0f / (rho0 + L.rho_jp) * d3Sy3 * dt);
+            st1<true>(Vz + p, L.vz + 2.0f / (rho0 + L.rho_kp) * d3Sz3 * dt);
+
+            sxz_m2 = sxz_m1; sxz_m1 = sxz_0; sxz_0 = L.sxz_p1;
+            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = L.syz_p1;
+            szz_m1 = szz_0; szz_0 = szz_p1; szz_p1 = L.szz_p2;
+            rho0 = L.rho_kp;
+            L = N;
+        }
+    } else if (ka1 >= k0) {
         const long long c2 = g2idx(g, i, j);
         const int kft = a.m.kfs_top[c2], kfb = a.m.kfs_bot[c2];
         const int kot = a.m.kob_top[c2], kobb = a.m.kob_bot[c2];
@@ -590,18 +671,19 @@ __device__ __forceinline__ void vel_int_body(const StepArgs &a, int bx, int byy,
 
 // 4 waves per SIMD: the velocity update fits 128 VGPRs without spilling and
 // runs about 6 % faster than at the compiler's default of 3 waves
-template <bool OOP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vel_int(StepArgs a)
+template <bool OOP, bool PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PF ? 3 : 4))) k_vel_int(StepArgs a)
 {
     int bx, byy, bz;
     plain_tile(a, bx, byy, bz);
-    vel_int_body<OOP>(a, bx, byy, bz);
+    vel_int_body<OOP, PF>(a, bx, byy, bz);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vel_int_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+template <bool PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PF ? 3 : 4))) k_vel_int_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    vel_int_body<false>(mem[t.x], t.y, t.z, t.w);
+    vel_int_body<false, PF>(mem[t.x], t.y, t.z, t.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,8 +908,15 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
     step_args(c, a, mask);
     a.zc = c->d_zchunks + c->nbz + 1;
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
-    if (c->fields_in[0]) hipLaunchKernelGGL(k_vel_int<true>, dim3(a.nbx * a.nby * a.nbz), dim3(a.bxw, a.byi), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_vel_int<false>, dim3(a.nbx * a.nby * a.nbz), dim3(a.bxw, a.byi), 0, c->stream, a);
+    const dim3 grid(a.nbx * a.nby * a.nbz), block(a.bxw, a.byi);
+    const bool pf = (c->variant & 8) != 0;
+    if (c->fields_in[0]) {
+        if (pf) hipLaunchKernelGGL((k_vel_int<true, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_vel_int<true, false>), grid, block, 0, c->stream, a);
+    } else {
+        if (pf) hipLaunchKernelGGL((k_vel_int<false, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_vel_int<false, false>), grid, block, 0, c->stream, a);
+    }
     return hipGetLastError();
 }
 
@@ -861,11 +950,8 @@ struct CerjanArgs {
     int kend;
 };
 
-__global__ void __launch_bounds__(256) k_cerjan_stress(CerjanArgs a)
+__device__ static inline void cerjan_stress_cell(const CerjanArgs &a, int i, int j, int k)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y;
-    const int k = 1 + blockIdx.z;
     if (i > a.g.nx || k > a.kend) return;
     const long long p = gidx(a.g, i, j, k);
     const float gxc = a.gx_c[i + 2], gxb = a.gx_b[i + 2];
@@ -880,11 +966,13 @@ __global__ void __launch_bounds__(256) k_cerjan_stress(CerjanArgs a)
     a.f.Sxy[p] = a.f.Sxy[p] * gxb * gyb * gzc;
 }
 
-__global__ void __launch_bounds__(256) k_cerjan_vel(CerjanArgs a)
+__global__ void __launch_bounds__(256) k_cerjan_stress(CerjanArgs a)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y;
-    const int k = 1 + blockIdx.z;
+    cerjan_stress_cell(a, 1 + blockIdx.x * blockDim.x + threadIdx.x, 1 + blockIdx.y, 1 + blockIdx.z);
+}
+
+__device__ static inline void cerjan_vel_cell(const CerjanArgs &a, int i, int j, int k)
+{
     if (i > a.g.nx || k > a.kend) return;
     const long long p = gidx(a.g, i, j, k);
     const float gxc = a.gx_c[i + 2], gxb = a.gx_b[i + 2];
@@ -893,6 +981,33 @@ __global__ void __launch_bounds__(256) k_cerjan_vel(CerjanArgs a)
     a.f.Vx[p] = a.f.Vx[p] * gxb * gyc * gzc;
     a.f.Vy[p] = a.f.Vy[p] * gxc * gyb * gzc;
     a.f.Vz[p] = a.f.Vz[p] * gxc * gyc * gzb;
+}
+
+__global__ void __launch_bounds__(256) k_cerjan_vel(CerjanArgs a)
+{
+    cerjan_vel_cell(a, 1 + blockIdx.x * blockDim.x + threadIdx.x, 1 + blockIdx.y, 1 + blockIdx.z);
+}
+
+// batched sponges (swpc_batch): one flat launch over every member's blocks;
+// member m owns blocks blk0[m] .. blk0[m+1]-1, its (row chunk, j, k) blocks
+// in the order of its own launch
+__global__ void __launch_bounds__(256) k_cerjan_b(const CerjanArgs *__restrict__ args, const int *__restrict__ blk0,
+                                                  int nmem, int phase)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const CerjanArgs &a = args[lo];
+    const int lb = b - blk0[lo];
+    const int gx = (a.g.nx + 255) / 256;
+    const int t = lb / gx;
+    const int i = 1 + (lb % gx) * 256 + threadIdx.x, j = 1 + t % a.g.ny, k = 1 + t / a.g.ny;
+    if (phase == 0) cerjan_stress_cell(a, i, j, k);
+    else cerjan_vel_cell(a, i, j, k);
 }
 
 static void cerjan_fill(swpc_ctx *c, CerjanArgs &a, int kend)
@@ -923,6 +1038,25 @@ hipError_t launch_cerjan_vel(swpc_ctx *c, hipStream_t s)
     return hipGetLastError();
 }
 
+size_t batch_cerjan_size() { return sizeof(CerjanArgs); }
+
+int batch_cerjan_fill(swpc_ctx *c, int phase, void *out)
+{
+    if (c->desc.abc != SWPC_ABC_CERJAN) return 0;
+    CerjanArgs a;
+    const int kend = phase == 0 ? c->g.ke : c->g.nz;
+    cerjan_fill(c, a, kend);
+    *(CerjanArgs *)out = a;
+    return (c->g.nx + 255) / 256 * c->g.ny * kend;
+}
+
+hipError_t launch_batch_cerjan(int phase, const void *d_args, const int *d_blk0, int nmem, int nblocks, hipStream_t s)
+{
+    if (nmem <= 0 || nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cerjan_b, dim3(nblocks), dim3(256), 0, s, (const CerjanArgs *)d_args, d_blk0, nmem, phase);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // batched launches: the arguments of every member live in HBM, the tile
 // tables carry the member index (swpc_batch, batch.hip)
@@ -949,9 +1083,10 @@ void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz)
     nbz = a.nbz;
 }
 
-hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, int bxw, const void *d_args, const int4 *d_cols,
+hipError_t launch_batch_phase(int phase, int nm, int variant, int by, int bxw, const void *d_args, const int4 *d_cols,
                               int ncols, const int4 *d_side, int nside, hipStream_t s)
 {
+    const bool nt = (variant & 1) != 0;
     const StepArgs *mem = (const StepArgs *)d_args;
     const dim3 side(BX, by), block(bxw, by * (BX / bxw));
     if (nside > 0) {
@@ -962,7 +1097,8 @@ hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, int bxw, const
     }
     if (ncols <= 0) return hipSuccess;
     if (phase == 1) {
-        hipLaunchKernelGGL(k_vel_int_b, dim3(ncols), block, 0, s, mem, d_cols);
+        if (variant & 8) hipLaunchKernelGGL(k_vel_int_b<true>, dim3(ncols), block, 0, s, mem, d_cols);
+        else hipLaunchKernelGGL(k_vel_int_b<false>, dim3(ncols), block, 0, s, mem, d_cols);
         return hipGetLastError();
     }
 #define SWPC_BSTRESS_CASE(NMV)                                                                      \

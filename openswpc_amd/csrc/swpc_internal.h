@@ -193,10 +193,17 @@ struct swpc_ctx {
     int nbz;           // k chunks of the interior-column kernels
     int *d_zchunks;    // first plane of each chunk, stress phase then velocity phase (2 x (nbz+1))
     int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel,
-                       // bit 1: side-column PML kernels on a second stream
+                       // bit 1: side-column PML kernels on a second stream,
+                       // bit 2: 32-wide interior tiles, bit 3: velocity loads one plane ahead,
+                       // bit 4: adjoint records copied
     long long bytes;
     struct AdjWorkspace *adj_ws;  // adjoint state, gradient sums and checkpoints (adjoint.hip)
 };
+
+// records msg as the context's last error and returns code (api.hip)
+int ctx_fail(swpc_ctx *c, int code, const char *msg);
+// launch arguments changed: drops the captured graphs (api.hip)
+void ctx_reset_graph(swpc_ctx *c);
 
 // ---- launchers (stencil.hip / aux.hip) ----
 // mask: 1 interior, 2 PML, 3 both; the PML kernel goes to pml_stream when
@@ -228,13 +235,26 @@ hipError_t launch_counter_advance(int *d, hipStream_t s);
 size_t step_args_size();
 void fill_batch_args(swpc_ctx *c, void *out, int phase);
 void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz);
-// batched receivers and sources (aux.hip): kinds of per-member argument arrays
-enum { BATCH_AUX_WAV = 0, BATCH_AUX_GLUT = 1, BATCH_AUX_BODYFORCE = 2, BATCH_AUX_N = 3 };
+// batched receivers, sources, Green's-function stores and sources (aux.hip,
+// green.hip): kinds of per-member argument arrays
+enum { BATCH_AUX_WAV = 0, BATCH_AUX_GLUT = 1, BATCH_AUX_BODYFORCE = 2, BATCH_AUX_GREEN_STORE = 3,
+       BATCH_AUX_GREEN_SOURCE = 4, BATCH_AUX_N = 5 };
 size_t batch_aux_size(int kind);
 // fills one member's arguments of a kind; returns its thread count (0: nothing to launch)
 int batch_aux_fill(swpc_ctx *c, int kind, void *out);
 hipError_t launch_batch_aux(int kind, const void *d_args, int nmem, int nmax, int it, const int *d_it, hipStream_t s);
-hipError_t launch_batch_phase(int phase, int nm, bool nt, int by, int bxw, const void *d_args, const int4 *d_cols,
+size_t batch_green_size(int kind);
+int batch_green_fill(swpc_ctx *c, int kind, void *out);
+hipError_t launch_batch_green(int kind, const void *d_args, int nmem, int nmax, int it, const int *d_it, hipStream_t s);
+// batched sections (snap.hip): returns the member's block count (0: no sections)
+size_t batch_snap_size();
+int batch_snap_fill(swpc_ctx *c, void *out);
+hipError_t launch_batch_snap(const void *d_args, int nmem, int maxblk, int it, const int *d_it, hipStream_t s);
+// batched Cerjan sponges (stencil.hip): phase 0 stress, 1 velocity; returns the member's block count
+size_t batch_cerjan_size();
+int batch_cerjan_fill(swpc_ctx *c, int phase, void *out);
+hipError_t launch_batch_cerjan(int phase, const void *d_args, const int *d_blk0, int nmem, int nblocks, hipStream_t s);
+hipError_t launch_batch_phase(int phase, int nm, int variant, int by, int bxw, const void *d_args, const int4 *d_cols,
                               int ncols, const int4 *d_side, int nside, hipStream_t s);
 hipError_t launch_vmax(swpc_ctx *c);
 hipError_t launch_ref_to_dev(swpc_ctx *c, const float *src_ref, float *dst, hipStream_t s);

@@ -98,6 +98,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_get_snap_max.argtypes = [vp, i, fp]
     L.swpc_snap_dims.argtypes = [vp, i, ip]
     L.swpc_kernel_vmax.argtypes = [vp, fp]
+    L.swpc_medium_layered.argtypes = [vp, fp, vp, fp, fp, ip, ip, fp]
+    L.swpc_get_medium.argtypes = [vp, i, vp]
+    L.swpc_staging_buffer.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.swpc_staging_buffer.restype = vp
     L.swpc_get_field.argtypes = [vp, i, fp]
     L.swpc_set_field.argtypes = [vp, i, fp]
     L.swpc_get_seismograms.argtypes = [vp, i, fp]
@@ -113,7 +117,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_adj_accumulate.argtypes = [vp, i, fp, i, i, C.POINTER(C.c_double)]
     L.swpc_adj_get_gradient.argtypes = [vp, vp, vp, vp]
     L.swpc_adj_release.argtypes = [vp]
-    L.swpc_rmedia_3d.argtypes = [i, i, i, i, i, f, f, f, f, f, f, f, f, C.c_uint64, fp]
+    L.swpc_rmedia_3d.argtypes = [i, i, i, i, i, f, f, f, f, f, f, f, f, C.c_uint64, C.POINTER(C.c_float)]
     L.swpc_rmedia_last_error.argtypes = []
     L.swpc_rmedia_last_error.restype = C.c_char_p
     L.swpc_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
@@ -125,6 +129,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_batch_tune.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_float)]
     L.swpc_batch_stream.argtypes = [vp]
+    L.swpc_batch_set_packing.argtypes = [vp, i]
+    L.swpc_batch_get_packing.argtypes = [vp]
     L.swpc_batch_stream.restype = vp
     _lib = L
     return L
@@ -358,6 +364,53 @@ class SwpcGPU:
         self._chk(self.L.swpc_kernel_vmax(self.h, out))
         return out
 
+    MEDIUM = ("rho", "lam", "mu", "taup", "taus", "kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")
+
+    def get_medium(self, name):
+        """the device's rho/lam/mu/taup/taus ((k,i,j) halo layout) or band
+        and ocean-bottom maps ((i,j) halo layout)"""
+        w = self.MEDIUM.index(name)
+        if w < 5:
+            out = np.empty((self.ny + 6, self.nx + 6, self.nz + 6), np.float32)
+        else:
+            out = np.empty((self.ny + 6, self.nx + 6), np.int32)
+        self._chk(self.L.swpc_get_medium(self.h, w, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def staging_buffer(self):
+        """(device address, floats) of the context's staging buffer, one
+        (k,i,j) halo array long"""
+        n = C.c_int64()
+        ptr = self.L.swpc_staging_buffer(self.h, C.byref(n))
+        return int(ptr), int(n.value)
+
+    def medium_layered(self, prof, xi, clip, tau):
+        """medium__setup of a layered model on the device: prof (5, nz+6) =
+        rho, lam, mu, qp, qs per plane; xi None, a (k,i,j) halo numpy array or
+        the device address of one (an int, e.g. the staging buffer's); clip =
+        (vmin, vmax, rhomin) of the perturbation; tau =
+        (zeta, Re cc, Im cc).  Returns the kfs and kob maps ((ny+6, nx+6),
+        the reference's halo layout) and (vmin, vmax)."""
+        ny6, nx6, nz6 = self.ny + 6, self.nx + 6, self.nz + 6
+        prof = np.ascontiguousarray(prof, np.float32)
+        if prof.shape != (5, nz6):
+            raise ValueError(f"prof must be (5, {nz6})")
+        if xi is None:
+            xp = None
+        elif isinstance(xi, int):
+            xp = C.c_void_p(xi)          # a device address of one (k,i,j) halo array
+        else:
+            xi = np.ascontiguousarray(xi, np.float32)
+            if xi.shape != (ny6, nx6, nz6):
+                raise ValueError("xi: the (k,i,j) halo shape")
+            xp = xi.ctypes.data_as(C.c_void_p)
+        kfs = np.zeros((ny6, nx6), np.int32)
+        kob = np.zeros((ny6, nx6), np.int32)
+        vr = np.zeros(2, np.float32)
+        self._chk(self.L.swpc_medium_layered(self.h, prof, xp, np.asarray(clip, np.float32),
+                                             np.asarray(tau, np.float32), kfs, kob, vr))
+        return kfs, kob, vr
+
     # ---- data out / in ----
     def get_field(self, name):
         out = np.empty((self.ny + 6, self.nx + 6, self.nz + 6), np.float32)
@@ -409,11 +462,18 @@ class SwpcBatch:
         self._chk(self.L.swpc_batch_synchronize(self.h))
 
     def tune(self, trial_steps=3):
-        """(k chunk, tile rows, interior tile width, ms per trial step) of the fastest shape"""
+        """(k chunk, tile rows, interior tile width, ms per trial step) of the
+        fastest shape; the packing chosen is self.packing()"""
         kc, by, w, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
         self._chk(self.L.swpc_batch_tune(self.h, int(trial_steps), C.byref(kc), C.byref(by), C.byref(w),
                                          C.byref(ms)))
         return kc.value, by.value, w.value, ms.value
+
+    def set_packing(self, packing):
+        self._chk(self.L.swpc_batch_set_packing(self.h, int(packing)))
+
+    def packing(self):
+        return int(self.L.swpc_batch_get_packing(self.h))
 
     def stream(self):
         return self.L.swpc_batch_stream(self.h)
@@ -433,17 +493,23 @@ class SwpcBatch:
 RMEDIA_TYPES = {"gauss": 1, "exp": 2, "vonkarman": 3}
 
 
-def rmedia_3d(ptype, shape, d, a, epsil, kappa=0.5, seed=1, device=0):
+def rmedia_3d(ptype, shape, d, a, epsil, kappa=0.5, seed=1, device=0, out_ptr=None):
     """Random medium on the device (m_rmedia.f90 rmedia__3dgen_ani).
     shape = (n1, n2, n3) with axis 1 fastest, d and a per axis; returns a
-    float32 numpy array of shape (n3, n2, n1).  ptype: 1/2/3 or
-    'gauss'/'exp'/'vonkarman'."""
+    float32 numpy array of shape (n3, n2, n1), or, given out_ptr (the device
+    address of n1*n2*n3 floats on `device`), writes the field there and
+    returns None.  ptype: 1/2/3 or 'gauss'/'exp'/'vonkarman'."""
     L = load_library()
     pt = RMEDIA_TYPES.get(ptype, ptype)
     n1, n2, n3 = (int(v) for v in shape)
-    out = np.empty((n3, n2, n1), np.float32)
+    if out_ptr is not None:
+        out = None
+        ptr = C.cast(C.c_void_p(int(out_ptr)), C.POINTER(C.c_float))
+    else:
+        out = np.empty((n3, n2, n1), np.float32)
+        ptr = out.ctypes.data_as(C.POINTER(C.c_float))
     rc = L.swpc_rmedia_3d(int(device), int(pt), n1, n2, n3, *(float(v) for v in d), *(float(v) for v in a),
-                          float(epsil), float(kappa), int(seed) & (2**64 - 1), out)
+                          float(epsil), float(kappa), int(seed) & (2**64 - 1), ptr)
     if rc != 0:
         raise SwpcError(f"swpc_rmedia_3d failed ({rc}): {L.swpc_rmedia_last_error().decode()}")
     return out

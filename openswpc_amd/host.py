@@ -46,6 +46,8 @@ def host_lib() -> C.CDLL:
     L.swpch_stable_dt.restype = f
     L.swpch_stable_dt.argtypes = [f, f, f, f]
     L.swpch_vmodel_layers.argtypes = [i, i, i, i, f, f, f, i, i, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_int)]
+    L.swpch_vmodel_profile.argtypes = [i, i, f, f, f, i, i, fp, fp, C.POINTER(C.c_int)]
+    L.swpch_visco_setup.argtypes = [i, f, f, f, fp, C.POINTER(C.c_float), fp]
     L.swpch_seawater_vel.restype = f
     L.swpch_seawater_vel.argtypes = [f, i]
     L.swpch_perturb.argtypes = [i, i, i, fp, f, f, f, fp, fp, fp]
@@ -131,6 +133,9 @@ class ModelSpec:
     vcut: float = 0.0
     munk: bool = False            # seawater sound speed: Munk's profile instead of 1.5 km/s
     xi: np.ndarray | None = None  # optional relative perturbation, (k,i,j) halo layout
+    # or a random medium generated on the device (m_rmedia.f90 method, openswpc_amd.gpu.rmedia_3d):
+    # dict(ptype='gauss'|'exp'|'vonkarman', a=(ax, ay, az) km, epsil, kappa, seed)
+    rmedia: dict | None = None
     sources: list = field(default_factory=list)
     bf_mode: bool = False
     stftype: str = "kupper"
@@ -216,6 +221,34 @@ def _station_inputs(spec: ModelSpec, kfs: np.ndarray, kob: np.ndarray) -> dict:
     return dict(ist=np.zeros(0, np.int32), jst=np.zeros(0, np.int32), kst=np.zeros(0, np.int32))
 
 
+def _rmedia_args(spec: ModelSpec):
+    """arguments of gpu.rmedia_3d for spec.rmedia: the (k,i,j) halo grid, z fastest"""
+    r = spec.rmedia
+    ax, ay, az = r["a"]
+    return (r["ptype"], (spec.nz + 6, spec.nx + 6, spec.ny + 6), (spec.dz, spec.dx, spec.dy), (az, ax, ay),
+            r["epsil"]), dict(kappa=r.get("kappa", 0.5), seed=r["seed"])
+
+
+def _perturbation(spec: ModelSpec):
+    """the relative perturbation of spec as a host array, or None"""
+    if spec.xi is not None and spec.rmedia is not None:
+        raise ValueError("ModelSpec: give xi or rmedia, not both")
+    if spec.rmedia is not None:
+        from .gpu import rmedia_3d
+        a, kw = _rmedia_args(spec)
+        return rmedia_3d(*a, **kw)
+    return spec.xi
+
+
+def _perturb_clip(spec: ModelSpec):
+    """vmin, vmax and rhomin of the random-medium perturbation: the velocity
+    cut-off, the velocity the 4th-order stability condition allows at dt, 1"""
+    f32 = np.float32
+    dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
+    vmax_cfl = (f32(6) / f32(7)) * f32(1) / np.sqrt(1 / dx32**2 + 1 / dy32**2 + 1 / dz32**2) / f32(spec.dt)
+    return f32(spec.vcut), f32(vmax_cfl), f32(1.0)
+
+
 def build_inputs(spec: ModelSpec) -> Inputs:
     L = host_lib()
     nx, ny, nz, na, nm = spec.nx, spec.ny, spec.nz, spec.na, spec.nm
@@ -224,22 +257,31 @@ def build_inputs(spec: ModelSpec) -> Inputs:
     layers = np.ascontiguousarray(np.asarray(spec.layers, np.float32).reshape(-1))
     rc = C.c_int()
     kind = {"lhm": 0, "lgm": 1}[spec.vmodel]
-    f32 = np.float32
     L.swpch_vmodel_layers(kind, nx, ny, nz, spec.zbeg, spec.dz, spec.vcut, int(spec.munk), len(spec.layers), layers,
                           rho, lam, mu, taup, taus, C.byref(rc))
     if rc.value != 0:
         raise ValueError(f"velocity model rejected (rc={rc.value})")
-    if spec.xi is not None:
-        dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
-        vmax_cfl = (f32(6) / f32(7)) * f32(1) / np.sqrt(1 / dx32**2 + 1 / dy32**2 + 1 / dz32**2) / f32(spec.dt)
-        L.swpch_perturb(nx, ny, nz, np.ascontiguousarray(spec.xi, np.float32), f32(spec.vcut), f32(vmax_cfl),
-                        f32(1.0), rho, lam, mu)
+    xi = _perturbation(spec)
+    if xi is not None:
+        L.swpch_perturb(nx, ny, nz, np.ascontiguousarray(xi, np.float32), *_perturb_clip(spec), rho, lam, mu)
+        del xi
     ts = np.zeros(max(nm, 1), np.float32)
     zeta = C.c_float()
     kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot = (np.zeros((ny + 6, nx + 6), np.int32) for _ in range(6))
     vmin, vmax = C.c_float(), C.c_float()
     L.swpch_medium_setup(nx, ny, nz, na, nm, spec.fq_min, spec.fq_max, spec.fq_ref, rho, lam, mu, taup, taus, ts,
                          C.byref(zeta), kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot, C.byref(vmin), C.byref(vmax))
+    medium = dict(rho=rho, lam=lam, mu=mu, taup=taup, taus=taus, kfs_top=kfs_top, kfs_bot=kfs_bot,
+                  kob_top=kob_top, kob_bot=kob_bot)
+    return _complete_inputs(spec, medium, kfs, kob, ts, float(zeta.value), float(vmin.value), float(vmax.value))
+
+
+def _complete_inputs(spec: ModelSpec, medium: dict, kfs, kob, ts, zeta: float, vmin: float, vmax: float) -> Inputs:
+    """the inputs after the medium: kernel coefficients (kernel__setup), sources,
+    absorber profiles, stations and Green's-function points"""
+    L = host_lib()
+    nx, ny, nz, na, nm = spec.nx, spec.ny, spec.nz, spec.na, spec.nm
+    f32 = np.float32
     c1, c2, d1 = (np.zeros(max(nm, 1), np.float32) for _ in range(3))
     d2 = C.c_float()
     L.swpch_kernel_coef(nm, ts, spec.dt, c1, c2, d1, C.byref(d2))
@@ -250,9 +292,7 @@ def build_inputs(spec: ModelSpec) -> Inputs:
     dx32, dy32, dz32 = f32(spec.dx), f32(spec.dy), f32(spec.dz)
 
     # absorbers
-    arrays = dict(rho=rho, lam=lam, mu=mu, taup=taup, taus=taus, kfs=kfs, kob=kob, kfs_top=kfs_top,
-                  kfs_bot=kfs_bot, kob_top=kob_top, kob_bot=kob_bot, ts=ts[:nm].copy(), c1=c1, c2=c2, d1=d1,
-                  **sa)
+    arrays = dict(**medium, kfs=kfs, kob=kob, ts=ts[:nm].copy(), c1=c1, c2=c2, d1=d1, **sa)
     if spec.abc == "pml":
         for ax, n, beg, h in (("x", nx, spec.xbeg, dx32), ("y", ny, spec.ybeg, dy32), ("z", nz, spec.zbeg, dz32)):
             gc = np.zeros(4 * n, np.float32)
@@ -270,7 +310,7 @@ def build_inputs(spec: ModelSpec) -> Inputs:
 
     arrays.update(_station_inputs(spec, kfs, kob))
     ntw = int(math.floor(float(f32(spec.nt - 1) / f32(spec.ntdec_w) + f32(1.0))))
-    scalars = dict(zeta=float(zeta.value), d2=float(d2.value), vmin=float(vmin.value), vmax=float(vmax.value),
+    scalars = dict(zeta=zeta, d2=float(d2.value), vmin=vmin, vmax=vmax,
                    fcut=float(fcut), M0=float(M0), UC=float(UC), ntw=ntw, nsrc=nsrc)
     if spec.green is not None:
         _green_inputs(spec, arrays, scalars, kob)
@@ -315,10 +355,59 @@ def _green_inputs(spec: ModelSpec, arrays: dict, scalars: dict, kob: np.ndarray)
 def make_simulation(inp: Inputs, device: int = 0):
     """Create a SwpcGPU context loaded with the inputs of ``build_inputs``."""
     from .gpu import SwpcGPU
-    s, a, sc = inp.spec, inp.arrays, inp.scalars
+    s, a = inp.spec, inp.arrays
     g = SwpcGPU(s.nx, s.ny, s.nz, s.na, s.nm, s.abc, s.dx, s.dy, s.dz, s.dt, device=device)
     g.set_medium(a["rho"], a["lam"], a["mu"], a["taup"], a["taus"], a["kfs_top"], a["kfs_bot"], a["kob_top"],
                  a["kob_bot"], a["kob"])
+    return _load_rest(g, inp)
+
+
+def make_simulation_device(spec: ModelSpec, device: int = 0):
+    """build_inputs and make_simulation with the medium built on the device
+    (swpc_medium_layered): the host library gives the per-plane profile of
+    the layered model and the scalars of the tau method, and the 3D medium
+    arrays never exist on the host.  A spec.rmedia field is generated into
+    the context's staging buffer on the device; a spec.xi array is uploaded
+    through it.  Returns (simulation, inputs); inputs.arrays holds everything
+    build_inputs gives but the 3D medium and the band maps."""
+    from .gpu import SwpcGPU
+    L = host_lib()
+    nx, ny, nz, nm = spec.nx, spec.ny, spec.nz, spec.nm
+    prof = np.zeros((5, nz + 6), np.float32)
+    layers = np.ascontiguousarray(np.asarray(spec.layers, np.float32).reshape(-1))
+    rc = C.c_int()
+    kind = {"lhm": 0, "lgm": 1}[spec.vmodel]
+    L.swpch_vmodel_profile(kind, nz, spec.zbeg, spec.dz, spec.vcut, int(spec.munk), len(spec.layers), layers, prof,
+                           C.byref(rc))
+    if rc.value != 0:
+        raise ValueError(f"velocity model rejected (rc={rc.value})")
+    ts = np.zeros(max(nm, 1), np.float32)
+    zeta = C.c_float()
+    ccv = np.zeros(2, np.float32)
+    L.swpch_visco_setup(nm, spec.fq_min, spec.fq_max, spec.fq_ref, ts, C.byref(zeta), ccv)
+    if spec.xi is not None and spec.rmedia is not None:
+        raise ValueError("ModelSpec: give xi or rmedia, not both")
+    g = SwpcGPU(nx, ny, nz, spec.na, nm, spec.abc, spec.dx, spec.dy, spec.dz, spec.dt, device=device)
+    try:
+        xi = spec.xi
+        if spec.rmedia is not None:
+            from .gpu import rmedia_3d
+            ptr, n = g.staging_buffer()
+            a, kw = _rmedia_args(spec)
+            rmedia_3d(*a, **kw, device=device, out_ptr=ptr)
+            xi = ptr
+        kfs, kob, vr = g.medium_layered(prof, xi, _perturb_clip(spec), (zeta.value, ccv[0], ccv[1]))
+        inp = _complete_inputs(spec, {}, kfs, kob, ts, float(zeta.value), float(vr[0]), float(vr[1]))
+        _load_rest(g, inp)
+    except Exception:
+        g.close()
+        raise
+    return g, inp
+
+
+def _load_rest(g, inp: Inputs):
+    """everything but the medium into a context"""
+    s, a, sc = inp.spec, inp.arrays, inp.scalars
     g.set_visco(a["c1"], a["c2"], a["d1"], sc["d2"])
     if s.abc == "pml":
         g.set_pml(a["gxc"], a["gxe"], a["gyc"], a["gye"], a["gzc"], a["gze"])

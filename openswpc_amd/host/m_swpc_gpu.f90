@@ -16,6 +16,9 @@ module m_swpc_gpu
     integer(c_int), parameter :: SWPC_NTIMER = 10
     integer(c_int), parameter :: SWPC_SNP_XY = 0, SWPC_SNP_YZ = 1, SWPC_SNP_XZ = 2, SWPC_SNP_FS = 3, SWPC_SNP_OB = 4
     integer(c_int), parameter :: SWPC_SNP_PS = 0, SWPC_SNP_V = 1, SWPC_SNP_U = 2, SWPC_SNP_NSECTION = 15
+    integer(c_int), parameter :: SWPC_MED_RHO = 0, SWPC_MED_LAM = 1, SWPC_MED_MU = 2, SWPC_MED_TAUP = 3, &
+                                 SWPC_MED_TAUS = 4, SWPC_MED_KFS_TOP = 5, SWPC_MED_KFS_BOT = 6, SWPC_MED_KOB_TOP = 7, &
+                                 SWPC_MED_KOB_BOT = 8, SWPC_MED_KOB = 9
 
     type, bind(C) :: swpc_desc
         integer(c_int) :: nx, ny, nz, na, nm, abc
@@ -53,6 +56,29 @@ module m_swpc_gpu
             type(c_ptr), value :: ctx
             real(c_float), intent(in) :: rho(*), lam(*), mu(*), taup(*), taus(*)
             integer(c_int), intent(in) :: kfs_top(*), kfs_bot(*), kob_top(*), kob_bot(*), kob(*)
+        end function
+        !! medium__setup of a layered model built on the device: prof(-2:nz+3, 5)
+        !! = rho, lam, mu, Qp, Qs per plane (swpch_vmodel_profile), xi = the
+        !! perturbation's address (c_loc of a (k,i,j) array, the staging buffer or
+        !! c_null_ptr), clip = vmin, vmax, rhomin, tau = zeta, Re cc, Im cc
+        !! (swpch_visco_setup); returns the kfs/kob maps and vmin, vmax
+        integer(c_int) function swpc_medium_layered(ctx, prof, xi, clip, tau, kfs, kob, vrange) &
+            bind(C, name='swpc_medium_layered')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx, xi
+            real(c_float), intent(in) :: prof(*), clip(3), tau(3)
+            integer(c_int), intent(out) :: kfs(*), kob(*)
+            real(c_float), intent(out) :: vrange(2)
+        end function
+        integer(c_int) function swpc_get_medium(ctx, which, out) bind(C, name='swpc_get_medium')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, out
+            integer(c_int), value :: which
+        end function
+        type(c_ptr) function swpc_staging_buffer(ctx, nfloats) bind(C, name='swpc_staging_buffer')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(out) :: nfloats
         end function
         integer(c_int) function swpc_set_visco(ctx, c1, c2, d1, d2) bind(C, name='swpc_set_visco')
             import :: c_int, c_ptr, c_float
@@ -290,6 +316,15 @@ module m_swpc_gpu
             integer(c_int), value :: trial_steps
             integer(c_int), intent(out) :: kchunk, by, width
             real(c_float), intent(out) :: ms
+        end function
+        integer(c_int) function swpc_batch_set_packing(batch, packing) bind(C, name='swpc_batch_set_packing')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: packing
+        end function
+        integer(c_int) function swpc_batch_get_packing(batch) bind(C, name='swpc_batch_get_packing')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
         end function
         type(c_ptr) function swpc_batch_stream(batch) bind(C, name='swpc_batch_stream')
             import :: c_ptr

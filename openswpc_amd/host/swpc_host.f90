@@ -21,6 +21,7 @@ module swpc_host
     real(SP), parameter :: EPSL = epsilon(1.0_SP)
 
     public :: swpch_i2x, swpch_x2i, swpch_seawater_vel, swpch_vmodel_layers, swpch_perturb, swpch_medium_setup
+    public :: swpch_vmodel_profile, swpch_visco_setup
     public :: swpch_kernel_coef, swpch_pml_profile, swpch_cerjan_profile, swpch_sdr2moment
     public :: swpch_momentrate, swpch_stf_table, swpch_station_index, swpch_stable_dt
 
@@ -60,20 +61,19 @@ contains
         swpch_seawater_vel = real(1.5_DP * (1.0_DP + e * (zb - 1.0_DP + exp(-zb))), SP)
     end function swpch_seawater_vel
 
-    !! Layered medium on the halo grid: kind 0 = homogeneous layers
-    !! (vmodel_lhm, src/swpc_3d/m_vmodel_lhm.f90:20-158), kind 1 = linear
+    !! Layered medium as one value per plane k = -2..nz+3: kind 0 = homogeneous
+    !! layers (vmodel_lhm, src/swpc_3d/m_vmodel_lhm.f90:20-158), kind 1 = linear
     !! gradient between layer tops (vmodel_lgm, m_vmodel_lgm.f90:20-174).
     !! layers(6,nl) = depth, rho, vp, vs, qp, qs.  Above depth(1): air where
     !! z < 0, seawater (rho 1, mu 0, Q 1e6, sound speed swpch_seawater_vel)
-    !! where 0 <= z (m_vmodel_lhm.f90:111-133).
-    subroutine swpch_vmodel_layers(kind, nx, ny, nz, zbeg, dz, vcut, munk, nl, layers, rho, lam, mu, qp, qs, rc) &
-        bind(C, name='swpch_vmodel_layers')
-        integer(c_int), value :: kind, nx, ny, nz, nl, munk
+    !! where 0 <= z (m_vmodel_lhm.f90:111-133).  prof(:,1..5) = rho, lam, mu,
+    !! qp, qs of each plane.
+    subroutine swpch_vmodel_profile(kind, nz, zbeg, dz, vcut, munk, nl, layers, prof, rc) &
+        bind(C, name='swpch_vmodel_profile')
+        integer(c_int), value :: kind, nz, nl, munk
         real(c_float), value :: zbeg, dz, vcut
         real(c_float), intent(in) :: layers(6, nl)
-        real(c_float), intent(out) :: rho(-2:nz + 3, -2:nx + 3, -2:ny + 3), lam(-2:nz + 3, -2:nx + 3, -2:ny + 3)
-        real(c_float), intent(out) :: mu(-2:nz + 3, -2:nx + 3, -2:ny + 3), qp(-2:nz + 3, -2:nx + 3, -2:ny + 3)
-        real(c_float), intent(out) :: qs(-2:nz + 3, -2:nx + 3, -2:ny + 3)
+        real(c_float), intent(out) :: prof(-2:nz + 3, 5)
         integer(c_int), intent(out) :: rc
         real(SP) :: dep(nl), r0(nl), vp0(nl), vs0(nl), q0p(nl), q0s(nl)
         real(SP) :: z, rho1, vp1, vs1, qp1, qs1, h, d
@@ -123,11 +123,35 @@ contains
                     end if
                 end do
             end if
-            rho(k, :, :) = rho1
-            mu(k, :, :) = rho1 * vs1 * vs1
-            lam(k, :, :) = rho1 * (vp1 * vp1 - 2 * vs1 * vs1)
-            qp(k, :, :) = qp1
-            qs(k, :, :) = qs1
+            prof(k, 1) = rho1
+            prof(k, 2) = rho1 * (vp1 * vp1 - 2 * vs1 * vs1)
+            prof(k, 3) = rho1 * vs1 * vs1
+            prof(k, 4) = qp1
+            prof(k, 5) = qs1
+        end do
+    end subroutine swpch_vmodel_profile
+
+    !! The layered medium of swpch_vmodel_profile on the halo grid.
+    subroutine swpch_vmodel_layers(kind, nx, ny, nz, zbeg, dz, vcut, munk, nl, layers, rho, lam, mu, qp, qs, rc) &
+        bind(C, name='swpch_vmodel_layers')
+        integer(c_int), value :: kind, nx, ny, nz, nl, munk
+        real(c_float), value :: zbeg, dz, vcut
+        real(c_float), intent(in) :: layers(6, nl)
+        real(c_float), intent(out) :: rho(-2:nz + 3, -2:nx + 3, -2:ny + 3), lam(-2:nz + 3, -2:nx + 3, -2:ny + 3)
+        real(c_float), intent(out) :: mu(-2:nz + 3, -2:nx + 3, -2:ny + 3), qp(-2:nz + 3, -2:nx + 3, -2:ny + 3)
+        real(c_float), intent(out) :: qs(-2:nz + 3, -2:nx + 3, -2:ny + 3)
+        integer(c_int), intent(out) :: rc
+        real(c_float) :: prof(-2:nz + 3, 5)
+        integer :: k
+
+        call swpch_vmodel_profile(kind, nz, zbeg, dz, vcut, munk, nl, layers, prof, rc)
+        if (rc /= 0) return
+        do k = -2, nz + 3
+            rho(k, :, :) = prof(k, 1)
+            lam(k, :, :) = prof(k, 2)
+            mu(k, :, :) = prof(k, 3)
+            qp(k, :, :) = prof(k, 4)
+            qs(k, :, :) = prof(k, 5)
         end do
     end subroutine swpch_vmodel_layers
 
@@ -195,8 +219,8 @@ contains
         integer(c_int), intent(out) :: kfs_top(-2:nx + 3, -2:ny + 3), kfs_bot(-2:nx + 3, -2:ny + 3)
         integer(c_int), intent(out) :: kob_top(-2:nx + 3, -2:ny + 3), kob_bot(-2:nx + 3, -2:ny + 3)
         real(c_float), intent(out) :: vmin, vmax
-        integer :: i, j, k, m
-        real(SP) :: om_a, om_b, omega, rb2, ra2, chi_mu, chi_lam, vp, vs
+        integer :: i, j, k
+        real(SP) :: rb2, ra2, chi_mu, chi_lam, vp, vs, ccv(2)
         complex(SP) :: cc
 
         call homogenize(rho)
@@ -205,18 +229,8 @@ contains
         call homogenize(taup)
         call homogenize(taus)
 
-        ! relaxation times (src/shared/m_fdtool.f90:691-728) and zeta (:756-812)
-        om_a = real(2 * PI * fq_min, SP)
-        om_b = real(2 * PI * fq_max, SP)
-        if (nm == 1) then
-            ts(1) = 1.0_SP / sqrt(om_a * om_b)
-        else if (nm > 1) then
-            do m = 1, nm
-                omega = real(om_a * real(om_b / om_a, DP)**(real(m - 1, DP) / real(nm - 1, DP)), SP)
-                ts(m) = 1.0_SP / omega
-            end do
-        end if
-        zeta = constq_zeta(nm, om_a, om_b, ts)
+        call swpch_visco_setup(nm, fq_min, fq_max, fq_ref, ts, zeta, ccv)
+        cc = cmplx(ccv(1), ccv(2), SP)
 
         !$omp parallel do private(i, k)
         do j = -2, ny + 3
@@ -231,13 +245,6 @@ contains
 
         ! relaxed moduli at the reference frequency (m_medium.f90:231-271)
         if (nm > 0) then
-            omega = real(2 * PI * fq_ref, SP)
-            cc = (0.0_SP, 0.0_SP)
-            do m = 1, nm
-                cc = cc + cmplx((cmplx(0.0_DP, 1.0_DP, DP) * omega * ts(m)) / (1.0_DP - cmplx(0.0_DP, 1.0_DP, DP) &
-                                * omega * ts(m)), kind=SP)
-            end do
-            cc = cc / nm
             !$omp parallel do private(i, k, rb2, ra2, chi_mu, chi_lam)
             do j = -2, ny + 3
                 do i = -2, nx + 3
@@ -317,6 +324,43 @@ contains
         end subroutine homogenize
 
     end subroutine swpch_medium_setup
+
+    !! The scalars of the tau method (src/shared/m_fdtool.f90:691-812): the nm
+    !! relaxation times ts log-spaced over [1/(2 pi fq_max), 1/(2 pi fq_min)],
+    !! zeta, and cc = mean_m i w ts_m / (1 - i w ts_m) at w = 2 pi fq_ref
+    !! (real and imaginary part), which relaxes the moduli
+    !! (src/swpc_3d/m_medium.f90:231-271).
+    subroutine swpch_visco_setup(nm, fq_min, fq_max, fq_ref, ts, zeta, ccv) bind(C, name='swpch_visco_setup')
+        integer(c_int), value :: nm
+        real(c_float), value :: fq_min, fq_max, fq_ref
+        real(c_float), intent(out) :: ts(*), zeta, ccv(2)
+        real(SP) :: om_a, om_b, omega
+        complex(SP) :: cc
+        integer :: m
+
+        om_a = real(2 * PI * fq_min, SP)
+        om_b = real(2 * PI * fq_max, SP)
+        if (nm == 1) then
+            ts(1) = 1.0_SP / sqrt(om_a * om_b)
+        else if (nm > 1) then
+            do m = 1, nm
+                omega = real(om_a * real(om_b / om_a, DP)**(real(m - 1, DP) / real(nm - 1, DP)), SP)
+                ts(m) = 1.0_SP / omega
+            end do
+        end if
+        zeta = constq_zeta(nm, om_a, om_b, ts)
+        cc = (0.0_SP, 0.0_SP)
+        if (nm > 0) then
+            omega = real(2 * PI * fq_ref, SP)
+            do m = 1, nm
+                cc = cc + cmplx((cmplx(0.0_DP, 1.0_DP, DP) * omega * ts(m)) / (1.0_DP - cmplx(0.0_DP, 1.0_DP, DP) &
+                                * omega * ts(m)), kind=SP)
+            end do
+            cc = cc / nm
+        end if
+        ccv(1) = real(cc)
+        ccv(2) = aimag(cc)
+    end subroutine swpch_visco_setup
 
     !! Blanch's tau-method zeta for a constant Q over [om_a, om_b]
     !! (src/shared/m_fdtool.f90:756-812)

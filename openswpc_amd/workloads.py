@@ -6,7 +6,6 @@ from __future__ import annotations
 
 import math
 import sys
-import time
 
 import numpy as np
 
@@ -17,7 +16,6 @@ def log(*a):
 
 def production_spec(nx=1024, ny=1024, nz=512, seed_medium=3, seed_st=4, seed_src=5, with_xi=True):
     from .host import ModelSpec
-    from .gpu import rmedia_3d
     dx = 0.25
     vs0, vs1 = 2.0, 4.5
     layers = [[0.0, 2.3, vs0 * math.sqrt(3), vs0, 100.0 * vs0, 50.0 * vs0],
@@ -32,15 +30,10 @@ def production_spec(nx=1024, ny=1024, nz=512, seed_medium=3, seed_st=4, seed_src
     rs = np.random.default_rng(seed_st)
     half = nx * dx / 2 - 30 * dx
     st = [(float(x), float(y), 0.0, "fsb") for x, y in rs.uniform(-half, half, size=(256, 2))]
-    xi = None
-    if with_xi:
-        t = time.perf_counter()
-        # (k,i,j) layout: z fastest, then x, then y (device generator, m_rmedia.f90 method)
-        xi = rmedia_3d("vonkarman", (nz + 6, nx + 6, ny + 6), (dx, dx, dx), (5.0, 5.0, 5.0), 0.05, 0.5,
-                       seed=seed_medium)
-        log(f"[bench] von Karman field {xi.shape} in {time.perf_counter() - t:.1f}s std={xi.std():.4f}")
+    # von Karman medium on the (k,i,j) halo grid, generated on the device when the medium is built
+    rmedia = dict(ptype="vonkarman", a=(5.0, 5.0, 5.0), epsil=0.05, kappa=0.5, seed=seed_medium) if with_xi else None
     return ModelSpec(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx, dt=dt, nt=3000, na=20, nm=3, zbeg=-1.0,
-                     vmodel="lgm", layers=layers, xi=xi, sources=src, stations=st, ntdec_w=1, wav_what=1,
+                     vmodel="lgm", layers=layers, rmedia=rmedia, sources=src, stations=st, ntdec_w=1, wav_what=1,
                      fq_min=0.05, fq_max=5.0, fq_ref=1.0)
 
 

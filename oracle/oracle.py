@@ -39,13 +39,18 @@ def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_LIBS: dict[int, C.CDLL] = {}
+_LIBS: dict = {}
 
 
-def lib(prec: int = 32) -> C.CDLL:
-    if prec in _LIBS:
-        return _LIBS[prec]
-    path = os.path.join(BUILD, f"liborc{prec}.so")
+def lib(prec: int = 32, vectorised: bool = False) -> C.CDLL:
+    """the oracle library of a precision variant; vectorised: the FP32
+    variant built for AVX2 hosts (same arithmetic, for timing)"""
+    key = (prec, vectorised)
+    if key in _LIBS:
+        return _LIBS[key]
+    if vectorised and prec != 32:
+        raise ValueError("the vectorised build is FP32 only")
+    path = os.path.join(BUILD, f"liborc{prec}{'v' if vectorised else ''}.so")
     if not os.path.exists(path):
         build_oracle()
     L = C.CDLL(path)
@@ -95,7 +100,7 @@ def lib(prec: int = 32) -> C.CDLL:
     L.orc_green_store.argtypes = [vp, i, i, _i32p, _i32p, _i32p, i, i, i, vp, vp]
     L.orc_green_source.argtypes = [vp, i, i, i, f, f, f, f]
     L.orc_prec.restype = i
-    _LIBS[prec] = L
+    _LIBS[key] = L
     return L
 
 
@@ -398,10 +403,10 @@ def kernel_box(cfg: Config):
 class OracleRun:
     """Owns the state arrays of one oracle integration."""
 
-    def __init__(self, model: Model, prec: int = 32):
+    def __init__(self, model: Model, prec: int = 32, vectorised: bool = False):
         self.m = model
         self.prec = prec
-        self.L = lib(prec)
+        self.L = lib(prec, vectorised)
         cfg = model.cfg
         mp = np.float32 if prec == 32 else np.float64
         sp = np.float64 if prec == 64 else np.float32

@@ -87,9 +87,18 @@ def test_null_context_is_rejected():
     assert L.swpc_set_green(None, 1, z, z, z, 1, 1, 1, 0, 0, 1, 1, 1, np.zeros(1, np.float32)) == -1
     assert L.swpc_get_green(None, np.zeros(1, np.float32)) == -1
     # argument checks of the context-free generator happen before any device call
-    assert L.swpc_rmedia_3d(0, 4, 8, 8, 8, 1, 1, 1, 1, 1, 1, 0.1, 0.5, 1, np.zeros(512, np.float32)) == -1
+    buf = np.zeros(512, np.float32)
+    assert L.swpc_rmedia_3d(0, 4, 8, 8, 8, 1, 1, 1, 1, 1, 1, 0.1, 0.5, 1,
+                            buf.ctypes.data_as(C.POINTER(C.c_float))) == -1
     assert b"bad arguments" in L.swpc_rmedia_last_error()
     assert L.swpc_step(None, 1, 1, 0, 1) == -1
+    assert L.swpc_get_medium(None, 0, None) == -1
+    n = C.c_int64(7)
+    assert L.swpc_staging_buffer(None, C.byref(n)) is None and n.value == 0
+    assert L.swpc_batch_set_packing(None, 1) == -1 and L.swpc_batch_get_packing(None) == -1
+    f2 = np.zeros(2, np.float32)
+    i1 = np.zeros(1, np.int32)
+    assert L.swpc_medium_layered(None, f2, None, f2, f2, i1, i1, f2) == -1
     for name in ("swpc_adj_reset", "swpc_adj_release"):
         assert getattr(L, name)(None) == -1, name
     assert L.swpc_adj_plan(None, 10, 0, None, None, None, None) == -1

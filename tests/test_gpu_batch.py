@@ -32,7 +32,7 @@ def _check_members(gs, runs):
         assert np.any(run.field("Vz") != 0), n
 
 
-@pytest.mark.parametrize("graph,variant", [(False, 1), (True, 1), (True, 5)])
+@pytest.mark.parametrize("graph,variant", [(False, 1), (True, 1), (True, 5), (True, 13)])
 def test_batch_members_match_their_oracles(orc, graph, variant):
     nt = 18
     cfgs = _members()
@@ -112,3 +112,48 @@ def test_full_greens_batch_matches_members_stepped_alone():
         b.close()
         for s in batched + alone:
             s.close()
+
+
+@pytest.mark.parametrize("packing", [0, 1])
+def test_batch_sections_and_packing(orc, packing):
+    """Sections of every member taken by the batch's one section launch per
+    step (different sections, sizes and decimations per member), under both
+    member packings: each member's frames and maximum maps equal its own
+    oracle run's, bit for bit."""
+    nt, ntdec_s = 14, 3
+    nframe = (nt - 1) // ntdec_s + 1
+    cfgs = _members()[:3]
+    secs = [["xy_v", "fs_v", "yz_ps"], ["xz_u", "ob_v"], ["fs_u", "xy_ps", "xz_v", "ob_u"]]
+    pars = [dict(k0_xy=7, i0_yz=12, j0_xz=11, idec=1, jdec=1, kdec=1),
+            dict(k0_xy=5, i0_yz=9, j0_xz=10, idec=2, jdec=1, kdec=2),
+            dict(k0_xy=6, i0_yz=8, j0_xz=9, idec=1, jdec=3, kdec=1)]
+    models = [O.build_model(c) for c in cfgs]
+    runs, gs = [], []
+    for m, sec, par in zip(models, secs, pars):
+        par = dict(par, ntdec_s=ntdec_s, nframe=nframe)
+        r = O.OracleRun(m, 32)
+        r.snap_setup(sec, **par)
+        r.run(nt)
+        runs.append(r)
+        g = gpu_from_model(m)
+        g.set_snapshots(sec, m.kfs, M0=m.M0, UC=m.UC, **par)
+        gs.append(g)
+    b = SwpcBatch(gs)
+    b.set_packing(packing)
+    assert b.packing() == packing
+    b.step(1, nt, use_graph=True, block=7)
+    b.synchronize()
+    _check_members(gs, runs)
+    nonzero = 0
+    for g, r, sec in zip(gs, runs, secs):
+        for name in sec:
+            for f in range(nframe):
+                want = r.snap_frames[name][f]
+                assert ulp_diff(g.snapshot(name, f), want) == 0, (name, f)
+                nonzero += int(np.any(want != 0))
+            pl, q = name.split("_")
+            if pl in ("fs", "ob") and q in ("v", "u"):
+                want = (r.snap_vmax if q == "v" else r.snap_umax)[pl]
+                assert ulp_diff(g.snap_max(name), want) == 0, name
+    assert nonzero > 0
+    b.close()

@@ -135,7 +135,9 @@ def test_vmax_matches(orc):
 
 @pytest.mark.parametrize("kchunk,by,variant,graph", [(7, 1, 0, False), (16, 2, 1, True), (200, 4, 3, True),
                                                      (64, 4, 3, False), (32, 2, 2, True), (64, 4, 2, False),
-                                                     (7, 1, 4, False), (16, 2, 5, True), (64, 4, 7, False)])
+                                                     (7, 1, 4, False), (16, 2, 5, True), (64, 4, 7, False),
+                                                     (7, 1, 8, False), (16, 2, 9, True), (1, 4, 9, False),
+                                                     (64, 4, 13, True)])
 def test_launch_shapes_same_result(orc, kchunk, by, variant, graph):
     cfg = small_config(nx=44, ny=40, nz=36)
     m = O.build_model(cfg)
@@ -146,7 +148,8 @@ def test_launch_shapes_same_result(orc, kchunk, by, variant, graph):
     _check(g, run, f"kchunk={kchunk} by={by} variant={variant} graph={graph}")
 
 
-@pytest.mark.parametrize("na,by,variant", [(7, 4, 1), (5, 2, 1), (13, 1, 1), (7, 2, 5), (13, 4, 5)])
+@pytest.mark.parametrize("na,by,variant", [(7, 4, 1), (5, 2, 1), (13, 1, 1), (7, 2, 5), (13, 4, 5), (7, 2, 9),
+                                           (13, 4, 13)])
 def test_odd_sizes_and_absorber_widths(orc, na, by, variant):
     """PML tiles where na is not a multiple of the tile height and the grid
     is not a multiple of the tile width: every PML cell updated exactly once."""

@@ -82,3 +82,19 @@ def test_rerun_oracle_trace_bit_for_bit(orc):
             assert tuple(es92(x) for x in va) == fix[it][1]
         if it < NT:
             run.step()
+
+
+def test_vectorised_build_is_the_same_scheme(orc):
+    """The AVX2 build of the FP32 oracle (the CPU baseline's) steps to the
+    same bits as the parity build: vectorisation without contraction or
+    fast-math keeps every rounding."""
+    cfg = O.Config(nx=40, ny=36, nz=32, na=8, nt=30, dx=0.5, dy=0.5, dz=0.5, dt=0.02, zbeg=-1.0,
+                   sources=[dict(x=0.3, y=-0.6, z=4.2, t0=0.0, tr=0.6, mo=1e15, strike=30.0, dip=60.0, rake=90.0)],
+                   stations=[(2.1, 1.3, 0.0, "fsb")])
+    m = O.build_model(cfg)
+    a, b = O.OracleRun(m, 32), O.OracleRun(m, 32, vectorised=True)
+    a.run(30)
+    b.run(30)
+    for n in O.FIELDS_V + O.FIELDS_S:
+        assert np.array_equal(a.field(n).view(np.uint32), b.field(n).view(np.uint32)), n
+    assert np.any(a.field("Vz"))
