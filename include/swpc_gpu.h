@@ -207,6 +207,16 @@ int swpc_step_timed(swpc_ctx *ctx, int it0, int nsteps, float *ms);
    the ocean bottom, 5 cells inside the absorber. Synchronises. */
 int swpc_kernel_vmax(swpc_ctx *ctx, float out[3]);
 
+/* swpc_kernel_vmax without allocation.  No counterpart in the reference: it
+   is for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops).  Writes the three maxima to d_out, device
+   memory of the context's GPU; stream-ordered on the context's stream, does
+   not synchronise.  Its partial maxima live in a fixed per-GPU scratch array
+   of the library (16384 blocks of 256 surface cells, 192 KiB); a larger
+   sampled surface returns SWPC_EINVAL.  Calls that overlap on one GPU share
+   that array and race. */
+int swpc_kernel_vmax_noalloc(swpc_ctx *ctx, float *d_out);
+
 /* Field transfer in the reference (k,i,j) halo-3 layout, FP32. */
 int swpc_get_field(swpc_ctx *ctx, int field, float *out);
 int swpc_set_field(swpc_ctx *ctx, int field, const float *in);

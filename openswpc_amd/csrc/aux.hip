@@ -6,7 +6,9 @@
 // applies its contributions in source order, exactly the sequence of
 // subtractions the reference performs (m_source.f90:798-841), so no atomics
 // and no order-dependent rounding.
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "swpc_internal.h"
 
@@ -384,6 +386,139 @@ hipError_t launch_vmax(swpc_ctx *c)
         hipLaunchKernelGGL(k_vmax, grid, dim3(256), 0, c->stream, a);
     }
     return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// kernel__vmax without allocation (swpc_kernel_vmax_noalloc), for callers
+// that cannot allocate device memory (code under HIP graph capture, tight
+// serving loops).  The sampling blocks leave their partial maxima in
+// `scratch`, a file-scope device array, and one block reduces them into the
+// caller's device output.  Every partial is written before it is read, so
+// nothing needs clearing.  Each GPU holds its own copy of the array (192 KiB)
+// until the process ends; calls that overlap on one GPU share it and race.
+// --------------------------------------------------------------------------
+#define VMAX_SCRATCH_BLOCKS 16384
+static __device__ __align__(256) float scratch[3 * VMAX_SCRATCH_BLOCKS];
+
+// the sampling of k_vmax; partials (x, y, z) of block b at part[3b .. 3b+2]
+__global__ void __launch_bounds__(256) k_vmax_part(VmaxArgs a, float *part)
+{
+    __shared__ float w[4][3];
+    const int i = a.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = a.j0 + blockIdx.y;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (i <= a.i1 && j <= a.j1) {
+        const int k = a.kob[g2idx(a.g, i, j)] + 1;
+        if (k >= 1 && k <= a.g.nz) {
+            const long long p = gidx(a.g, i, j, k);
+            x = fabsf(a.Vx[p]);
+            y = fabsf(a.Vy[p]);
+            z = fabsf(a.Vz[p]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        x = fmaxf(x, __shfl_xor(x, o));
+        y = fmaxf(y, __shfl_xor(y, o));
+        z = fmaxf(z, __shfl_xor(z, o));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        w[wave][0] = x;
+        w[wave][1] = y;
+        w[wave][2] = z;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const float m = fmaxf(fmaxf(w[0][threadIdx.x], w[1][threadIdx.x]), fmaxf(w[2][threadIdx.x], w[3][threadIdx.x]));
+        part[3LL * (blockIdx.y * (long long)gridDim.x + blockIdx.x) + threadIdx.x] = m;
+    }
+}
+
+// one block: the maxima of nb partials into out[0..2] (0 when nb = 0)
+__global__ void __launch_bounds__(256) k_vmax_reduce(const float *part, int nb, float *out)
+{
+    __shared__ float w[4][3];
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        x = fmaxf(x, part[3 * b]);
+        y = fmaxf(y, part[3 * b + 1]);
+        z = fmaxf(z, part[3 * b + 2]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        x = fmaxf(x, __shfl_xor(x, o));
+        y = fmaxf(y, __shfl_xor(y, o));
+        z = fmaxf(z, __shfl_xor(z, o));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        w[wave][0] = x;
+        w[wave][1] = y;
+        w[wave][2] = z;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        out[threadIdx.x] = fmaxf(fmaxf(w[0][threadIdx.x], w[1][threadIdx.x]), fmaxf(w[2][threadIdx.x], w[3][threadIdx.x]));
+}
+
+// Device address of `scratch` on each device, taken once per device with
+// that device current (the array's host-side name is not a device address).
+#define VMAX_SCRATCH_MAXDEV 64
+static float *g_scratch_ptr[VMAX_SCRATCH_MAXDEV];
+static std::mutex g_scratch_mu;
+
+static hipError_t scratch_ptr(int dev, float **out)
+{
+    if (dev < 0 || dev >= VMAX_SCRATCH_MAXDEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+extern "C" int swpc_kernel_vmax_noalloc(swpc_ctx *c, float *d_out)
+{
+    if (!c || !d_out) return SWPC_EINVAL;
+    const Grid &g = c->g;
+    const int margin = 5;
+    VmaxArgs a;
+    a.g = g;
+    a.j0 = max(g.na + margin + 1, g.jb);
+    a.j1 = min(g.ny - g.na - margin, g.je);
+    a.i0 = max(g.na + margin + 1, g.ib);
+    a.i1 = min(g.nx - g.na - margin, g.ie);
+    const bool any = a.i1 >= a.i0 && a.j1 >= a.j0;
+    const long long nbx = any ? (a.i1 - a.i0 + 1 + 255) / 256 : 0, nby = any ? a.j1 - a.j0 + 1 : 0;
+    if (nbx * nby > VMAX_SCRATCH_BLOCKS) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "swpc_kernel_vmax_noalloc: the sampled surface needs %lld blocks of 256 cells, "
+                 "beyond the %d its scratch holds", nbx * nby, VMAX_SCRATCH_BLOCKS);
+        return ctx_fail(c, SWPC_EINVAL, msg);
+    }
+    if (!c->has_medium) return ctx_fail(c, SWPC_ESTATE, "swpc_kernel_vmax_noalloc: medium not set");
+    hipError_t e = hipSetDevice(c->device);
+    // the output must be device memory of the context's GPU
+    hipPointerAttribute_t at;
+    if (e == hipSuccess && (hipPointerGetAttributes(&at, d_out) != hipSuccess || at.type != hipMemoryTypeDevice ||
+                            at.device != c->device)) {
+        (void)hipGetLastError();
+        return ctx_fail(c, SWPC_EINVAL, "swpc_kernel_vmax_noalloc: d_out is not device memory of the context's GPU");
+    }
+    float *part = nullptr;
+    if (e == hipSuccess) e = scratch_ptr(c->device, &part);
+    if (e != hipSuccess) return ctx_fail(c, SWPC_EHIP, hipGetErrorString(e));
+    a.Vx = c->fields[SWPC_VX]; a.Vy = c->fields[SWPC_VY]; a.Vz = c->fields[SWPC_VZ];
+    a.kob = c->kob;
+    a.out = nullptr;
+    if (any) hipLaunchKernelGGL(k_vmax_part, dim3((unsigned)nbx, (unsigned)nby), dim3(256), 0, c->stream, a, part);
+    hipLaunchKernelGGL(k_vmax_reduce, dim3(1), dim3(256), 0, c->stream, part, (int)(nbx * nby), d_out);
+    e = hipGetLastError();
+    if (e != hipSuccess) return ctx_fail(c, SWPC_EHIP, hipGetErrorString(e));
+    return SWPC_OK;
 }
 
 // --------------------------------------------------------------------------

@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_get_snap_max.argtypes = [vp, i, fp]
     L.swpc_snap_dims.argtypes = [vp, i, ip]
     L.swpc_kernel_vmax.argtypes = [vp, fp]
+    L.swpc_kernel_vmax_noalloc.argtypes = [vp, vp]
     L.swpc_medium_layered.argtypes = [vp, fp, vp, fp, fp, ip, ip, fp]
     L.swpc_get_medium.argtypes = [vp, i, vp]
     L.swpc_staging_buffer.argtypes = [vp, C.POINTER(C.c_int64)]
@@ -410,6 +411,11 @@ class SwpcGPU:
         self._chk(self.L.swpc_medium_layered(self.h, prof, xp, np.asarray(clip, np.float32),
                                              np.asarray(tau, np.float32), kfs, kob, vr))
         return kfs, kob, vr
+
+    def kernel_vmax_noalloc(self, d_out):
+        """the three maxima written to d_out (a device address of 3 floats on
+        this context's GPU), stream-ordered; nothing is allocated"""
+        self._chk(self.L.swpc_kernel_vmax_noalloc(self.h, C.c_void_p(int(d_out))))
 
     # ---- data out / in ----
     def get_field(self, name):

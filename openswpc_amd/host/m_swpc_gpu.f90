@@ -197,6 +197,11 @@ module m_swpc_gpu
             type(c_ptr), value :: ctx
             real(c_float), intent(out) :: out(3)
         end function
+        !! kernel__vmax without allocation: the maxima into device memory d_out
+        integer(c_int) function swpc_kernel_vmax_noalloc(ctx, d_out) bind(C, name='swpc_kernel_vmax_noalloc')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, d_out
+        end function
         integer(c_int) function swpc_get_field(ctx, field, out) bind(C, name='swpc_get_field')
             import :: c_int, c_ptr, c_float
             type(c_ptr), value :: ctx

@@ -96,6 +96,7 @@ def test_null_context_is_rejected():
     n = C.c_int64(7)
     assert L.swpc_staging_buffer(None, C.byref(n)) is None and n.value == 0
     assert L.swpc_batch_set_packing(None, 1) == -1 and L.swpc_batch_get_packing(None) == -1
+    assert L.swpc_kernel_vmax_noalloc(None, None) == -1
     f2 = np.zeros(2, np.float32)
     i1 = np.zeros(1, np.int32)
     assert L.swpc_medium_layered(None, f2, None, f2, f2, i1, i1, f2) == -1
