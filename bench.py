@@ -69,7 +69,8 @@ def measure_batch(steps, warmup, block_req):
     b = SwpcBatch(sims)
     t0 = time.perf_counter()
     kc, by, width, tune_ms = b.tune(trial_steps=3)
-    log(f"[bench] autotuned kchunk={kc} by={by} width={width} ({tune_ms:.2f} ms/step in trials, "
+    packing = b.packing()
+    log(f"[bench] autotuned kchunk={kc} by={by} width={width} packing={packing} ({tune_ms:.2f} ms/step in trials, "
         f"{time.perf_counter() - t0:.1f}s)")
     ev = HipEvents()
     block = min(block_req, max(steps, 1))
@@ -83,7 +84,11 @@ def measure_batch(steps, warmup, block_req):
     ev.record(e1, b.stream())
     ms_batch = ev.elapsed_ms(e0, e1) / steps
     it += steps
-    # the same members one after another (each its own graph of `block` steps)
+    # the same members one after another (each its own graph of `block`
+    # steps), each with the plain defaults rather than the batch's tuned shape
+    b.close()
+    for s, sp in zip(sims, specs):
+        s.set_tuning(16 if sp.nm else 32, 4, 1 | 8)
     for s in sims:
         s.step(it, block, use_graph=True, block=block)
     for s in sims:
@@ -103,11 +108,10 @@ def measure_batch(steps, warmup, block_req):
     n_pml = cells - n_int
     b_si, b_vi, b_sp, b_vp = bytes_model(nm)
     bytes_step = n_int * (b_si + b_vi) + n_pml * (b_sp + b_vp)
-    b.close()
     for s in sims:
         s.close()
-    return dict(members=len(specs), cells=cells, block=block, kc=kc, by=by, width=width, ms_batch=ms_batch,
-                ms_seq=ms_seq, bytes_step=bytes_step)
+    return dict(members=len(specs), cells=cells, block=block, kc=kc, by=by, width=width, packing=packing,
+                ms_batch=ms_batch, ms_seq=ms_seq, bytes_step=bytes_step)
 
 
 def batch_summary(r):
@@ -119,7 +123,7 @@ def batch_summary(r):
             "unit": "Gcell-updates/s",
             "step_roofline_frac": round(r["bytes_step"] / (ms_b * 1e-3) / HBM_PEAK, 4),
             "sequential_ms_per_step": round(ms_s, 4), "batched_over_sequential": round(ms_s / ms_b, 3),
-            "autotuned": {"kchunk": r["kc"], "by": r["by"], "tile_width": r["width"]}}
+            "autotuned": {"kchunk": r["kc"], "by": r["by"], "tile_width": r["width"], "packing": r["packing"]}}
 
 
 def run_batch(args):
@@ -145,7 +149,7 @@ def run_batch(args):
         "config": {"workload": f"batched Green's functions: {r['members']} uneven grids 96..256 x 96..256 x "
                                f"96..192 (seed 6), NM=2, 10-cell PML, unit single forces, 8 receivers each, 4 "
                                f"layered profiles", "members": r["members"], "cells": cells, "graph_block": block,
-                   "autotuned": {"kchunk": kc, "by": by, "tile_width": width}},
+                   "autotuned": {"kchunk": kc, "by": by, "tile_width": width, "packing": r["packing"]}},
         "step_roofline": {"bytes_per_step": bytes_step, "achieved": round(bytes_step / (ms_batch * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "frac": round(bytes_step / (ms_batch * 1e-3) / HBM_PEAK, 4)},
         "sequential": {"ms_per_step": round(ms_seq, 4), "value": round(cells / (ms_seq * 1e-3) / 1e9, 4),

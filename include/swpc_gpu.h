@@ -319,8 +319,9 @@ int swpc_batch_synchronize(swpc_batch *batch);
 int swpc_batch_tune(swpc_batch *batch, int trial_steps, int *kchunk, int *by, int *width, float *ms_per_step);
 /* Order of the members' tiles in the batched launches: 0 member-major,
    members largest first (the default); 1 chunk-major, every member's first
-   k chunk, then every member's second, and so on.  Results do not depend
-   on it. */
+   k chunk, then every member's second, and so on; 2 member-major with each
+   member's side-column PML tiles in the interior launch, next to the tile
+   rows they border (one launch per phase).  Results do not depend on it. */
 int swpc_batch_set_packing(swpc_batch *batch, int packing);
 int swpc_batch_get_packing(const swpc_batch *batch);
 void *swpc_batch_stream(swpc_batch *batch);
@@ -386,7 +387,9 @@ int64_t swpc_device_bytes(const swpc_ctx *ctx);
    interior tiles 32 cells wide with 2 by rows (narrow grids idle fewer
    lanes), bit 3 = the interior velocity update loads plane k+1 while it
    computes plane k, bit 4 = the adjoint copies its step records instead of
-   stepping out of place into them.  All variants give bit-identical results. */
+   stepping out of place into them, bit 5 = the interior and side columns of
+   a phase in one launch, x-slab tiles next to the interior rows they
+   border.  All variants give bit-identical results. */
 int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a

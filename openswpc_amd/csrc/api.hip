@@ -285,6 +285,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = SWPC_EHIP; goto bad; }
     if ((rc = build_pml_tiles(c)) != SWPC_OK) goto bad;
     if ((rc = build_zchunks(c)) != SWPC_OK) goto bad;
+    if ((rc = build_schedule(c)) != SWPC_OK) goto bad;
     *out = c;
     return SWPC_OK;
 bad:
@@ -320,6 +321,7 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     dfree(c->rho); dfree(c->lam); dfree(c->mu); dfree(c->taup); dfree(c->taus);
     dfree(c->kfs_top); dfree(c->kfs_bot); dfree(c->kob_top); dfree(c->kob_bot); dfree(c->kob);
     dfree(c->R); dfree(c->ade); dfree(c->d_pml_tiles); dfree(c->d_zchunks);
+    dfree(c->d_sched[0]); dfree(c->d_sched[1]);
     dfree(c->gxc); dfree(c->gxe); dfree(c->gyc); dfree(c->gye); dfree(c->gzc); dfree(c->gze);
     dfree(c->cgx_c); dfree(c->cgx_b); dfree(c->cgy_c); dfree(c->cgy_b); dfree(c->cgz_c); dfree(c->cgz_b);
     dfree(c->d_targets); dfree(c->d_contrib); dfree(c->d_comp); dfree(c->d_mo); dfree(c->d_stf);
@@ -929,18 +931,29 @@ static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it, hipEvent_t 
     if ((e = launch_green_store(c, it, d_it)) != hipSuccess) return e;
     if ((e = launch_wav_store(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if ((e = launch_snap(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
+    const bool merged = pml && merged_launch_ready(c);
     if ((e = fork_to_side()) != hipSuccess) return e;
-    if (pml && (e = launch_stress_side(c, side)) != hipSuccess) return e;
+    if (pml && !merged && (e = launch_stress_side(c, side)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
-    if ((e = launch_stress_cols(c, pml ? 3 : 1)) != hipSuccess || (e = join_side()) != hipSuccess) return e;
+    if (merged) {
+        if ((e = launch_stress_all(c)) != hipSuccess) return e;
+    } else if ((e = launch_stress_cols(c, pml ? 3 : 1)) != hipSuccess) {
+        return e;
+    }
+    if ((e = join_side()) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
     if (!pml && (e = launch_cerjan_stress(c)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
     if ((e = launch_stressglut(c, it, d_it)) != hipSuccess || (e = mark()) != hipSuccess) return e;
     if ((e = fork_to_side()) != hipSuccess) return e;
-    if (pml && (e = launch_vel_side(c, side)) != hipSuccess) return e;
+    if (pml && !merged && (e = launch_vel_side(c, side)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
-    if ((e = launch_vel_cols(c, pml ? 3 : 1)) != hipSuccess || (e = join_side()) != hipSuccess) return e;
+    if (merged) {
+        if ((e = launch_vel_all(c)) != hipSuccess) return e;
+    } else if ((e = launch_vel_cols(c, pml ? 3 : 1)) != hipSuccess) {
+        return e;
+    }
+    if ((e = join_side()) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
     if (!pml && (e = launch_cerjan_vel(c)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
@@ -1104,5 +1117,7 @@ extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by, int variant)
     if (variant >= 0) c->variant = variant;
     reset_graph(c);
     int rc = build_pml_tiles(c);
-    return rc != SWPC_OK ? rc : build_zchunks(c);
+    if (rc == SWPC_OK) rc = build_zchunks(c);
+    if (rc == SWPC_OK && (rc = build_schedule(c)) != SWPC_OK) rc = fail(c, rc, "swpc_set_tuning: launch schedule");
+    return rc;
 }

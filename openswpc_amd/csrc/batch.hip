@@ -17,7 +17,10 @@
 //      largest first, so the smallest members fill the end of the launch;
 //   1: chunk-major: the tiles of every member's first k chunk, then of every
 //      member's second chunk, and so on (members largest first within a
-//      chunk), so each stretch of the launch streams many members at once.
+//      chunk), so each stretch of the launch streams many members at once;
+//   2: member-major with the side-column tiles in the interior launch: one
+//      launch per phase, each member's x-slab tiles next to the interior tile
+//      rows they border, so the cache lines the two share are fetched once.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -36,6 +39,8 @@ struct swpc_batch {
     void *d_args[2];               // StepArgs of every member: stress, velocity
     int4 *d_cols, *d_side;
     int ncols, nside;
+    int4 *d_all;                   // packing 2: interior and side tiles in one table
+    int nall;
     void *d_aux[BATCH_AUX_N];      // receiver, source and Green's-function arguments of the members that have them
     int naux[BATCH_AUX_N], maxaux[BATCH_AUX_N];
     void *d_snap;                  // section arguments of the members with sections
@@ -87,6 +92,10 @@ static void free_tables(swpc_batch *b)
     bfree(p);
     b->d_cols = b->d_side = nullptr;
     b->ncols = b->nside = 0;
+    p = b->d_all;
+    bfree(p);
+    b->d_all = nullptr;
+    b->nall = 0;
     for (int k = 0; k < BATCH_AUX_N; ++k) {
         bfree(b->d_aux[k]);
         b->d_aux[k] = nullptr;
@@ -103,6 +112,9 @@ static void free_tables(swpc_batch *b)
     }
     b->ncerj = 0;
 }
+
+// interior tile width of the members (variant bit 2: 32 cells, else 64)
+static int batch_bxw(const swpc_batch *b) { return (b->mem[0]->variant & 4) ? 32 : 64; }
 
 // (re)build the argument arrays and tile tables from the members' current state
 static int build_tables(swpc_batch *b)
@@ -148,8 +160,35 @@ static int build_tables(swpc_batch *b)
             }
         }
     }
-    std::vector<int4> cols, side;
-    if (b->packing == 1) {
+    std::vector<int4> cols, side, all;
+    if (b->packing == 2) {
+        const int byi = b->by * (64 / batch_bxw(b));
+        for (int m : order) {
+            const Grid &g = b->mem[m]->g;
+            for (size_t ch = 0; ch < mcols[m].size(); ++ch) {
+                std::vector<int4> low, high, xs;
+                for (const int4 &t : mside[m][ch]) {
+                    const int4 e = make_int4(-(m + 1), t.x, t.y, t.z);
+                    if ((t.x >> 28) & 1) xs.push_back(e);
+                    else if (t.y <= g.na) low.push_back(e);
+                    else high.push_back(e);
+                }
+                all.insert(all.end(), low.begin(), low.end());
+                size_t q = 0;
+                int row = -1;
+                for (const int4 &t : mcols[m][ch]) {
+                    if (t.z != row) {
+                        row = t.z;
+                        const int j1 = g.jb + (row + 1) * byi - 1;
+                        while (q < xs.size() && xs[q].z <= j1) all.push_back(xs[q++]);
+                    }
+                    all.push_back(t);
+                }
+                for (; q < xs.size(); ++q) all.push_back(xs[q]);
+                all.insert(all.end(), high.begin(), high.end());
+            }
+        }
+    } else if (b->packing == 1) {
         for (int ch = 0; ch < maxchunks; ++ch)
             for (int m : order) {
                 if (ch < (int)mcols[m].size()) cols.insert(cols.end(), mcols[m][ch].begin(), mcols[m][ch].end());
@@ -174,8 +213,13 @@ static int build_tables(swpc_batch *b)
         BCHK(b, hipMalloc((void **)&b->d_side, side.size() * sizeof(int4)));
         BCHK(b, hipMemcpy(b->d_side, side.data(), side.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
+    if (!all.empty()) {
+        BCHK(b, hipMalloc((void **)&b->d_all, all.size() * sizeof(int4)));
+        BCHK(b, hipMemcpy(b->d_all, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
     b->ncols = (int)cols.size();
     b->nside = (int)side.size();
+    b->nall = (int)all.size();
     // receivers and sources: one launch per kind for every member that has them
     for (int k = 0; k < BATCH_AUX_N; ++k) {
         const size_t sz = batch_aux_size(k);
@@ -250,8 +294,6 @@ static bool tables_stale(const swpc_batch *b)
     return false;
 }
 
-// interior tile width of the members (variant bit 2: 32 cells, else 64)
-static int batch_bxw(const swpc_batch *b) { return (b->mem[0]->variant & 4) ? 32 : 64; }
 
 // one step of every member, in the order of src/swpc_3d/main.f90:122-138
 static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
@@ -265,18 +307,22 @@ static hipError_t enqueue_batch_step(swpc_batch *b, int it, const int *d_it)
     if ((e = launch_batch_aux(BATCH_AUX_WAV, b->d_aux[BATCH_AUX_WAV], b->naux[BATCH_AUX_WAV],
                               b->maxaux[BATCH_AUX_WAV], it, d_it, s)) != hipSuccess)
         return e;
-    if ((e = launch_batch_phase(0, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[0], b->d_cols, b->ncols, b->d_side,
-                                b->nside, s)) != hipSuccess)
-        return e;
+    if (b->packing == 2) e = launch_batch_phase_all(0, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[0],
+                                                    b->d_all, b->nall, s);
+    else e = launch_batch_phase(0, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[0], b->d_cols, b->ncols,
+                                b->d_side, b->nside, s);
+    if (e != hipSuccess) return e;
     if ((e = launch_batch_cerjan(0, b->d_cerj[0], b->d_cerj_blk0[0], b->ncerj, b->ncerj_blocks[0], s)) != hipSuccess)
         return e;
     // each member's sponge before its stress glut, as in its own step
     if ((e = launch_batch_aux(BATCH_AUX_GLUT, b->d_aux[BATCH_AUX_GLUT], b->naux[BATCH_AUX_GLUT],
                               b->maxaux[BATCH_AUX_GLUT], it, d_it, s)) != hipSuccess)
         return e;
-    if ((e = launch_batch_phase(1, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[1], b->d_cols, b->ncols, b->d_side,
-                                b->nside, s)) != hipSuccess)
-        return e;
+    if (b->packing == 2) e = launch_batch_phase_all(1, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[1],
+                                                    b->d_all, b->nall, s);
+    else e = launch_batch_phase(1, b->nm, b->mem[0]->variant, b->by, batch_bxw(b), b->d_args[1], b->d_cols, b->ncols,
+                                b->d_side, b->nside, s);
+    if (e != hipSuccess) return e;
     if ((e = launch_batch_cerjan(1, b->d_cerj[1], b->d_cerj_blk0[1], b->ncerj, b->ncerj_blocks[1], s)) != hipSuccess)
         return e;
     // sponge, body force, then the Green's-function source, per member as in its own step
@@ -398,7 +444,7 @@ extern "C" int swpc_batch_step(swpc_batch *b, int it0, int nsteps, int use_graph
 extern "C" int swpc_batch_set_packing(swpc_batch *b, int packing)
 {
     if (!b) return SWPC_EINVAL;
-    if (packing < 0 || packing > 1) return bfail(b, SWPC_EINVAL, "swpc_batch_set_packing: packing 0 or 1");
+    if (packing < 0 || packing > 2) return bfail(b, SWPC_EINVAL, "swpc_batch_set_packing: packing 0, 1 or 2");
     BCHK(b, hipSetDevice(b->device));
     if (packing == b->packing) return SWPC_OK;
     b->packing = packing;
@@ -428,7 +474,7 @@ extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, 
     const int kc_cand[] = {16, 24, 32, 48, 64};
     const int by_cand[] = {2, 4};
     const int w_cand[] = {64, 32};
-    const int pk_cand[] = {0, 1};
+    const int pk_cand[] = {0, 1, 2};
     const int base = b->mem[0]->variant & ~4;
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));

@@ -13,6 +13,8 @@
 //
 // Cerjan tapers (m_absorb_c) are separate per-cell sweeps, as in the
 // reference.
+#include <vector>
+
 #include "swpc_internal.h"
 
 #define BX 64
@@ -810,6 +812,48 @@ __global__ void __launch_bounds__(256) k_vel_pml_b(const StepArgs *__restrict__ 
     vel_pml_body<false>(mem[t.w], unpack_bside(t));
 }
 
+// One launch per phase for the interior columns and the side columns
+// together (variant bit 5): a schedule lists interior tiles (bx, by, bz, -1)
+// and side tiles (i0 | width << 20 | xslab << 28, j0, k0 | k1 << 16, 0), the
+// x-slab tiles of a k chunk next to the interior tile rows they border, so
+// the 128-byte lines the two kinds share are fetched once through the L2 of
+// one XCD.  The two kinds update disjoint cells from the other phase's
+// fields, so their order does not change any result.
+template <int NM, bool NT>
+__global__ void __launch_bounds__(256) k_stress_all(StepArgs a, const int4 *__restrict__ sched)
+{
+    const int4 t = sched[xcd_remap(blockIdx.x, gridDim.x)];
+    if (t.w < 0) stress_int_body<NM, NT, false>(a, t.x, t.y, t.z);
+    else stress_pml_body<false>(a, make_int4(t.x, t.y, t.z & 0xFFFF, t.z >> 16));
+}
+
+template <bool PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PF ? 3 : 4))) k_vel_all(StepArgs a, const int4 *__restrict__ sched)
+{
+    const int4 t = sched[xcd_remap(blockIdx.x, gridDim.x)];
+    if (t.w < 0) vel_int_body<false, PF>(a, t.x, t.y, t.z);
+    else vel_pml_body<false>(a, make_int4(t.x, t.y, t.z & 0xFFFF, t.z >> 16));
+}
+
+// batched form of the merged launch: entries (m, bx, by, bz) are interior
+// tiles of member m, entries (-(m+1), i0 | width << 20 | xslab << 28, j0,
+// k0 | k1 << 16) its side tiles
+template <int NM, bool NT>
+__global__ void __launch_bounds__(256) k_stress_all_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    if (t.x >= 0) stress_int_body<NM, NT, false>(mem[t.x], t.y, t.z, t.w);
+    else stress_pml_body<false>(mem[-t.x - 1], make_int4(t.y, t.z, t.w & 0xFFFF, t.w >> 16));
+}
+
+template <bool PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PF ? 3 : 4))) k_vel_all_b(const StepArgs *__restrict__ mem, const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    if (t.x >= 0) vel_int_body<false, PF>(mem[t.x], t.y, t.z, t.w);
+    else vel_pml_body<false>(mem[-t.x - 1], make_int4(t.y, t.z, t.w & 0xFFFF, t.w >> 16));
+}
+
 void fill_step_args(swpc_ctx *c, StepArgs &a, int mask);
 static void step_args(swpc_ctx *c, StepArgs &a, int mask) { fill_step_args(c, a, mask); }
 
@@ -917,6 +961,106 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
         if (pf) hipLaunchKernelGGL((k_vel_int<false, true>), grid, block, 0, c->stream, a);
         else hipLaunchKernelGGL((k_vel_int<false, false>), grid, block, 0, c->stream, a);
     }
+    return hipGetLastError();
+}
+
+// schedules of the merged launches, one per phase (see k_stress_all)
+int build_schedule(swpc_ctx *c)
+{
+    for (int ph = 0; ph < 2; ++ph) {
+        if (c->d_sched[ph]) (void)hipFree(c->d_sched[ph]);
+        c->d_sched[ph] = nullptr;
+        c->nsched[ph] = 0;
+    }
+    const Grid &g = c->g;
+    if (c->desc.abc != SWPC_ABC_PML || c->npml == 0 || c->npml_tiles <= 0 || g.nxk <= 0 || g.nyk <= 0 ||
+        g.nz > 0x7FFF)
+        return SWPC_OK;
+    StepArgs a;
+    step_args(c, a, MASK_INT | MASK_PML);
+    std::vector<int4> side(c->npml_tiles);
+    std::vector<int> zc(2 * (c->nbz + 1));
+    if (hipMemcpy(side.data(), c->d_pml_tiles, side.size() * sizeof(int4), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(zc.data(), c->d_zchunks, zc.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return SWPC_EHIP;
+    for (int ph = 0; ph < 2; ++ph) {
+        const int *z = zc.data() + ph * (c->nbz + 1);
+        std::vector<int4> out;
+        std::vector<char> used(side.size(), 0);
+        for (int bz = 0; bz < a.nbz; ++bz) {
+            const int kb = z[bz], ke = z[bz + 1] - 1;
+            std::vector<int> low, high, xs;
+            for (size_t n = 0; n < side.size(); ++n) {
+                const int4 t = side[n];
+                if (used[n] || t.z < kb || t.z > ke) continue;
+                used[n] = 1;
+                const bool xslab = (t.x >> 28) & 1;
+                if (xslab) xs.push_back((int)n);
+                else if (t.y <= g.na) low.push_back((int)n);
+                else high.push_back((int)n);
+            }
+            auto emit_side = [&](int n) {
+                const int4 t = side[n];
+                out.push_back(make_int4(t.x, t.y, t.z | (t.w << 16), 0));
+            };
+            for (int n : low) emit_side(n);
+            size_t q = 0;
+            for (int by = 0; by < a.nby; ++by) {
+                const int j1 = g.jb + (by + 1) * a.byi - 1;
+                while (q < xs.size() && side[xs[q]].y <= j1) emit_side(xs[q++]);
+                for (int bx = 0; bx < a.nbx; ++bx) out.push_back(make_int4(bx, by, bz, -1));
+            }
+            for (; q < xs.size(); ++q) emit_side(xs[q]);
+            for (int n : high) emit_side(n);
+        }
+        for (size_t n = 0; n < side.size(); ++n)
+            if (!used[n]) {
+                const int4 t = side[n];
+                out.push_back(make_int4(t.x, t.y, t.z | (t.w << 16), 0));
+            }
+        if (hipMalloc((void **)&c->d_sched[ph], out.size() * sizeof(int4)) != hipSuccess) return SWPC_ENOMEM;
+        if (hipMemcpy(c->d_sched[ph], out.data(), out.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+            return SWPC_EHIP;
+        c->nsched[ph] = (int)out.size();
+    }
+    return SWPC_OK;
+}
+
+// both kinds of columns of a phase in one launch, when the schedule exists
+// and the step is in place
+bool merged_launch_ready(const swpc_ctx *c)
+{
+    return (c->variant & 32) && c->d_sched[0] && c->d_sched[1] && !c->fields_in[0];
+}
+
+hipError_t launch_stress_all(swpc_ctx *c)
+{
+    StepArgs a;
+    step_args(c, a, MASK_INT | MASK_PML);
+    const dim3 grid(c->nsched[0]), block(a.bxw, a.byi);
+    const bool nt = (c->variant & 1) != 0;
+#define SWPC_ALL_CASE(NMV)                                                                                 \
+    case NMV:                                                                                              \
+        if (nt) hipLaunchKernelGGL((k_stress_all<NMV, true>), grid, block, 0, c->stream, a, c->d_sched[0]); \
+        else hipLaunchKernelGGL((k_stress_all<NMV, false>), grid, block, 0, c->stream, a, c->d_sched[0]);   \
+        break;
+    switch (c->g.nm) {
+        SWPC_ALL_CASE(0) SWPC_ALL_CASE(1) SWPC_ALL_CASE(2) SWPC_ALL_CASE(3) SWPC_ALL_CASE(4)
+        SWPC_ALL_CASE(5) SWPC_ALL_CASE(6) SWPC_ALL_CASE(7) SWPC_ALL_CASE(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef SWPC_ALL_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_vel_all(swpc_ctx *c)
+{
+    StepArgs a;
+    step_args(c, a, MASK_INT | MASK_PML);
+    a.zc = c->d_zchunks + c->nbz + 1;
+    const dim3 grid(c->nsched[1]), block(a.bxw, a.byi);
+    if (c->variant & 8) hipLaunchKernelGGL((k_vel_all<true>), grid, block, 0, c->stream, a, c->d_sched[1]);
+    else hipLaunchKernelGGL((k_vel_all<false>), grid, block, 0, c->stream, a, c->d_sched[1]);
     return hipGetLastError();
 }
 
@@ -1081,6 +1225,33 @@ void member_tiles(swpc_ctx *c, int &nbx, int &nby, int &nbz)
     nbx = a.nbx;
     nby = a.nby;
     nbz = a.nbz;
+}
+
+// one launch per phase for both kinds of tiles (the batch's packing 2)
+hipError_t launch_batch_phase_all(int phase, int nm, int variant, int by, int bxw, const void *d_args,
+                                  const int4 *d_all, int nall, hipStream_t s)
+{
+    if (nall <= 0) return hipSuccess;
+    const StepArgs *mem = (const StepArgs *)d_args;
+    const dim3 block(bxw, by * (BX / bxw));
+    const bool nt = (variant & 1) != 0;
+    if (phase == 1) {
+        if (variant & 8) hipLaunchKernelGGL(k_vel_all_b<true>, dim3(nall), block, 0, s, mem, d_all);
+        else hipLaunchKernelGGL(k_vel_all_b<false>, dim3(nall), block, 0, s, mem, d_all);
+        return hipGetLastError();
+    }
+#define SWPC_BALL_CASE(NMV)                                                                           \
+    case NMV:                                                                                         \
+        if (nt) hipLaunchKernelGGL((k_stress_all_b<NMV, true>), dim3(nall), block, 0, s, mem, d_all); \
+        else hipLaunchKernelGGL((k_stress_all_b<NMV, false>), dim3(nall), block, 0, s, mem, d_all);   \
+        break;
+    switch (nm) {
+        SWPC_BALL_CASE(0) SWPC_BALL_CASE(1) SWPC_BALL_CASE(2) SWPC_BALL_CASE(3) SWPC_BALL_CASE(4)
+        SWPC_BALL_CASE(5) SWPC_BALL_CASE(6) SWPC_BALL_CASE(7) SWPC_BALL_CASE(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef SWPC_BALL_CASE
+    return hipGetLastError();
 }
 
 hipError_t launch_batch_phase(int phase, int nm, int variant, int by, int bxw, const void *d_args, const int4 *d_cols,

@@ -192,10 +192,12 @@ struct swpc_ctx {
     int kchunk, by;
     int nbz;           // k chunks of the interior-column kernels
     int *d_zchunks;    // first plane of each chunk, stress phase then velocity phase (2 x (nbz+1))
+    int4 *d_sched[2];  // merged interior + side-column schedules of the two phases (variant bit 5)
+    int nsched[2];
     int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel,
                        // bit 1: side-column PML kernels on a second stream,
                        // bit 2: 32-wide interior tiles, bit 3: velocity loads one plane ahead,
-                       // bit 4: adjoint records copied
+                       // bit 4: adjoint records copied, bit 5: interior and side columns in one launch
     long long bytes;
     struct AdjWorkspace *adj_ws;  // adjoint state, gradient sums and checkpoints (adjoint.hip)
 };
@@ -213,6 +215,10 @@ hipError_t launch_vel(swpc_ctx *c, int mask, hipStream_t pml_stream);
 hipError_t launch_stress_side(swpc_ctx *c, hipStream_t s);
 hipError_t launch_vel_side(swpc_ctx *c, hipStream_t s);
 hipError_t launch_stress_cols(swpc_ctx *c, int mask);
+int build_schedule(swpc_ctx *c);
+bool merged_launch_ready(const swpc_ctx *c);
+hipError_t launch_stress_all(swpc_ctx *c);
+hipError_t launch_vel_all(swpc_ctx *c);
 hipError_t launch_vel_cols(swpc_ctx *c, int mask);
 hipError_t launch_kernel_stress(swpc_ctx *c);
 hipError_t launch_kernel_vel(swpc_ctx *c);
@@ -254,6 +260,8 @@ hipError_t launch_batch_snap(const void *d_args, int nmem, int maxblk, int it, c
 size_t batch_cerjan_size();
 int batch_cerjan_fill(swpc_ctx *c, int phase, void *out);
 hipError_t launch_batch_cerjan(int phase, const void *d_args, const int *d_blk0, int nmem, int nblocks, hipStream_t s);
+hipError_t launch_batch_phase_all(int phase, int nm, int variant, int by, int bxw, const void *d_args,
+                                  const int4 *d_all, int nall, hipStream_t s);
 hipError_t launch_batch_phase(int phase, int nm, int variant, int by, int bxw, const void *d_args, const int4 *d_cols,
                               int ncols, const int4 *d_side, int nside, hipStream_t s);
 hipError_t launch_vmax(swpc_ctx *c);

@@ -1,7 +1,7 @@
 """Batches of uneven simulations stepped together (swpc_batch_*): every
 member must end bit-identical to its own oracle run, with and without graph
-replays, with Cerjan and PML members mixed, and after the autotuner's dry
-trials."""
+replays, with Cerjan and PML members mixed, under the three member packings,
+and after the autotuner's dry trials."""
 import numpy as np
 import pytest
 
@@ -32,8 +32,9 @@ def _check_members(gs, runs):
         assert np.any(run.field("Vz") != 0), n
 
 
-@pytest.mark.parametrize("graph,variant", [(False, 1), (True, 1), (True, 5), (True, 13)])
-def test_batch_members_match_their_oracles(orc, graph, variant):
+@pytest.mark.parametrize("graph,variant,packing", [(False, 1, 0), (True, 1, 0), (True, 5, 0), (True, 13, 0),
+                                                   (True, 9, 1), (True, 9, 2), (False, 13, 2)])
+def test_batch_members_match_their_oracles(orc, graph, variant, packing):
     nt = 18
     cfgs = _members()
     models = [O.build_model(c) for c in cfgs]
@@ -44,6 +45,7 @@ def test_batch_members_match_their_oracles(orc, graph, variant):
         runs.append(r)
     gs = [gpu_from_model(m, variant=variant) for m in models]
     b = SwpcBatch(gs)
+    b.set_packing(packing)
     if graph:
         b.step(1, 12, use_graph=True, block=4)
         b.step(13, nt - 12, use_graph=True, block=3)
@@ -114,7 +116,7 @@ def test_full_greens_batch_matches_members_stepped_alone():
             s.close()
 
 
-@pytest.mark.parametrize("packing", [0, 1])
+@pytest.mark.parametrize("packing", [0, 1, 2])
 def test_batch_sections_and_packing(orc, packing):
     """Sections of every member taken by the batch's one section launch per
     step (different sections, sizes and decimations per member), under both

@@ -124,3 +124,33 @@ def test_full_grid_sources_match_the_oracle_at_three_offsets(orc):
             gc.collect()
     finally:
         g.close()
+
+
+@pytest.mark.timeout(900)
+def test_production_medium_on_the_device_equals_the_host_library():
+    """The benchmark's own set-up at full size: the production workload's
+    medium built on the device (the von Karman field generated into the
+    context's staging buffer, swpc_medium_layered) equals the host library's
+    set-up of the same workload (the same field copied to the host,
+    swpch_perturb and swpch_medium_setup), bit for bit in every medium array,
+    band map and kernel coefficient."""
+    from openswpc_amd.host import build_inputs, make_simulation_device
+    from openswpc_amd.workloads import production_spec
+    spec = production_spec()
+    gd, inp_d = make_simulation_device(spec)
+    inp = build_inputs(spec)
+    a = inp.arrays
+    for n in ("kfs", "kob", "ist", "jst", "kst", "c1", "c2", "d1", "ts"):
+        assert np.array_equal(inp_d.arrays[n], a[n]), n
+    for n in ("vmin", "vmax", "zeta", "d2"):
+        assert np.float32(inp_d.scalars[n]) == np.float32(inp.scalars[n]), n
+    for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob"):
+        assert np.array_equal(gd.get_medium(n), a[n]), n
+    for n in ("rho", "lam", "mu", "taup", "taus"):
+        got = gd.get_medium(n)
+        want = a.pop(n)
+        nbad = int(np.count_nonzero(got.view(np.uint32) != want.view(np.uint32)))
+        assert nbad == 0, (n, nbad)
+        del got, want
+        gc.collect()
+    gd.close()

@@ -137,7 +137,8 @@ def test_vmax_matches(orc):
                                                      (64, 4, 3, False), (32, 2, 2, True), (64, 4, 2, False),
                                                      (7, 1, 4, False), (16, 2, 5, True), (64, 4, 7, False),
                                                      (7, 1, 8, False), (16, 2, 9, True), (1, 4, 9, False),
-                                                     (64, 4, 13, True)])
+                                                     (64, 4, 13, True), (16, 4, 41, True), (7, 2, 33, False),
+                                                     (32, 4, 45, True), (5, 1, 43, True)])
 def test_launch_shapes_same_result(orc, kchunk, by, variant, graph):
     cfg = small_config(nx=44, ny=40, nz=36)
     m = O.build_model(cfg)
@@ -149,7 +150,7 @@ def test_launch_shapes_same_result(orc, kchunk, by, variant, graph):
 
 
 @pytest.mark.parametrize("na,by,variant", [(7, 4, 1), (5, 2, 1), (13, 1, 1), (7, 2, 5), (13, 4, 5), (7, 2, 9),
-                                           (13, 4, 13)])
+                                           (13, 4, 13), (7, 4, 41), (5, 2, 45), (13, 1, 41)])
 def test_odd_sizes_and_absorber_widths(orc, na, by, variant):
     """PML tiles where na is not a multiple of the tile height and the grid
     is not a multiple of the tile width: every PML cell updated exactly once."""
