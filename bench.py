@@ -50,6 +50,35 @@ def log(*a):
 
 
 # ---------------------------------------------------------------------------
+# placement of the per-cell arrays over the HBM regions (swpc_stripe_reserve)
+# ---------------------------------------------------------------------------
+STRIPE = {"on": True}
+
+
+def stripe_on(bytes_per_region):
+    """Reserve the striped arena for the contexts created next; returns a
+    description of the placement (hipMalloc when striping is off or fails)."""
+    from openswpc_amd.gpu import SwpcError, stripe_reserve
+    if not STRIPE["on"]:
+        return "hipMalloc"
+    try:
+        t = time.perf_counter()
+        stripe_reserve(int(bytes_per_region))
+        log(f"[bench] striped arena: {bytes_per_region / 2**30:.1f} GiB in each of 3 HBM regions "
+            f"({time.perf_counter() - t:.2f}s)")
+        return "striped over 3 HBM regions"
+    except SwpcError as e:
+        log(f"[bench] striped arena not reserved ({e}); hipMalloc placement")
+        return "hipMalloc"
+
+
+def stripe_off(placement):
+    from openswpc_amd.gpu import stripe_release
+    if placement != "hipMalloc":
+        stripe_release()
+
+
+# ---------------------------------------------------------------------------
 # runs of the workloads (openswpc_amd/workloads.py)
 # ---------------------------------------------------------------------------
 def measure_batch(steps, warmup, block_req):
@@ -57,8 +86,11 @@ def measure_batch(steps, warmup, block_req):
     another, each as graph replays.  Returns the measured figures."""
     from openswpc_amd.gpu import SwpcBatch
     from openswpc_amd.host import build_inputs, make_simulation
+    from openswpc_amd.gpu import stripe_bytes_per_region
     t0 = time.perf_counter()
     specs = batch_specs()
+    placement = stripe_on(sum(stripe_bytes_per_region(sp.nx, sp.ny, sp.nz, sp.nm, sp.na) for sp in specs)
+                          + (1 << 30))
     sims = []
     for sp in specs:
         inp = build_inputs(sp)
@@ -110,8 +142,9 @@ def measure_batch(steps, warmup, block_req):
     bytes_step = n_int * (b_si + b_vi) + n_pml * (b_sp + b_vp)
     for s in sims:
         s.close()
+    stripe_off(placement)
     return dict(members=len(specs), cells=cells, block=block, kc=kc, by=by, width=width, packing=packing,
-                ms_batch=ms_batch, ms_seq=ms_seq, bytes_step=bytes_step)
+                ms_batch=ms_batch, ms_seq=ms_seq, bytes_step=bytes_step, placement=placement)
 
 
 def batch_summary(r):
@@ -123,7 +156,8 @@ def batch_summary(r):
             "unit": "Gcell-updates/s",
             "step_roofline_frac": round(r["bytes_step"] / (ms_b * 1e-3) / HBM_PEAK, 4),
             "sequential_ms_per_step": round(ms_s, 4), "batched_over_sequential": round(ms_s / ms_b, 3),
-            "autotuned": {"kchunk": r["kc"], "by": r["by"], "tile_width": r["width"], "packing": r["packing"]}}
+            "autotuned": {"kchunk": r["kc"], "by": r["by"], "tile_width": r["width"], "packing": r["packing"]},
+            "placement": r["placement"]}
 
 
 def run_batch(args):
@@ -149,7 +183,8 @@ def run_batch(args):
         "config": {"workload": f"batched Green's functions: {r['members']} uneven grids 96..256 x 96..256 x "
                                f"96..192 (seed 6), NM=2, 10-cell PML, unit single forces, 8 receivers each, 4 "
                                f"layered profiles", "members": r["members"], "cells": cells, "graph_block": block,
-                   "autotuned": {"kchunk": kc, "by": by, "tile_width": width, "packing": r["packing"]}},
+                   "autotuned": {"kchunk": kc, "by": by, "tile_width": width, "packing": r["packing"]},
+                   "placement": r["placement"]},
         "step_roofline": {"bytes_per_step": bytes_step, "achieved": round(bytes_step / (ms_batch * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "frac": round(bytes_step / (ms_batch * 1e-3) / HBM_PEAK, 4)},
         "sequential": {"ms_per_step": round(ms_seq, 4), "value": round(cells / (ms_seq * 1e-3) / 1e9, 4),
@@ -520,10 +555,13 @@ def main():
     ap.add_argument("--host-medium", action="store_true",
                     help="build the medium with the host library and upload it, instead of on the device")
     ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per kernel with events")
+    ap.add_argument("--no-stripe", action="store_true",
+                    help="plain hipMalloc placement instead of the arrays striped over the HBM regions")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--by", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
     args = ap.parse_args()
+    STRIPE["on"] = not args.no_stripe
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this library runs on one GPU only: the whole domain lives on one "
@@ -546,18 +584,19 @@ def main():
         run_rmedia(args)
         return
 
+    from openswpc_amd.gpu import stripe_bytes_per_region
     from openswpc_amd.host import build_inputs, make_simulation, make_simulation_device
 
     t0 = time.perf_counter()
+    spec = production_spec() if args.workload == "production" else plain_spec()
+    placement = stripe_on(stripe_bytes_per_region(spec.nx, spec.ny, spec.nz, spec.nm, spec.na) + (512 << 20))
     if args.host_medium:
-        spec = production_spec() if args.workload == "production" else plain_spec()
         inp = build_inputs(spec)
         log(f"[bench] host set-up {time.perf_counter() - t0:.1f}s")
         sim = make_simulation(inp)
     else:
         # the medium is built on the device (swpc_medium_layered), from a
         # random medium generated into the context's staging buffer
-        spec = production_spec() if args.workload == "production" else plain_spec()
         sim, inp = make_simulation_device(spec)
     setup_s = time.perf_counter() - t0
     log(f"[bench] set-up {setup_s:.2f}s ({'host' if args.host_medium else 'device'} medium) dt={spec.dt} "
@@ -632,6 +671,7 @@ def main():
                          "medium, 1 moment-tensor source, 16 surface receivers"),
             "grid": [nx, ny, nz], "dt": spec.dt, "nm": nm, "na": na, "graph_block": block if use_graph else 0,
             "parallelism": "single GPU, whole domain resident in HBM",
+            "placement": placement,
         },
         "roofline": {
             "bound": "hbm",
@@ -657,6 +697,7 @@ def main():
         "vmax_end": [float(v) for v in vmax],
     }
     sim.close()
+    stripe_off(placement)
     if workload == "production" and not args.no_forms:
         out["forms"] = measure_forms(args, value)
     print(json.dumps(out), flush=True)

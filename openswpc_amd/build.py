@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libswpc_gpu.so")
-SOURCES = ["api.hip", "stencil.hip", "aux.hip", "snap.hip", "green.hip", "batch.hip", "adjoint.hip", "rmedia.hip", "medium.hip"]
+SOURCES = ["api.hip", "stencil.hip", "aux.hip", "snap.hip", "green.hip", "batch.hip", "adjoint.hip", "rmedia.hip", "medium.hip", "arena.hip"]
 # hipFFT for the random-media generator
 LIBS = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -39,9 +39,19 @@ static hipError_t dalloc(swpc_ctx *c, void **p, size_t bytes)
     return hipMemsetAsync(*p, 0, bytes, c->stream);
 }
 
+// a per-cell array: from the striped arena when the device has one (each
+// segment in the HBM region the caller names), else hipMalloc
+static hipError_t dalloc_striped(swpc_ctx *c, void **p, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes)
+{
+    *p = arena_alloc(c->device, bytes, seg_region, nseg, seg_bytes);
+    if (!*p) return dalloc(c, p, bytes);
+    c->bytes += (long long)bytes;
+    return hipMemsetAsync(*p, 0, bytes, c->stream);
+}
+
 static void dfree(void *p)
 {
-    if (p) (void)hipFree(p);
+    if (p && !arena_free(p)) (void)hipFree(p);
 }
 
 static void reset_graph(swpc_ctx *c)
@@ -215,18 +225,39 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     do {                                                                                    \
         if (dalloc(c, (void **)&(ptr), (bytes)) != hipSuccess) { rc = SWPC_ENOMEM; goto bad; } \
     } while (0)
-    for (int q = 0; q < SWPC_NFIELD; ++q) ALLOC(c->fields[q], fb);
-    ALLOC(c->rho, fb);
-    ALLOC(c->lam, fb);
-    ALLOC(c->mu, fb);
-    ALLOC(c->taup, fb);
-    ALLOC(c->taus, fb);
+#define ALLOC_STRIPED(ptr, bytes, reg, nseg, segb)                                                  \
+    do {                                                                                            \
+        if (dalloc_striped(c, (void **)&(ptr), (bytes), (reg), (nseg), (segb)) != hipSuccess) {     \
+            rc = SWPC_ENOMEM;                                                                       \
+            goto bad;                                                                               \
+        }                                                                                           \
+    } while (0)
+    // HBM regions of the per-cell arrays when the device has a striped arena:
+    // the stress update's 18 streams (V, S, lam, mu, taup, taus and the
+    // memory-variable packs) fall 6/6/6 on the three regions, the velocity
+    // update's 10 (V, S, rho) 4/3/3
+    {
+        static const int field_region[SWPC_NFIELD] = {0, 1, 2, 0, 1, 2, 0, 1, 2};  // Vx..Vz, Sxx..Szz, Syz, Sxz, Sxy
+        static const int med_region[5] = {0, 0, 1, 2, 2};                           // rho, lam, mu, taup, taus
+        for (int q = 0; q < SWPC_NFIELD; ++q) ALLOC_STRIPED(c->fields[q], fb, &field_region[q], 1, 0);
+        ALLOC_STRIPED(c->rho, fb, &med_region[0], 1, 0);
+        ALLOC_STRIPED(c->lam, fb, &med_region[1], 1, 0);
+        ALLOC_STRIPED(c->mu, fb, &med_region[2], 1, 0);
+        ALLOC_STRIPED(c->taup, fb, &med_region[3], 1, 0);
+        ALLOC_STRIPED(c->taus, fb, &med_region[4], 1, 0);
+    }
     ALLOC(c->kfs_top, mb);
     ALLOC(c->kfs_bot, mb);
     ALLOC(c->kob_top, mb);
     ALLOC(c->kob_bot, mb);
     ALLOC(c->kob, mb);
-    if (d->nm > 0) ALLOC(c->R, (size_t)6 * d->nm * g.rn * sizeof(float));
+    if (d->nm > 0) {
+        // memory variables: float4 packs of rn cells each, then a float2 pack
+        // (odd 6*nm/4); pack s in region s mod 3
+        static const int pack_region[SWPC_MAXNM * 6 / 4 + 1] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0};
+        const int npack = (6 * d->nm + 3) / 4;
+        ALLOC_STRIPED(c->R, (size_t)6 * d->nm * g.rn * sizeof(float), pack_region, npack, (size_t)g.rn * 16);
+    }
     if (d->abc == SWPC_ABC_PML && d->na > 0) {
         const int na = d->na, nx = d->nx, ny = d->ny, nz = d->nz;
         PmlBox b[5] = {
@@ -246,7 +277,11 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
             c->box[q] = b[q];
         }
         c->npml = off;
-        ALLOC(c->ade, (size_t)A_N * c->npml * sizeof(float));
+        {
+            // auxiliary component c in region c mod 3: the nine of each phase fall 3/3/3
+            static const int aux_region[A_N] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
+            ALLOC_STRIPED(c->ade, (size_t)A_N * c->npml * sizeof(float), aux_region, A_N, (size_t)c->npml * sizeof(float));
+        }
         ALLOC(c->gxc, (size_t)nx * sizeof(float4));
         ALLOC(c->gxe, (size_t)nx * sizeof(float4));
         ALLOC(c->gyc, (size_t)ny * sizeof(float4));
@@ -267,6 +302,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     ALLOC(c->d_vmax, 4 * sizeof(float));
     ALLOC(c->d_it, 4 * sizeof(int));
 #undef ALLOC
+#undef ALLOC_STRIPED
     {
         // 4th-order coefficients of both band states (m_kernel.f90:43-54,106-111)
         const float dx = (float)d->dx, dy = (float)d->dy, dz = (float)d->dz;

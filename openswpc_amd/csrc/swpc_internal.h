@@ -207,6 +207,16 @@ int ctx_fail(swpc_ctx *c, int code, const char *msg);
 // launch arguments changed: drops the captured graphs (api.hip)
 void ctx_reset_graph(swpc_ctx *c);
 
+// ---- striped device arena (arena.hip) ----
+// memory from the arena reserved on the device, or nullptr when there is
+// none or it is full; an array of nseg segments of seg_bytes each puts
+// segment s in region seg_region[s] (nseg = 1: the whole array in
+// seg_region[0])
+bool arena_active(int device);
+void *arena_alloc(int device, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes);
+// returns the block to its arena; false when p is not arena memory
+bool arena_free(void *p);
+
 // ---- launchers (stencil.hip / aux.hip) ----
 // mask: 1 interior, 2 PML, 3 both; the PML kernel goes to pml_stream when
 // given (it updates cells disjoint from the interior kernel's)

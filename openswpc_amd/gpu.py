@@ -133,6 +133,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.swpc_batch_set_packing.argtypes = [vp, i]
     L.swpc_batch_get_packing.argtypes = [vp]
     L.swpc_batch_stream.restype = vp
+    L.swpc_stripe_reserve.argtypes = [i, C.c_int64, i]
+    L.swpc_stripe_release.argtypes = [i]
+    L.swpc_stripe_free_bytes.argtypes = [i, i]
+    L.swpc_stripe_free_bytes.restype = C.c_int64
     _lib = L
     return L
 
@@ -519,4 +523,58 @@ def rmedia_3d(ptype, shape, d, a, epsil, kappa=0.5, seed=1, device=0, out_ptr=No
     if rc != 0:
         raise SwpcError(f"swpc_rmedia_3d failed ({rc}): {L.swpc_rmedia_last_error().decode()}")
     return out
+
+
+# ---- device memory striped over the HBM regions (arena.hip) ----
+HBM_REGIONS = 3   # MI355X: 288 GiB in three 96 GiB regions
+
+
+def stripe_reserve(bytes_per_region, device=0, nregions=HBM_REGIONS):
+    """Reserve bytes_per_region of device memory in each HBM region; contexts
+    created on the device afterwards take their per-cell arrays from it, each
+    array in a fixed region (swpc_stripe_reserve)."""
+    L = load_library()
+    rc = L.swpc_stripe_reserve(int(device), int(bytes_per_region), int(nregions))
+    if rc != 0:
+        raise SwpcError(f"swpc_stripe_reserve failed ({rc})")
+
+
+def stripe_release(device=0):
+    """Return the arena's memory; fails while a context still uses it."""
+    L = load_library()
+    rc = L.swpc_stripe_release(int(device))
+    if rc != 0:
+        raise SwpcError(f"swpc_stripe_release failed ({rc}): arena memory still in use")
+
+
+def stripe_free_bytes(device=0, region=0):
+    """Bytes left in a region of the device's arena (0 without an arena)."""
+    return int(load_library().swpc_stripe_free_bytes(int(device), int(region)))
+
+
+def stripe_bytes_per_region(nx, ny, nz, nm, na=0, nregions=HBM_REGIONS, margin=1.05):
+    """Arena bytes per region that hold one context's per-cell arrays:
+    9 fields and 5 medium arrays of (nx+6+32-aligned) x (ny+6) x (nz+6)
+    floats, 6*nm memory variables per interior cell, 18 auxiliaries per PML
+    cell; the largest region share, with 256 MiB granules and a 5 % margin
+    (add a granule or two of slack for a single context)."""
+    pitch = -(-(32 + nx + 3) // 32) * 32
+    n = pitch * (ny + 6) * (nz + 6) * 4
+    rn = -(-nx // 32) * 32 * max(ny - 2 * na, 0) * max(nz - na, 0) * 4
+    npml = nx * ny * nz - max(nx - 2 * na, 0) * max(ny - 2 * na, 0) * max(nz - na, 0) if na > 0 else 0
+    gran = 256 << 20
+    # arrays of a granule or more take whole granules; smaller ones share slabs
+    up = lambda b: -(-int(b) // gran) * gran if b >= gran else -(-int(b) // 65536) * 65536  # noqa: E731
+    per = [0] * nregions
+    # fields and medium: 14 arrays; regions as the library assigns them
+    for r in (0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 0, 1, 2, 2):
+        per[r % nregions] += up(n)
+    npack = (6 * nm + 3) // 4
+    if nm > 0:
+        for s in range(npack):
+            per[s % nregions] += up(rn * (4 if 4 * (s + 1) <= 6 * nm else 2))
+    if npml:
+        for c in range(18):
+            per[c % nregions] += npml * 4
+    return int(max(per) * margin)
 

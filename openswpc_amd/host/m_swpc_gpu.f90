@@ -343,6 +343,21 @@ module m_swpc_gpu
             import :: c_int64_t, c_ptr
             type(c_ptr), value :: ctx
         end function
+        ! per-cell arrays striped over the HBM regions (call before swpc_create)
+        integer(c_int) function swpc_stripe_reserve(device, bytes_per_region, nregions) &
+                bind(C, name='swpc_stripe_reserve')
+            import :: c_int, c_int64_t
+            integer(c_int), value :: device, nregions
+            integer(c_int64_t), value :: bytes_per_region
+        end function
+        integer(c_int) function swpc_stripe_release(device) bind(C, name='swpc_stripe_release')
+            import :: c_int
+            integer(c_int), value :: device
+        end function
+        integer(c_int64_t) function swpc_stripe_free_bytes(device, region) bind(C, name='swpc_stripe_free_bytes')
+            import :: c_int, c_int64_t
+            integer(c_int), value :: device, region
+        end function
         integer(c_int) function swpc_set_tuning(ctx, kchunk, by, variant) bind(C, name='swpc_set_tuning')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

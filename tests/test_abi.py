@@ -97,6 +97,11 @@ def test_null_context_is_rejected():
     assert L.swpc_staging_buffer(None, C.byref(n)) is None and n.value == 0
     assert L.swpc_batch_set_packing(None, 1) == -1 and L.swpc_batch_get_packing(None) == -1
     assert L.swpc_kernel_vmax_noalloc(None, None) == -1
+    # the striped arena validates its arguments before touching a device, and
+    # reports an empty arena where none was reserved
+    assert L.swpc_stripe_reserve(0, 0, 3) == -1 and L.swpc_stripe_reserve(0, 1 << 30, 0) == -1
+    assert L.swpc_stripe_reserve(0, 1 << 30, 9) == -1
+    assert L.swpc_stripe_free_bytes(0, 0) == 0 and L.swpc_stripe_release(0) == 0
     f2 = np.zeros(2, np.float32)
     i1 = np.zeros(1, np.int32)
     assert L.swpc_medium_layered(None, f2, None, f2, f2, i1, i1, f2) == -1

@@ -1,0 +1,104 @@
+"""Per-cell arrays striped over the HBM regions (swpc_stripe_reserve).
+
+The arena changes where the arrays live, not what the kernels compute: a
+context whose arrays come from the arena must step to the same bits as the
+FP32 oracle (small grid: every array sub-allocated from the regions' slabs)
+and as a context with hipMalloc placement (larger grids: fields and medium
+arrays in whole 256 MiB granules, the memory-variable packs and the PML
+auxiliaries split over the three regions; and an arena too small for the
+grid, so that some arrays fall back to hipMalloc).  The bookkeeping: arena
+memory is taken while a context holds it and comes back when the context is
+destroyed, and the arena refuses to be released while in use."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_helpers import compare_fields, gpu_from_model, small_config
+
+pytestmark = pytest.mark.gpu
+GRAN = 256 << 20
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _free():
+    from openswpc_amd.gpu import stripe_free_bytes
+    return [stripe_free_bytes(0, r) for r in range(3)]
+
+
+def test_small_grid_in_the_arena_equals_the_oracle():
+    from openswpc_amd.gpu import SwpcError, stripe_release, stripe_reserve
+    cfg = small_config(nm=3)            # 18 memory variables: four float4 packs and a float2 pack
+    m = O.build_model(cfg)
+    run = O.OracleRun(m, 32)
+    run.run(30)
+    stripe_reserve(GRAN)
+    try:
+        free0 = _free()
+        assert all(f == GRAN for f in free0), free0
+        g = gpu_from_model(m)
+        try:
+            taken = [a - b for a, b in zip(free0, _free())]
+            print("arena bytes taken per region", taken, flush=True)
+            assert all(t > 0 for t in taken), taken
+            with pytest.raises(SwpcError):
+                stripe_release()
+            g.step(1, 30, use_graph=True, block=10)
+            g.synchronize()
+            res = compare_fields(g, run)
+            print(res, flush=True)
+            for n, (ulp, rel) in res.items():
+                assert ulp == 0, (n, ulp, rel)
+        finally:
+            g.close()
+        assert _free() == free0
+    finally:
+        stripe_release()
+
+
+def _spec(nx, ny, nz):
+    from openswpc_amd.host import ModelSpec
+    src = [dict(x=0.4, y=-0.7, z=6.3, t0=0.0, tr=0.8, mo=1e15, strike=30.0, dip=60.0, rake=90.0)]
+    st = [(2.1, 1.3, 0.0, "fsb"), (-3.2, 2.7, 0.0, "fsb"), (1.5, 0.5, 3.0, "dep")]
+    return ModelSpec(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, dt=0.01, nt=40, na=20, nm=3, zbeg=-1.0,
+                     vmodel="lhm", sources=src, stations=st, wav_what=3, fq_min=0.05, fq_max=5.0, fq_ref=1.0,
+                     layers=[[0.0, 2.4, 4.5, 2.6, 200.0, 100.0], [4.0, 2.6, 5.6, 3.2, 300.0, 150.0]])
+
+
+# (grid, arena bytes per region): 512x512x256 fits (fields of 312 MB take
+# whole granules, the 4.1 GB of memory variables five packs over the three
+# regions); 256x256x320 in one granule per region does not (fields of 109 MB
+# share slabs until they are full, the rest falls back to hipMalloc)
+@pytest.mark.parametrize("shape,per_region", [((512, 512, 256), None), ((256, 256, 320), GRAN)])
+def test_striped_context_equals_hipmalloc_placement(shape, per_region):
+    from openswpc_amd.gpu import stripe_bytes_per_region, stripe_release, stripe_reserve
+    from openswpc_amd.host import build_inputs, make_simulation
+    spec = _spec(*shape)
+    inp = build_inputs(spec)
+    plain = make_simulation(inp)                # before the arena: hipMalloc
+    sims = [plain]
+    per = per_region or stripe_bytes_per_region(spec.nx, spec.ny, spec.nz, spec.nm, spec.na) + GRAN
+    stripe_reserve(per)
+    try:
+        free0 = _free()
+        striped = make_simulation(inp)
+        sims.append(striped)
+        taken = sum(a - b for a, b in zip(free0, _free()))
+        print(shape, "arena bytes taken", taken, "of", striped.device_bytes(), flush=True)
+        if per_region is None:
+            assert taken >= 0.9 * striped.device_bytes()
+        else:
+            assert 0 < taken <= 3 * GRAN
+        for s in sims:
+            s.step(1, 40, use_graph=True, block=20)
+        for n in ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy"):
+            a, b = striped.get_field(n), plain.get_field(n)
+            assert np.array_equal(_bits(a), _bits(b)), (shape, n)
+        assert np.any(plain.get_field("Vz"))
+        assert np.array_equal(_bits(striped.seismograms("vel")), _bits(plain.seismograms("vel")))
+    finally:
+        for s in sims:
+            s.close()
+        stripe_release()
