@@ -78,6 +78,13 @@ def stripe_off(placement):
         stripe_release()
 
 
+def device_free_bytes():
+    h = C.CDLL("libamdhip64.so")
+    fr, tot = C.c_size_t(), C.c_size_t()
+    assert h.hipMemGetInfo(C.byref(fr), C.byref(tot)) == 0
+    return int(fr.value)
+
+
 # ---------------------------------------------------------------------------
 # runs of the workloads (openswpc_amd/workloads.py)
 # ---------------------------------------------------------------------------
@@ -200,6 +207,9 @@ def measure_adjoint(nt, block):
     from openswpc_amd.host import build_inputs, make_simulation
     t0 = time.perf_counter()
     true_spec, init_spec, info = adjoint_specs(nt=nt)
+    # contexts, adjoint workspace and checkpoints (45 states and 45 step
+    # records, about 75 GB) from the striped arena
+    placement = stripe_on(30 << 30)
     sim_true = make_simulation(build_inputs(true_spec))
     ev = HipEvents()
     e0, e1 = ev.create(), ev.create()
@@ -215,8 +225,10 @@ def measure_adjoint(nt, block):
     J, gl, gm, gr = sim.adjoint_gradient(nt, obs, ckp=45)
     wall = time.perf_counter() - t
     sim.close()
+    stripe_off(placement)
     inner = (slice(3, -3),) * 3
     return dict(cells=init_spec.nx * init_spec.ny * init_spec.nz, nt=nt, wall=wall, ms_fwd=ms_fwd, misfit=J,
+                placement=placement,
                 sign=info["sign"], grad_norms={"lam": float(np.linalg.norm(gl[inner])),
                                                "mu": float(np.linalg.norm(gm[inner])),
                                                "rho": float(np.linalg.norm(gr[inner]))})
@@ -230,7 +242,7 @@ def adjoint_summary(r):
             "ms_per_step": round(wall * 1e3 / nt, 4), "value": round(cells * nt / wall / 1e9, 4),
             "unit": "Gcell-updates/s of gradient (forward, recomputation and adjoint of every step)",
             "forward_ms_per_step": round(ms_fwd, 4), "forward_value": round(cells / (ms_fwd * 1e-3) / 1e9, 4),
-            "gradient_over_forward_time": round(wall * 1e3 / nt / ms_fwd, 3)}
+            "gradient_over_forward_time": round(wall * 1e3 / nt / ms_fwd, 3), "placement": r["placement"]}
 
 
 def run_adjoint(args):
@@ -266,6 +278,9 @@ def measure_adjoint_multi(nt, ns, block):
     from openswpc_amd.host import build_inputs, make_simulation, set_acquisition
     geo = adjoint_multi_geometry()
     t0 = time.perf_counter()
+    # the checkpoint planner fills the device: the arena keeps all but 8 GiB
+    # of the free memory, and the planner counts what it keeps free
+    placement = stripe_on((device_free_bytes() - (8 << 30)) // 3)
     spec_true = adjoint_multi_spec(geo, nt=nt, true=True)
     inp_true = build_inputs(spec_true)
     sim = make_simulation(inp_true)
@@ -303,9 +318,10 @@ def measure_adjoint_multi(nt, ns, block):
     wall = time.perf_counter() - t
     held = sim.device_bytes()
     sim.close()
+    stripe_off(placement)
     inner = (slice(3, -3),) * 3
     return dict(geo=geo, cells=spec.nx * spec.ny * spec.nz, nt=nt, ns=ns, wall=wall, ms_fwd=ms_fwd, c1=c1, c2=c2,
-                nbytes=nbytes, nfwd=nfwd, held=held, misfits=misfits,
+                nbytes=nbytes, nfwd=nfwd, held=held, misfits=misfits, placement=placement,
                 grad_norms={"lam": float(np.linalg.norm(gl[inner])), "mu": float(np.linalg.norm(gm[inner])),
                             "rho": float(np.linalg.norm(gr[inner]))})
 
@@ -320,7 +336,7 @@ def adjoint_multi_summary(r):
             "checkpoints": {"outer": r["c1"], "inner": r["c2"], "device_bytes": r["nbytes"],
                             "forward_steps": r["nfwd"]},
             "forward_ms_per_step": round(ms_fwd, 4), "forward_value": round(cells / (ms_fwd * 1e-3) / 1e9, 4),
-            "gradient_over_forward_time": round(wall * 1e3 / (nt * ns) / ms_fwd, 3)}
+            "gradient_over_forward_time": round(wall * 1e3 / (nt * ns) / ms_fwd, 3), "placement": r["placement"]}
 
 
 def run_adjoint_multi(args):

@@ -636,18 +636,22 @@ namespace {
 void wfree(swpc_ctx *c, float *&p, long long n)
 {
     if (p) {
-        (void)hipFree(p);
+        if (!arena_free(p)) (void)hipFree(p);
         c->adj_ws->bytes -= n * (long long)sizeof(float);
         c->bytes -= n * (long long)sizeof(float);
     }
     p = nullptr;
 }
 
-bool walloc(swpc_ctx *c, float *&p, long long n)
+// n floats; with a striped arena on the device, in segments of seg floats
+// that rotate over its HBM regions (seg = nf: one field per segment, so the
+// fields a kernel streams together sit in different regions), else hipMalloc
+bool walloc(swpc_ctx *c, float *&p, long long n, long long seg = 0)
 {
     p = nullptr;
     if (n <= 0) return true;
-    if (hipMalloc((void **)&p, (size_t)n * sizeof(float)) != hipSuccess) {
+    p = (float *)arena_alloc(c->device, (size_t)n * sizeof(float), nullptr, 0, (size_t)seg * sizeof(float));
+    if (!p && hipMalloc((void **)&p, (size_t)n * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         p = nullptr;
         return false;
@@ -658,12 +662,12 @@ bool walloc(swpc_ctx *c, float *&p, long long n)
 }
 
 // a store of at least n floats (contents not kept)
-bool wgrow(swpc_ctx *c, float *&p, long long &have, long long n)
+bool wgrow(swpc_ctx *c, float *&p, long long &have, long long n, long long seg = 0)
 {
     if (have >= n && p) return true;
     wfree(c, p, have);
     have = 0;
-    if (!walloc(c, p, n)) return false;
+    if (!walloc(c, p, n, seg)) return false;
     have = n;
     return true;
 }
@@ -676,9 +680,9 @@ int ws_open(swpc_ctx *c)
     AdjWorkspace &w = *c->adj_ws;
     w.z = sizes_of(c);
     const long long nf = w.z.nf;
-    if (!walloc(c, w.adjF, 9 * nf) || !walloc(c, w.adjA, w.z.na18) || !walloc(c, w.dens, 9 * nf) ||
-        !walloc(c, w.qq, 6 * nf) || !walloc(c, w.gsrc, 3 * nf) || !walloc(c, w.grad, 3 * nf) ||
-        !walloc(c, w.cls, w.z.cls()) || !walloc(c, w.adjR, w.z.nR)) {
+    if (!walloc(c, w.adjF, 9 * nf, nf) || !walloc(c, w.adjA, w.z.na18, c->npml) || !walloc(c, w.dens, 9 * nf, nf) ||
+        !walloc(c, w.qq, 6 * nf, nf) || !walloc(c, w.gsrc, 3 * nf, nf) || !walloc(c, w.grad, 3 * nf, nf) ||
+        !walloc(c, w.cls, w.z.cls()) || !walloc(c, w.adjR, w.z.nR, c->g.rn)) {
         swpc_adj_release(c);
         return ctx_fail(c, SWPC_ENOMEM, "adjoint: device memory for the adjoint fields");
     }
@@ -729,7 +733,15 @@ extern "C" int swpc_adj_plan(swpc_ctx *c, int nt, int64_t budget, int *ckp_outer
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
         if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) return SWPC_EHIP;
-        budget = (int64_t)fr + (c->adj_ws ? c->adj_ws->bytes : 0) - (1LL << 30);
+        // each store comes from one pool: the striped arena when the device
+        // has one and it holds the store, else hipMalloc; plan within the
+        // larger of the two, less the arena's granule rounding (under one
+        // 256 MiB granule per allocation, sixteen granules in all)
+        const int64_t held = c->adj_ws ? c->adj_ws->bytes : 0;
+        const int64_t arena = arena_granule_bytes(c->device);
+        const int64_t from_free = (int64_t)fr + held - (1LL << 30);
+        const int64_t from_arena = arena > 0 ? arena + held - (16LL << 28) : 0;
+        budget = from_free > from_arena ? from_free : from_arena;
     }
     const AdjSizes z = sizes_of(c);
     const long long cap = budget / (long long)sizeof(float);
@@ -784,7 +796,8 @@ extern "C" int swpc_adj_accumulate(swpc_ctx *c, int nt, const float *obs, int ck
     const long long nck = scheme_ck_slots(nt, c1, c2), nout = (nt + c1 - 1) / c1;
     const long long nobs = (long long)c->ntw * 3 * c->nst;
     const float *seg_before = w.seg;
-    if (!wgrow(c, w.ck, w.ck_n, nck * per_state) || !wgrow(c, w.seg, w.seg_n, scheme_seg_slots(nt, c1, c2) * sstep) ||
+    if (!wgrow(c, w.ck, w.ck_n, nck * per_state, nf) ||
+        !wgrow(c, w.seg, w.seg_n, scheme_seg_slots(nt, c1, c2) * sstep, nf) ||
         !wgrow(c, w.obs, w.obs_n, nobs))
         return ctx_fail(c, SWPC_ENOMEM, "swpc_adj_accumulate: device memory for the checkpoints "
                                         "(swpc_adj_plan finds a scheme that fits)");

@@ -94,17 +94,25 @@ hipError_t map_range(Arena &a, char *va, const hipMemGenericAllocationHandle_t *
 void *alloc_big(Arena &a, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes)
 {
     const size_t ng = (bytes + a.gran - 1) / a.gran;
-    std::vector<std::pair<int, hipMemGenericAllocationHandle_t>> take;
-    take.reserve(ng);
     size_t need[kMaxRegions] = {};
     std::vector<int> rk(ng);
+    size_t avail[kMaxRegions] = {};
+    for (int r = 0; r < a.nreg; ++r) avail[r] = a.reg[r].granules.size();
     for (size_t k = 0; k < ng; ++k) {
-        const size_t seg = (nseg > 1 && seg_bytes > 0) ? std::min<size_t>(k * a.gran / seg_bytes, (size_t)nseg - 1) : 0;
-        rk[k] = ((seg_region[seg] % a.nreg) + a.nreg) % a.nreg;
-        ++need[rk[k]];
+        // the granule's segment names its region (no table: segments rotate
+        // over the regions); a region that has run out yields to the region
+        // with the most granules left
+        const size_t seg = seg_bytes > 0 ? k * a.gran / seg_bytes : 0;
+        int r = seg_region ? seg_region[std::min<size_t>(seg, (size_t)nseg - 1)] : (int)(seg % (size_t)a.nreg);
+        r = ((r % a.nreg) + a.nreg) % a.nreg;
+        if (avail[r] == 0)
+            for (int q = 0; q < a.nreg; ++q)
+                if (avail[q] > avail[r]) r = q;
+        if (avail[r] == 0) return nullptr;
+        --avail[r];
+        rk[k] = r;
+        ++need[r];
     }
-    for (int r = 0; r < a.nreg; ++r)
-        if (need[r] > a.reg[r].granules.size()) return nullptr;
     void *va = nullptr;
     if (hipMemAddressReserve(&va, ng * a.gran, 0, nullptr, 0) != hipSuccess) {
         (void)hipGetLastError();
@@ -201,13 +209,23 @@ bool arena_active(int device)
 
 void *arena_alloc(int device, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes)
 {
-    if (bytes == 0 || !seg_region || nseg < 1) return nullptr;
+    if (bytes == 0 || (seg_region && nseg < 1)) return nullptr;
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_arena.find(device);
     if (it == g_arena.end()) return nullptr;
     Arena &a = *it->second;
     if (bytes >= a.gran) return alloc_big(a, bytes, seg_region, nseg, seg_bytes);
-    return alloc_small(a, bytes, seg_region[0]);
+    return alloc_small(a, bytes, seg_region ? seg_region[0] : 0);
+}
+
+int64_t arena_granule_bytes(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_arena.find(device);
+    if (it == g_arena.end()) return 0;
+    int64_t b = 0;
+    for (int r = 0; r < it->second->nreg; ++r) b += (int64_t)(it->second->reg[r].granules.size() * it->second->gran);
+    return b;
 }
 
 bool arena_free(void *p)

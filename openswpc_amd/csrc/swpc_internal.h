@@ -211,9 +211,12 @@ void ctx_reset_graph(swpc_ctx *c);
 // memory from the arena reserved on the device, or nullptr when there is
 // none or it is full; an array of nseg segments of seg_bytes each puts
 // segment s in region seg_region[s] (nseg = 1: the whole array in
-// seg_region[0])
+// seg_region[0]; seg_region = nullptr: segment s in region s mod the
+// region count)
 bool arena_active(int device);
 void *arena_alloc(int device, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes);
+// bytes of whole free granules in the device's arena (0 without one)
+int64_t arena_granule_bytes(int device);
 // returns the block to its arena; false when p is not arena memory
 bool arena_free(void *p);
 

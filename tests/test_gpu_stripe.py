@@ -102,3 +102,36 @@ def test_striped_context_equals_hipmalloc_placement(shape, per_region):
         for s in sims:
             s.close()
         stripe_release()
+
+
+def test_adjoint_gradient_in_the_arena_equals_hipmalloc_placement():
+    """The adjoint workspace and its checkpoint stores come from the arena
+    too (in segments rotating over the regions): the misfit and the three
+    gradients are the same bits as with hipMalloc placement."""
+    import dataclasses
+    from openswpc_amd.gpu import stripe_release, stripe_reserve
+    from openswpc_amd.host import build_inputs, make_simulation
+    spec = dataclasses.replace(_spec(96, 88, 72), nt=60)
+    true = make_simulation(build_inputs(dataclasses.replace(spec, layers=[[0.0, 2.4, 4.7, 2.7, 200.0, 100.0],
+                                                                           [4.0, 2.6, 5.6, 3.2, 300.0, 150.0]])))
+    true.step(1, 60, use_graph=True, block=20)
+    obs = true.seismograms("vel")
+    true.close()
+    inp = build_inputs(spec)
+    plain = make_simulation(inp)
+    J0, *g0 = plain.adjoint_gradient(60, obs, ckp=8)
+    plain.close()
+    stripe_reserve(2 << 30)
+    try:
+        striped = make_simulation(inp)
+        try:
+            J1, *g1 = striped.adjoint_gradient(60, obs, ckp=8)
+        finally:
+            striped.close()
+    finally:
+        stripe_release()
+    print("misfit", J0, J1, flush=True)
+    assert J0 == J1 and J0 > 0
+    for a, b in zip(g0, g1):
+        assert np.array_equal(_bits(a), _bits(b))
+        assert np.any(a)
