@@ -67,6 +67,21 @@ struct Arena {
 std::mutex g_mu;
 std::map<int, Arena *> g_arena;
 
+// makes a device current for a scope and restores the caller's afterwards
+struct DeviceScope {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceScope(int device)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        err = hipSetDevice(device);
+    }
+    ~DeviceScope()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
 hipMemAllocationProp device_prop(int device)
 {
     hipMemAllocationProp prop = {};
@@ -238,7 +253,7 @@ bool arena_free(void *p)
         if (it == a.live.end()) continue;
         Live &L = it->second;
         // kernels still in flight may read the block: wait for the device
-        (void)hipSetDevice(a.device);
+        DeviceScope ds(a.device);
         (void)hipDeviceSynchronize();
         if (L.big) {
             (void)hipMemUnmap(p, L.bytes);
@@ -274,7 +289,8 @@ extern "C" int swpc_stripe_reserve(int device, int64_t bytes_per_region, int nre
     if (nregions < 1 || nregions > kMaxRegions || bytes_per_region <= 0) return SWPC_EINVAL;
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_arena.count(device)) return SWPC_ESTATE;
-    if (hipSetDevice(device) != hipSuccess) return SWPC_EHIP;
+    DeviceScope ds(device);
+    if (ds.err != hipSuccess) return SWPC_EHIP;
     int vmm = 0;
     if (hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, device) != hipSuccess || !vmm)
         return SWPC_EHIP;
@@ -334,7 +350,7 @@ extern "C" int swpc_stripe_release(int device)
     if (it == g_arena.end()) return SWPC_OK;
     Arena *a = it->second;
     if (!a->live.empty()) return SWPC_ESTATE;
-    (void)hipSetDevice(device);
+    DeviceScope ds(device);
     (void)hipDeviceSynchronize();
     release_all(*a);
     delete a;

@@ -28,7 +28,7 @@ def _free():
     return [stripe_free_bytes(0, r) for r in range(3)]
 
 
-def test_small_grid_in_the_arena_equals_the_oracle():
+def test_small_grid_in_the_arena_equals_the_oracle(orc):
     from openswpc_amd.gpu import SwpcError, stripe_release, stripe_reserve
     cfg = small_config(nm=3)            # 18 memory variables: four float4 packs and a float2 pack
     m = O.build_model(cfg)
@@ -135,3 +135,35 @@ def test_adjoint_gradient_in_the_arena_equals_hipmalloc_placement():
     for a, b in zip(g0, g1):
         assert np.array_equal(_bits(a), _bits(b))
         assert np.any(a)
+
+
+def test_batch_of_arena_members_matches_their_oracles(orc):
+    """Batch members whose arrays come from the arena's slabs (PML and Cerjan
+    members mixed, packing 2, graph replays) end bit-identical to their own
+    oracle runs."""
+    from openswpc_amd.gpu import SwpcBatch, stripe_release, stripe_reserve
+    from tests.test_gpu_batch import _check_members, _members
+    nt = 18
+    models = [O.build_model(c) for c in _members()]
+    runs = []
+    for m in models:
+        r = O.OracleRun(m, 32)
+        r.run(nt)
+        runs.append(r)
+    stripe_reserve(GRAN)
+    gs = []
+    try:
+        free0 = _free()
+        gs = [gpu_from_model(m) for m in models]
+        assert all(a > b for a, b in zip(free0, _free()))
+        b = SwpcBatch(gs)
+        b.set_packing(2)
+        b.step(1, 12, use_graph=True, block=4)
+        b.step(13, nt - 12, use_graph=True, block=3)
+        b.synchronize()
+        _check_members(gs, runs)
+        b.close()
+    finally:
+        for g in gs:
+            g.close()
+        stripe_release()
