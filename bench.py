@@ -572,12 +572,15 @@ def main():
                     help="build the medium with the host library and upload it, instead of on the device")
     ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per kernel with events")
     ap.add_argument("--no-stripe", action="store_true",
-                    help="plain hipMalloc placement instead of the arrays striped over the HBM regions")
+                    help="plain hipMalloc placement instead of the arrays striped over the HBM regions "
+                         "(no arena reserved, SWPC_STRIPE=0 for the library)")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--by", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
     args = ap.parse_args()
     STRIPE["on"] = not args.no_stripe
+    if args.no_stripe:
+        os.environ["SWPC_STRIPE"] = "0"     # no private arena for large contexts either
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this library runs on one GPU only: the whole domain lives on one "

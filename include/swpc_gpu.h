@@ -393,7 +393,10 @@ int64_t swpc_device_bytes(const swpc_ctx *ctx);
    Results do not depend on the placement.  swpc_stripe_release returns
    SWPC_ESTATE while any context still holds arena memory;
    swpc_stripe_free_bytes reports what is left in a region (0 without an
-   arena). */
+   arena).  Without a reserved arena, swpc_create gives a context whose
+   per-cell arrays come to 4 GiB or more a private arena sized for it (only
+   that context draws from it; it goes at swpc_destroy); SWPC_STRIPE=0 in the
+   environment turns that off. */
 int swpc_stripe_reserve(int device, int64_t bytes_per_region, int nregions);
 int swpc_stripe_release(int device);
 int64_t swpc_stripe_free_bytes(int device, int region);

@@ -650,7 +650,7 @@ bool walloc(swpc_ctx *c, float *&p, long long n, long long seg = 0)
 {
     p = nullptr;
     if (n <= 0) return true;
-    p = (float *)arena_alloc(c->device, (size_t)n * sizeof(float), nullptr, 0, (size_t)seg * sizeof(float));
+    p = (float *)arena_alloc(c->device, c, (size_t)n * sizeof(float), nullptr, 0, (size_t)seg * sizeof(float));
     if (!p && hipMalloc((void **)&p, (size_t)n * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         p = nullptr;
@@ -738,7 +738,7 @@ extern "C" int swpc_adj_plan(swpc_ctx *c, int nt, int64_t budget, int *ckp_outer
         // larger of the two, less the arena's granule rounding (under one
         // 256 MiB granule per allocation, sixteen granules in all)
         const int64_t held = c->adj_ws ? c->adj_ws->bytes : 0;
-        const int64_t arena = arena_granule_bytes(c->device);
+        const int64_t arena = arena_granule_bytes(c->device, c);
         const int64_t from_free = (int64_t)fr + held - (1LL << 30);
         const int64_t from_arena = arena > 0 ? arena + held - (16LL << 28) : 0;
         budget = from_free > from_arena ? from_free : from_arena;

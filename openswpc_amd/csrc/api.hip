@@ -1,5 +1,6 @@
 // api.hip -- C ABI of libswpc_gpu (declared in include/swpc_gpu.h).
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -43,7 +44,7 @@ static hipError_t dalloc(swpc_ctx *c, void **p, size_t bytes)
 // segment in the HBM region the caller names), else hipMalloc
 static hipError_t dalloc_striped(swpc_ctx *c, void **p, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes)
 {
-    *p = arena_alloc(c->device, bytes, seg_region, nseg, seg_bytes);
+    *p = arena_alloc(c->device, c, bytes, seg_region, nseg, seg_bytes);
     if (!*p) return dalloc(c, p, bytes);
     c->bytes += (long long)bytes;
     return hipMemsetAsync(*p, 0, bytes, c->stream);
@@ -52,6 +53,48 @@ static hipError_t dalloc_striped(swpc_ctx *c, void **p, size_t bytes, const int 
 static void dfree(void *p)
 {
     if (p && !arena_free(p)) (void)hipFree(p);
+}
+
+// HBM regions of the per-cell arrays when the device has a striped arena:
+// the stress update's 18 streams (V, S, lam, mu, taup, taus and the
+// memory-variable packs) fall 6/6/6 on the three regions, the velocity
+// update's 10 (V, S, rho) 4/3/3; PML auxiliary component c in region c mod 3
+// (the nine of each phase 3/3/3)
+static const int kFieldRegion[SWPC_NFIELD] = {0, 1, 2, 0, 1, 2, 0, 1, 2};  // Vx..Vz, Sxx..Szz, Syz, Sxz, Sxy
+static const int kMedRegion[5] = {0, 0, 1, 2, 2};                           // rho, lam, mu, taup, taus
+static const int kPackRegion[SWPC_MAXNM * 6 / 4 + 1] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0};
+static const int kAuxRegion[A_N] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
+static const int64_t kGranuleBytes = (int64_t)256 << 20;                    // the arena's granule
+static const int64_t kAutoStripeMin = (int64_t)4 << 30;                     // contexts this large stripe by themselves
+static const int64_t kStripeSlack = 2 * kGranuleBytes;
+
+// the bytes a context's arena arrays take in their busiest region: arrays of
+// a granule or more in whole granules (a segmented array's granules in the
+// regions of their segments), smaller ones in 64 KiB steps
+static int64_t busiest_region_bytes(const swpc_desc *d, const Grid &g)
+{
+    int64_t per[3] = {0, 0, 0};
+    auto add = [&](int64_t bytes, const int *reg, int nseg, int64_t seg) {
+        if (bytes < kGranuleBytes) {
+            per[reg[0] % 3] += (bytes + 65535) / 65536 * 65536;
+            return;
+        }
+        const int64_t ng = (bytes + kGranuleBytes - 1) / kGranuleBytes;
+        for (int64_t k = 0; k < ng; ++k) {
+            const int64_t sg = seg > 0 ? std::min<int64_t>(k * kGranuleBytes / seg, nseg - 1) : 0;
+            per[reg[sg] % 3] += kGranuleBytes;
+        }
+    };
+    const int64_t fb = g.n * (int64_t)sizeof(float);
+    for (int q = 0; q < SWPC_NFIELD; ++q) add(fb, &kFieldRegion[q], 1, 0);
+    for (int q = 0; q < 5; ++q) add(fb, &kMedRegion[q], 1, 0);
+    if (d->nm > 0) add((int64_t)6 * d->nm * g.rn * 4, kPackRegion, (6 * d->nm + 3) / 4, g.rn * 16);
+    if (d->abc == SWPC_ABC_PML && d->na > 0) {
+        const int64_t na = d->na, nx = d->nx, ny = d->ny, nz = d->nz;
+        const int64_t npml = 2 * nx * na * nz + 2 * na * (ny - 2 * na) * nz + (int64_t)g.rpitch * (ny - 2 * na) * na;
+        add((int64_t)A_N * npml * 4, kAuxRegion, A_N, npml * 4);
+    }
+    return std::max(per[0], std::max(per[1], per[2]));
 }
 
 static void reset_graph(swpc_ctx *c)
@@ -232,20 +275,21 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
             goto bad;                                                                               \
         }                                                                                           \
     } while (0)
-    // HBM regions of the per-cell arrays when the device has a striped arena:
-    // the stress update's 18 streams (V, S, lam, mu, taup, taus and the
-    // memory-variable packs) fall 6/6/6 on the three regions, the velocity
-    // update's 10 (V, S, rho) 4/3/3
+    // a large context on a device without an arena reserves one of its own
+    // (SWPC_STRIPE=0 in the environment turns this off)
     {
-        static const int field_region[SWPC_NFIELD] = {0, 1, 2, 0, 1, 2, 0, 1, 2};  // Vx..Vz, Sxx..Szz, Syz, Sxz, Sxy
-        static const int med_region[5] = {0, 0, 1, 2, 2};                           // rho, lam, mu, taup, taus
-        for (int q = 0; q < SWPC_NFIELD; ++q) ALLOC_STRIPED(c->fields[q], fb, &field_region[q], 1, 0);
-        ALLOC_STRIPED(c->rho, fb, &med_region[0], 1, 0);
-        ALLOC_STRIPED(c->lam, fb, &med_region[1], 1, 0);
-        ALLOC_STRIPED(c->mu, fb, &med_region[2], 1, 0);
-        ALLOC_STRIPED(c->taup, fb, &med_region[3], 1, 0);
-        ALLOC_STRIPED(c->taus, fb, &med_region[4], 1, 0);
+        const char *e = getenv("SWPC_STRIPE");
+        const int64_t busiest = busiest_region_bytes(d, g);
+        if (!(e && e[0] == '0') && !arena_active(c->device) && busiest >= kAutoStripeMin / 3 &&
+            arena_reserve(c->device, busiest + kStripeSlack, 3, c) == SWPC_OK)
+            c->own_arena = 1;
     }
+    for (int q = 0; q < SWPC_NFIELD; ++q) ALLOC_STRIPED(c->fields[q], fb, &kFieldRegion[q], 1, 0);
+    ALLOC_STRIPED(c->rho, fb, &kMedRegion[0], 1, 0);
+    ALLOC_STRIPED(c->lam, fb, &kMedRegion[1], 1, 0);
+    ALLOC_STRIPED(c->mu, fb, &kMedRegion[2], 1, 0);
+    ALLOC_STRIPED(c->taup, fb, &kMedRegion[3], 1, 0);
+    ALLOC_STRIPED(c->taus, fb, &kMedRegion[4], 1, 0);
     ALLOC(c->kfs_top, mb);
     ALLOC(c->kfs_bot, mb);
     ALLOC(c->kob_top, mb);
@@ -254,9 +298,8 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     if (d->nm > 0) {
         // memory variables: float4 packs of rn cells each, then a float2 pack
         // (odd 6*nm/4); pack s in region s mod 3
-        static const int pack_region[SWPC_MAXNM * 6 / 4 + 1] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0};
         const int npack = (6 * d->nm + 3) / 4;
-        ALLOC_STRIPED(c->R, (size_t)6 * d->nm * g.rn * sizeof(float), pack_region, npack, (size_t)g.rn * 16);
+        ALLOC_STRIPED(c->R, (size_t)6 * d->nm * g.rn * sizeof(float), kPackRegion, npack, (size_t)g.rn * 16);
     }
     if (d->abc == SWPC_ABC_PML && d->na > 0) {
         const int na = d->na, nx = d->nx, ny = d->ny, nz = d->nz;
@@ -277,11 +320,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
             c->box[q] = b[q];
         }
         c->npml = off;
-        {
-            // auxiliary component c in region c mod 3: the nine of each phase fall 3/3/3
-            static const int aux_region[A_N] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
-            ALLOC_STRIPED(c->ade, (size_t)A_N * c->npml * sizeof(float), aux_region, A_N, (size_t)c->npml * sizeof(float));
-        }
+        ALLOC_STRIPED(c->ade, (size_t)A_N * c->npml * sizeof(float), kAuxRegion, A_N, (size_t)c->npml * sizeof(float));
         ALLOC(c->gxc, (size_t)nx * sizeof(float4));
         ALLOC(c->gxe, (size_t)nx * sizeof(float4));
         ALLOC(c->gyc, (size_t)ny * sizeof(float4));
@@ -371,6 +410,10 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    // the arena this context reserved for itself goes with it (it stays, to
+    // be released at process exit or by swpc_stripe_release, while another
+    // context still holds memory from it)
+    if (c->own_arena) (void)swpc_stripe_release(c->device);
     delete c;
     return SWPC_OK;
 }

@@ -21,7 +21,9 @@
 // per-region slabs.  When the arena is missing or full, allocation falls
 // back to hipMalloc.  The arena is process-wide, one per device; contexts
 // hold no reference to it, so swpc_stripe_release refuses while any of its
-// memory is in use.
+// memory is in use.  A large context created on a device without an arena
+// reserves a private one sized for itself (swpc_create), which only that
+// context draws from and which goes when it is destroyed.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -57,6 +59,7 @@ struct Live {
 
 struct Arena {
     int device = 0;
+    const void *owner = nullptr;  // a context's private arena (swpc_create), nullptr: the caller's
     int nreg = 0;
     size_t gran = 0;
     hipMemAccessDesc access{};
@@ -222,22 +225,23 @@ bool arena_active(int device)
     return g_arena.count(device) != 0;
 }
 
-void *arena_alloc(int device, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes)
+void *arena_alloc(int device, const void *who, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes)
 {
     if (bytes == 0 || (seg_region && nseg < 1)) return nullptr;
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_arena.find(device);
     if (it == g_arena.end()) return nullptr;
     Arena &a = *it->second;
+    if (a.owner && a.owner != who) return nullptr;
     if (bytes >= a.gran) return alloc_big(a, bytes, seg_region, nseg, seg_bytes);
     return alloc_small(a, bytes, seg_region ? seg_region[0] : 0);
 }
 
-int64_t arena_granule_bytes(int device)
+int64_t arena_granule_bytes(int device, const void *who)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_arena.find(device);
-    if (it == g_arena.end()) return 0;
+    if (it == g_arena.end() || (it->second->owner && it->second->owner != who)) return 0;
     int64_t b = 0;
     for (int r = 0; r < it->second->nreg; ++r) b += (int64_t)(it->second->reg[r].granules.size() * it->second->gran);
     return b;
@@ -284,7 +288,7 @@ bool arena_free(void *p)
     return false;
 }
 
-extern "C" int swpc_stripe_reserve(int device, int64_t bytes_per_region, int nregions)
+int arena_reserve(int device, int64_t bytes_per_region, int nregions, const void *owner)
 {
     if (nregions < 1 || nregions > kMaxRegions || bytes_per_region <= 0) return SWPC_EINVAL;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -339,8 +343,14 @@ extern "C" int swpc_stripe_reserve(int device, int64_t bytes_per_region, int nre
     for (size_t i = 0; i < all.size(); ++i)
         if (!keep[i]) (void)hipMemRelease(all[i]);
     if (!ok) return SWPC_ENOMEM;
+    a->owner = owner;
     g_arena[device] = a;
     return SWPC_OK;
+}
+
+extern "C" int swpc_stripe_reserve(int device, int64_t bytes_per_region, int nregions)
+{
+    return arena_reserve(device, bytes_per_region, nregions, nullptr);
 }
 
 extern "C" int swpc_stripe_release(int device)

@@ -199,6 +199,7 @@ struct swpc_ctx {
                        // bit 2: 32-wide interior tiles, bit 3: velocity loads one plane ahead,
                        // bit 4: adjoint records copied, bit 5: interior and side columns in one launch
     long long bytes;
+    int own_arena;     // the context reserved the device's striped arena itself (swpc_create)
     struct AdjWorkspace *adj_ws;  // adjoint state, gradient sums and checkpoints (adjoint.hip)
 };
 
@@ -212,11 +213,13 @@ void ctx_reset_graph(swpc_ctx *c);
 // none or it is full; an array of nseg segments of seg_bytes each puts
 // segment s in region seg_region[s] (nseg = 1: the whole array in
 // seg_region[0]; seg_region = nullptr: segment s in region s mod the
-// region count)
+// region count); a context's private arena serves only `who`
 bool arena_active(int device);
-void *arena_alloc(int device, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes);
-// bytes of whole free granules in the device's arena (0 without one)
-int64_t arena_granule_bytes(int device);
+void *arena_alloc(int device, const void *who, size_t bytes, const int *seg_region, int nseg, size_t seg_bytes);
+// swpc_stripe_reserve with an owner: a private arena that serves only it
+int arena_reserve(int device, int64_t bytes_per_region, int nregions, const void *owner);
+// bytes of whole free granules in the device's arena that `who` may use (0 without one)
+int64_t arena_granule_bytes(int device, const void *who);
 // returns the block to its arena; false when p is not arena memory
 bool arena_free(void *p);
 

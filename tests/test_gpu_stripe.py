@@ -72,12 +72,14 @@ def _spec(nx, ny, nz):
 # regions); 256x256x320 in one granule per region does not (fields of 109 MB
 # share slabs until they are full, the rest falls back to hipMalloc)
 @pytest.mark.parametrize("shape,per_region", [((512, 512, 256), None), ((256, 256, 320), GRAN)])
-def test_striped_context_equals_hipmalloc_placement(shape, per_region):
+def test_striped_context_equals_hipmalloc_placement(shape, per_region, monkeypatch):
     from openswpc_amd.gpu import stripe_bytes_per_region, stripe_release, stripe_reserve
     from openswpc_amd.host import build_inputs, make_simulation
     spec = _spec(*shape)
     inp = build_inputs(spec)
-    plain = make_simulation(inp)                # before the arena: hipMalloc
+    monkeypatch.setenv("SWPC_STRIPE", "0")      # no private arena: hipMalloc placement
+    plain = make_simulation(inp)
+    monkeypatch.delenv("SWPC_STRIPE")
     sims = [plain]
     per = per_region or stripe_bytes_per_region(spec.nx, spec.ny, spec.nz, spec.nm, spec.na) + GRAN
     stripe_reserve(per)
@@ -167,3 +169,42 @@ def test_batch_of_arena_members_matches_their_oracles(orc):
         for g in gs:
             g.close()
         stripe_release()
+
+
+def test_large_context_stripes_itself(monkeypatch):
+    """A context of 4 GiB or more created on a device without an arena
+    reserves a private one for itself: another context does not draw from it
+    and cannot reserve over it, it goes with its owner, and the placement
+    changes no result.  SWPC_STRIPE=0 turns it off."""
+    from openswpc_amd.gpu import SwpcError, stripe_reserve
+    from openswpc_amd.host import build_inputs, make_simulation
+    inp = build_inputs(_spec(512, 512, 256))
+    assert _free() == [0, 0, 0]
+    own = make_simulation(inp)
+    sims = [own]
+    try:
+        slack = _free()
+        print("private arena, free bytes per region", slack, flush=True)
+        assert all(f > 0 for f in slack)
+        monkeypatch.setenv("SWPC_STRIPE", "0")
+        off = make_simulation(inp)              # hipMalloc placement
+        sims.append(off)
+        monkeypatch.delenv("SWPC_STRIPE")
+        other = make_simulation(inp)            # the private arena is not its to use
+        sims.append(other)
+        assert _free() == slack
+        with pytest.raises(SwpcError):
+            stripe_reserve(GRAN)
+        for s in sims:
+            s.step(1, 30, use_graph=True, block=15)
+        for n in ("Vx", "Vz", "Sxx", "Syz"):
+            a = own.get_field(n)
+            assert np.any(a) or n != "Vz"
+            for s in sims[1:]:
+                assert np.array_equal(_bits(a), _bits(s.get_field(n))), n
+        own.close()
+        sims.remove(own)
+        assert _free() == [0, 0, 0]
+    finally:
+        for s in sims:
+            s.close()
