@@ -608,7 +608,9 @@ def main():
 
     t0 = time.perf_counter()
     spec = production_spec() if args.workload == "production" else plain_spec()
-    placement = stripe_on(stripe_bytes_per_region(spec.nx, spec.ny, spec.nz, spec.nm, spec.na) + (512 << 20))
+    fused = args.variant >= 0 and (args.variant & 128) != 0     # a second field set (fused steps)
+    placement = stripe_on(stripe_bytes_per_region(spec.nx, spec.ny, spec.nz, spec.nm, spec.na,
+                                                  field_sets=2 if fused else 1) + (512 << 20))
     if args.host_medium:
         inp = build_inputs(spec)
         log(f"[bench] host set-up {time.perf_counter() - t0:.1f}s")

@@ -412,7 +412,13 @@ int64_t swpc_stripe_free_bytes(int device, int region);
    computes plane k, bit 4 = the adjoint copies its step records instead of
    stepping out of place into them, bit 5 = the interior and side columns of
    a phase in one launch, x-slab tiles next to the interior rows they
-   border.  All variants give bit-identical results. */
+   border, bit 7 = the velocity update of a step and the stress update of
+   the next in one interior launch (PML runs without body forces or the
+   reciprocity mode; the context then holds a second set of the 9 fields,
+   taken from the striped arena when it has room; kchunk sets the launch's
+   chunk depth), bit 8 = that launch with 8-row tiles instead of 4.  All
+   variants give bit-identical results.  Fails with SWPC_ENOMEM when the
+   second field set does not fit. */
 int swpc_set_tuning(swpc_ctx *ctx, int kchunk, int by, int variant);
 
 /* The HIP stream the context launches on (hipStream_t as void*), so a

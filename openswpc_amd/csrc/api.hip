@@ -179,7 +179,19 @@ static int build_zchunks(swpc_ctx *c)
             z[b] = std::min(std::max(k + 1, z[b - 1] + 1), nz - (nbz - b) + 1);
         }
         z[nbz] = nz + 1;
+        c->zband[ph] = 0;
+        for (int b = 0; b < nbz; ++b)
+            if (pml && z[b + 1] - 1 > g.ke) {
+                c->zband[ph] = b;
+                break;
+            }
     }
+    // fused launch: chunks of kchunk planes over the interior planes kb..ke
+    // (each chunk recomputes the velocities of two planes on either side)
+    const int nki = g.ke - g.kb + 1;
+    const int nfz = nki > 0 ? (nki + c->kchunk - 1) / c->kchunk : 0;
+    std::vector<int> fz(nfz + 1);
+    for (int b = 0; b <= nfz; ++b) fz[b] = g.kb + (int)((long long)nki * b / std::max(nfz, 1));
     if (c->d_zchunks) c->bytes -= 2LL * (c->nbz + 1) * (long long)sizeof(int);
     dfree(c->d_zchunks);
     c->d_zchunks = nullptr;
@@ -187,6 +199,13 @@ static int build_zchunks(swpc_ctx *c)
     if (dalloc(c, (void **)&c->d_zchunks, zc.size() * sizeof(int)) != hipSuccess)
         return fail(c, SWPC_ENOMEM, "k-chunk table allocation failed");
     HIPCHK(c, hipMemcpyAsync(c->d_zchunks, zc.data(), zc.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (c->d_fzc) c->bytes -= (c->nfz + 1) * (long long)sizeof(int);
+    dfree(c->d_fzc);
+    c->d_fzc = nullptr;
+    c->nfz = nfz;
+    if (dalloc(c, (void **)&c->d_fzc, fz.size() * sizeof(int)) != hipSuccess)
+        return fail(c, SWPC_ENOMEM, "k-chunk table allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_fzc, fz.data(), fz.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SWPC_OK;
 }
@@ -393,6 +412,8 @@ extern "C" int swpc_destroy(swpc_ctx *c)
     reset_graph(c);
     swpc_adj_release(c);
     for (int q = 0; q < SWPC_NFIELD; ++q) dfree(c->fields[q]);
+    for (int q = 0; q < SWPC_NFIELD; ++q) dfree(c->fields2[q]);
+    dfree(c->d_fzc);
     dfree(c->rho); dfree(c->lam); dfree(c->mu); dfree(c->taup); dfree(c->taus);
     dfree(c->kfs_top); dfree(c->kfs_bot); dfree(c->kob_top); dfree(c->kob_bot); dfree(c->kob);
     dfree(c->R); dfree(c->ade); dfree(c->d_pml_tiles); dfree(c->d_zchunks);
@@ -1042,6 +1063,82 @@ static hipError_t enqueue_step(swpc_ctx *c, int it, const int *d_it, hipEvent_t 
     return e;
 }
 
+// variant bit 7: the velocity update of a step and the stress update of the
+// next in one interior launch (k_fused), between the side-column and band
+// launches of the two phases.  It needs the second field set and the PML,
+// and no source that acts between the two updates (body forces, the
+// reciprocity force) and no routine that reads the fields between them.
+static bool fused_ready(const swpc_ctx *c)
+{
+    return (c->variant & 128) && c->fields2[0] && c->desc.abc == SWPC_ABC_PML && c->npml > 0 && c->nfz > 0 &&
+           c->g.nxk > 0 && c->g.nyk > 0 && !c->fields_in[0] && !(c->nsrc > 0 && c->bf_mode) && !c->green_on;
+}
+
+// steps it..it+n-1 (n >= 3) with fused launches.  The first step's stress
+// update and the last step's velocity update run in place on the current
+// fields; each fused launch between them reads one field set and writes the
+// other, an even number of times, so the fields end where they started.
+// The outputs of a step (receivers, sections) read the velocities and
+// stresses of the step before it, which the two sets hold then.
+static hipError_t enqueue_fused_steps(swpc_ctx *c, int it, int n, const int *d_it)
+{
+    float *A[SWPC_NFIELD], *B[SWPC_NFIELD];
+    for (int q = 0; q < SWPC_NFIELD; ++q) {
+        A[q] = c->fields[q];
+        B[q] = c->fields2[q];
+    }
+    auto view = [&](float *const *v, float *const *s, float *const *in) {
+        for (int q = 0; q < 3; ++q) c->fields[q] = v[q];
+        for (int q = 3; q < SWPC_NFIELD; ++q) c->fields[q] = s[q];
+        for (int q = 0; q < SWPC_NFIELD; ++q) c->fields_in[q] = in ? in[q] : nullptr;
+    };
+    auto step_no = [&](int s) { return d_it ? 0 : it + s; };
+    hipError_t e;
+    // step it up to its stress glut, as in enqueue_step
+    if ((e = launch_green_store(c, step_no(0), d_it)) != hipSuccess ||
+        (e = launch_wav_store(c, step_no(0), d_it)) != hipSuccess || (e = launch_snap(c, step_no(0), d_it)) != hipSuccess ||
+        (e = launch_stress_side(c, nullptr)) != hipSuccess || (e = launch_stress_cols(c, 3)) != hipSuccess ||
+        (e = launch_stressglut(c, step_no(0), d_it)) != hipSuccess)
+        return e;
+    const int nf = (n - 1) & ~1;
+    for (int s = 0; s < nf && e == hipSuccess; ++s) {
+        float *const *cur = (s & 1) ? B : A;
+        float *const *nxt = (s & 1) ? A : B;
+        // step it+s: velocities of the side columns and the bottom band (2: band only)
+        view(nxt, cur, cur);
+        if ((e = launch_vel_side(c, nullptr)) != hipSuccess || (e = launch_vel_cols(c, 2)) != hipSuccess) break;
+        // interior: velocities of step it+s and stresses of step it+s+1
+        view(nxt, nxt, cur);
+        if ((e = launch_fused(c)) != hipSuccess) break;
+        // step it+s+1: stresses of the side columns and the bottom band
+        if ((e = launch_stress_side(c, nullptr)) != hipSuccess || (e = launch_stress_cols(c, 2)) != hipSuccess) break;
+        if (d_it && (e = launch_step_advance(c)) != hipSuccess) break;
+        view(nxt, cur, nullptr);
+        if ((e = launch_wav_store(c, step_no(s + 1), d_it)) != hipSuccess ||
+            (e = launch_snap(c, step_no(s + 1), d_it)) != hipSuccess)
+            break;
+        view(nxt, nxt, nullptr);
+        e = launch_stressglut(c, step_no(s + 1), d_it);
+    }
+    view(A, A, nullptr);
+    if (e != hipSuccess) return e;
+    // step it+nf from its velocity update on, in place
+    if ((e = launch_vel_side(c, nullptr)) != hipSuccess || (e = launch_vel_cols(c, 3)) != hipSuccess ||
+        (e = launch_bodyforce(c, step_no(nf), d_it)) != hipSuccess ||
+        (e = launch_green_source(c, step_no(nf), d_it)) != hipSuccess || (d_it && (e = launch_step_advance(c)) != hipSuccess))
+        return e;
+    for (int s = nf + 1; s < n && e == hipSuccess; ++s) e = enqueue_step(c, step_no(s), d_it);
+    return e;
+}
+
+static hipError_t enqueue_steps(swpc_ctx *c, int it, int n, const int *d_it)
+{
+    if (n >= 3 && fused_ready(c)) return enqueue_fused_steps(c, it, n, d_it);
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < n && e == hipSuccess; ++s) e = enqueue_step(c, d_it ? 0 : it + s, d_it);
+    return e;
+}
+
 __global__ void k_set_int(int *p, int v)
 {
     if (threadIdx.x == 0) *p = v;
@@ -1061,8 +1158,7 @@ extern "C" int swpc_step(swpc_ctx *c, int it0, int nsteps, int use_graph, int bl
             reset_graph(c);
             hipGraph_t graph;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = hipSuccess;
-            for (int b = 0; b < block && e == hipSuccess; ++b) e = enqueue_step(c, 0, c->d_it);
+            hipError_t e = enqueue_steps(c, 0, block, c->d_it);
             hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
             if (e != hipSuccess) return fail(c, SWPC_EHIP, "graph capture: %s", hipGetErrorString(e));
             if (e2 != hipSuccess) return fail(c, SWPC_EHIP, "graph capture end: %s", hipGetErrorString(e2));
@@ -1078,7 +1174,7 @@ extern "C" int swpc_step(swpc_ctx *c, int it0, int nsteps, int use_graph, int bl
         HIPCHK(c, hipGetLastError());
         for (; done + block <= nsteps; done += block) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     }
-    for (; done < nsteps; ++done) HIPCHK(c, enqueue_step(c, it0 + done, nullptr));
+    if (done < nsteps) HIPCHK(c, enqueue_steps(c, it0 + done, nsteps - done, nullptr));
     return SWPC_OK;
 }
 
@@ -1195,6 +1291,27 @@ extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by, int variant)
     if (by > 0) c->by = by;
     if (variant >= 0) c->variant = variant;
     reset_graph(c);
+    // the second field set of the fused steps, held while variant bit 7 is set
+    const size_t fb = (size_t)c->g.n * sizeof(float);
+    const bool want2 = (c->variant & 128) && c->desc.abc == SWPC_ABC_PML;
+    for (int q = 0; q < SWPC_NFIELD; ++q) {
+        if (want2 && !c->fields2[q] && dalloc_striped(c, (void **)&c->fields2[q], fb, &kFieldRegion[q], 1, 0) != hipSuccess) {
+            c->fields2[q] = nullptr;
+            for (int r = 0; r < SWPC_NFIELD; ++r)
+                if (c->fields2[r]) {
+                    dfree(c->fields2[r]);
+                    c->fields2[r] = nullptr;
+                    c->bytes -= (long long)fb;
+                }
+            return fail(c, SWPC_ENOMEM, "swpc_set_tuning: no device memory for the second field set (variant bit 7)");
+        }
+        if (!want2 && c->fields2[q]) {
+            dfree(c->fields2[q]);
+            c->fields2[q] = nullptr;
+            c->bytes -= (long long)fb;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     int rc = build_pml_tiles(c);
     if (rc == SWPC_OK) rc = build_zchunks(c);
     if (rc == SWPC_OK && (rc = build_schedule(c)) != SWPC_OK) rc = fail(c, rc, "swpc_set_tuning: launch schedule");

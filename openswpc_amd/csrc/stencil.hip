@@ -298,6 +298,114 @@ __device__ __forceinline__ void pml_vel_cell(const StepArgs &a, long long p, lon
 
 // kernel__update_stress, m_kernel.f90:142-348 (normal and shear parts fused:
 // both read V and write disjoint arrays)
+// The velocities the stress update of one cell reads: the z, x and y lines
+// through the cell (m_kernel.f90:198-333)
+struct VStencil {
+    float vx_m1, vx_0, vx_p1, vx_p2, vx_im2, vx_im1, vx_ip1, vx_jm1, vx_jp1, vx_jp2;
+    float vy_m1, vy_0, vy_p1, vy_p2, vy_im1, vy_ip1, vy_ip2, vy_jm2, vy_jm1, vy_jp1;
+    float vz_m2, vz_m1, vz_0, vz_p1, vz_im1, vz_ip1, vz_ip2, vz_jm1, vz_jp1, vz_jp2;
+};
+
+// kernel__update_stress for one cell of the interior box, m_kernel.f90:198-333:
+// the normal and shear stresses and their memory variables.  p: field index,
+// r: memory-variable index, in: the cell is in the 2nd-order band, mu..: mu
+// at (i,j), (i+1,j), (i,j+1) on planes k and k+1 (n) and at (i+1,j+1) on
+// plane k.  OOP: the old stresses are read from a.fin.
+template <int NM, bool NT, bool OOP>
+__device__ __forceinline__ void stress_cell(const StepArgs &a, long long p, long long r, bool in, const VStencil &v,
+                                            float mu00, float mu00n, float mu10, float mu10n, float mu01, float mu01n,
+                                            float mu11)
+{
+    float *__restrict__ Sxx = a.f.Sxx;
+    float *__restrict__ Syy = a.f.Syy;
+    float *__restrict__ Szz = a.f.Szz;
+    float *__restrict__ Syz = a.f.Syz;
+    float *__restrict__ Sxz = a.f.Sxz;
+    float *__restrict__ Sxy = a.f.Sxy;
+    const float *__restrict__ SxxI = OOP ? a.fin.Sxx : Sxx;
+    const float *__restrict__ SyyI = OOP ? a.fin.Syy : Syy;
+    const float *__restrict__ SzzI = OOP ? a.fin.Szz : Szz;
+    const float *__restrict__ SyzI = OOP ? a.fin.Syz : Syz;
+    const float *__restrict__ SxzI = OOP ? a.fin.Sxz : Sxz;
+    const float *__restrict__ SxyI = OOP ? a.fin.Sxy : Sxy;
+    const float dt = a.dt;
+    const float d2 = a.v.d2;
+    const long long rn = a.g.rn;
+    const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
+    const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
+    const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
+    const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
+    const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
+    const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
+
+    // normal components, m_kernel.f90:198-238
+    const float dxVx = (v.vx_0 - v.vx_im1) * re40x - (v.vx_ip1 - v.vx_im2) * re41x;
+    const float dyVy = (v.vy_0 - v.vy_jm1) * re40y - (v.vy_jp1 - v.vy_jm2) * re41y;
+    const float dzVz = (v.vz_0 - v.vz_m1) * re40z - (v.vz_p1 - v.vz_m2) * re41z;
+
+    const float mu2 = 2.0f * mu00;
+    const float lam2mu = ld1<NT>(a.m.lam + p) + mu2;
+    const float taup1 = ld1<NT>(a.m.taup + p);
+    const float taus1 = ld1<NT>(a.m.taus + p);
+    const float d3v3 = dxVx + dyVy + dzVz;
+    const float dyVy_dzVz = dyVy + dzVz;
+    const float dxVx_dzVz = dxVx + dzVz;
+    const float dxVx_dyVy = dxVx + dyVy;
+
+    RCell<NM> R;
+    R.template load<NT>(a.R, rn, r);
+    float Rxx_n = 0.0f, Ryy_n = 0.0f, Rzz_n = 0.0f;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
+        const float nxx = c1 * R.v[0 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
+        const float nyy = c1 * R.v[1 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
+        const float nzz = c1 * R.v[2 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
+        R.v[0 * NM + m] = nxx;
+        R.v[1 * NM + m] = nyy;
+        R.v[2 * NM + m] = nzz;
+        Rxx_n = Rxx_n + d1 * nxx;
+        Ryy_n = Ryy_n + d1 * nyy;
+        Rzz_n = Rzz_n + d1 * nzz;
+    }
+    const float taup_plus1 = 1.0f + taup1 * (1.0f + d2);
+    const float taus_plus1 = 1.0f + taus1 * (1.0f + d2);
+    st1<NT>(Sxx + p, ld1<NT>(SxxI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
+    st1<NT>(Syy + p, ld1<NT>(SyyI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
+    st1<NT>(Szz + p, ld1<NT>(SzzI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
+
+    // shear components, m_kernel.f90:285-333
+    const float dxVy_dyVx = (v.vy_ip1 - v.vy_0) * re40x - (v.vy_ip2 - v.vy_im1) * re41x
+                          + (v.vx_jp1 - v.vx_0) * re40y - (v.vx_jp2 - v.vx_jm1) * re41y;
+    const float dxVz_dzVx = (v.vz_ip1 - v.vz_0) * re40x - (v.vz_ip2 - v.vz_im1) * re41x
+                          + (v.vx_p1 - v.vx_0) * re40z - (v.vx_p2 - v.vx_m1) * re41z;
+    const float dyVz_dzVy = (v.vz_jp1 - v.vz_0) * re40y - (v.vz_jp2 - v.vz_jm1) * re41y
+                          + (v.vy_p1 - v.vy_0) * re40z - (v.vy_p2 - v.vy_m1) * re41z;
+
+    const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
+    const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
+    const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
+
+    float Ryz_n = 0.0f, Rxz_n = 0.0f, Rxy_n = 0.0f;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
+        const float nyz = c1 * R.v[3 * NM + m] - c2 * muyz * taus1 * dyVz_dzVy * dt;
+        const float nxz = c1 * R.v[4 * NM + m] - c2 * muxz * taus1 * dxVz_dzVx * dt;
+        const float nxy = c1 * R.v[5 * NM + m] - c2 * muxy * taus1 * dxVy_dyVx * dt;
+        R.v[3 * NM + m] = nyz;
+        R.v[4 * NM + m] = nxz;
+        R.v[5 * NM + m] = nxy;
+        Ryz_n = Ryz_n + d1 * nyz;
+        Rxz_n = Rxz_n + d1 * nxz;
+        Rxy_n = Rxy_n + d1 * nxy;
+    }
+    R.template store<NT>(a.R, rn, r);
+    st1<NT>(Syz + p, ld1<NT>(SyzI + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
+    st1<NT>(Sxz + p, ld1<NT>(SxzI + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
+    st1<NT>(Sxy + p, ld1<NT>(SxyI + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
+}
+
 // logical tile (bx, by, bz) of a plain launch: blocks of one XCD take a
 // contiguous run of tiles, x fastest
 __device__ static inline void plain_tile(const StepArgs &a, int &bx, int &byy, int &bz)
@@ -330,24 +438,6 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
     const float *__restrict__ Vy = a.f.Vy;
     const float *__restrict__ Vz = a.f.Vz;
     const float *__restrict__ mu = a.m.mu;
-    const float *__restrict__ lam = a.m.lam;
-    const float *__restrict__ taup = a.m.taup;
-    const float *__restrict__ taus = a.m.taus;
-    float *__restrict__ Sxx = a.f.Sxx;
-    float *__restrict__ Syy = a.f.Syy;
-    float *__restrict__ Szz = a.f.Szz;
-    float *__restrict__ Syz = a.f.Syz;
-    float *__restrict__ Sxz = a.f.Sxz;
-    float *__restrict__ Sxy = a.f.Sxy;
-    const float *__restrict__ SxxI = OOP ? a.fin.Sxx : Sxx;
-    const float *__restrict__ SyyI = OOP ? a.fin.Syy : Syy;
-    const float *__restrict__ SzzI = OOP ? a.fin.Szz : Szz;
-    const float *__restrict__ SyzI = OOP ? a.fin.Syz : Syz;
-    const float *__restrict__ SxzI = OOP ? a.fin.Sxz : Sxz;
-    const float *__restrict__ SxyI = OOP ? a.fin.Sxy : Sxy;
-    const float dt = a.dt;
-    const float d2 = a.v.d2;
-    const long long rn = g.rn;
 
     long long p = gidx(g, i, j, ks);
     float vx_m1 = Vx[p - P], vx_0 = Vx[p], vx_p1 = Vx[p + P];
@@ -364,88 +454,18 @@ __device__ __forceinline__ void stress_int_body(const StepArgs &a, int bx, int b
             const float vx_p2 = Vx[p + 2 * P], vy_p2 = Vy[p + 2 * P], vz_p1 = Vz[p + P];
             const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
 
-            const bool in = in_band(k, kft, kfb, kot, kobb);
-            const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
-            const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
-            const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
-            const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
-            const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
-            const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
-
-            const float vx_im2 = Vx[p - 2], vx_im1 = Vx[p - 1], vx_ip1 = Vx[p + 1];
-            const float vx_jm1 = Vx[p - Y], vx_jp1 = Vx[p + Y], vx_jp2 = Vx[p + 2 * Y];
-            const float vy_jm2 = Vy[p - 2 * Y], vy_jm1 = Vy[p - Y], vy_jp1 = Vy[p + Y];
-            const float vy_im1 = Vy[p - 1], vy_ip1 = Vy[p + 1], vy_ip2 = Vy[p + 2];
-            const float vz_im1 = Vz[p - 1], vz_ip1 = Vz[p + 1], vz_ip2 = Vz[p + 2];
-            const float vz_jm1 = Vz[p - Y], vz_jp1 = Vz[p + Y], vz_jp2 = Vz[p + 2 * Y];
-
-            // normal components, m_kernel.f90:198-238
-            const float dxVx = (vx_0 - vx_im1) * re40x - (vx_ip1 - vx_im2) * re41x;
-            const float dyVy = (vy_0 - vy_jm1) * re40y - (vy_jp1 - vy_jm2) * re41y;
-            const float dzVz = (vz_0 - vz_m1) * re40z - (vz_p1 - vz_m2) * re41z;
-
-            const float mu2 = 2.0f * mu00;
-            const float lam2mu = ld1<NT>(lam + p) + mu2;
-            const float taup1 = ld1<NT>(taup + p);
-            const float taus1 = ld1<NT>(taus + p);
-
-            const float d3v3 = dxVx + dyVy + dzVz;
-            const float dyVy_dzVz = dyVy + dzVz;
-            const float dxVx_dzVz = dxVx + dzVz;
-            const float dxVx_dyVy = dxVx + dyVy;
-
-            RCell<NM> R;
-            R.template load<NT>(a.R, rn, r);
-            float Rxx_n = 0.0f, Ryy_n = 0.0f, Rzz_n = 0.0f;
-#pragma unroll
-            for (int m = 0; m < NM; ++m) {
-                const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-                const float nxx = c1 * R.v[0 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz) * dt;
-                const float nyy = c1 * R.v[1 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz) * dt;
-                const float nzz = c1 * R.v[2 * NM + m] - c2 * (lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy) * dt;
-                R.v[0 * NM + m] = nxx;
-                R.v[1 * NM + m] = nyy;
-                R.v[2 * NM + m] = nzz;
-                Rxx_n = Rxx_n + d1 * nxx;
-                Ryy_n = Ryy_n + d1 * nyy;
-                Rzz_n = Rzz_n + d1 * nzz;
-            }
-            const float taup_plus1 = 1.0f + taup1 * (1.0f + d2);
-            const float taus_plus1 = 1.0f + taus1 * (1.0f + d2);
-            st1<NT>(Sxx + p, ld1<NT>(SxxI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt);
-            st1<NT>(Syy + p, ld1<NT>(SyyI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt);
-            st1<NT>(Szz + p, ld1<NT>(SzzI + p) + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt);
-
-            // shear components, m_kernel.f90:285-333
-            const float dxVy_dyVx = (vy_ip1 - vy_0) * re40x - (vy_ip2 - vy_im1) * re41x
-                                  + (vx_jp1 - vx_0) * re40y - (vx_jp2 - vx_jm1) * re41y;
-            const float dxVz_dzVx = (vz_ip1 - vz_0) * re40x - (vz_ip2 - vz_im1) * re41x
-                                  + (vx_p1 - vx_0) * re40z - (vx_p2 - vx_m1) * re41z;
-            const float dyVz_dzVy = (vz_jp1 - vz_0) * re40y - (vz_jp2 - vz_jm1) * re41y
-                                  + (vy_p1 - vy_0) * re40z - (vy_p2 - vy_m1) * re41z;
-
-            const float muxz = hmean4(mu00, mu00n, mu10, mu10n, EPSL);
-            const float muxy = hmean4(mu00, mu10, mu01, mu11, EPSL);
-            const float muyz = hmean4(mu00, mu00n, mu01, mu01n, EPSL);
-
-            float Ryz_n = 0.0f, Rxz_n = 0.0f, Rxy_n = 0.0f;
-#pragma unroll
-            for (int m = 0; m < NM; ++m) {
-                const float c1 = a.v.c1[m], c2 = a.v.c2[m], d1 = a.v.d1[m];
-                const float nyz = c1 * R.v[3 * NM + m] - c2 * muyz * taus1 * dyVz_dzVy * dt;
-                const float nxz = c1 * R.v[4 * NM + m] - c2 * muxz * taus1 * dxVz_dzVx * dt;
-                const float nxy = c1 * R.v[5 * NM + m] - c2 * muxy * taus1 * dxVy_dyVx * dt;
-                R.v[3 * NM + m] = nyz;
-                R.v[4 * NM + m] = nxz;
-                R.v[5 * NM + m] = nxy;
-                Ryz_n = Ryz_n + d1 * nyz;
-                Rxz_n = Rxz_n + d1 * nxz;
-                Rxy_n = Rxy_n + d1 * nxy;
-            }
-            R.template store<NT>(a.R, rn, r);
-            st1<NT>(Syz + p, ld1<NT>(SyzI + p) + (muyz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt);
-            st1<NT>(Sxz + p, ld1<NT>(SxzI + p) + (muxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt);
-            st1<NT>(Sxy + p, ld1<NT>(SxyI + p) + (muxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt);
+            VStencil v;
+            v.vx_m1 = vx_m1; v.vx_0 = vx_0; v.vx_p1 = vx_p1; v.vx_p2 = vx_p2;
+            v.vy_m1 = vy_m1; v.vy_0 = vy_0; v.vy_p1 = vy_p1; v.vy_p2 = vy_p2;
+            v.vz_m2 = vz_m2; v.vz_m1 = vz_m1; v.vz_0 = vz_0; v.vz_p1 = vz_p1;
+            v.vx_im2 = Vx[p - 2]; v.vx_im1 = Vx[p - 1]; v.vx_ip1 = Vx[p + 1];
+            v.vx_jm1 = Vx[p - Y]; v.vx_jp1 = Vx[p + Y]; v.vx_jp2 = Vx[p + 2 * Y];
+            v.vy_jm2 = Vy[p - 2 * Y]; v.vy_jm1 = Vy[p - Y]; v.vy_jp1 = Vy[p + Y];
+            v.vy_im1 = Vy[p - 1]; v.vy_ip1 = Vy[p + 1]; v.vy_ip2 = Vy[p + 2];
+            v.vz_im1 = Vz[p - 1]; v.vz_ip1 = Vz[p + 1]; v.vz_ip2 = Vz[p + 2];
+            v.vz_jm1 = Vz[p - Y]; v.vz_jp1 = Vz[p + Y]; v.vz_jp2 = Vz[p + 2 * Y];
+            stress_cell<NM, NT, OOP>(a, p, r, in_band(k, kft, kfb, kot, kobb), v, mu00, mu00n, mu10, mu10n, mu01, mu01n,
+                                     mu11);
 
             // rotate the z queues
             vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
@@ -689,6 +709,197 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PF ? 3
 }
 
 // ---------------------------------------------------------------------------
+// Velocity and stress in one z-march (variant bit 7).  One launch runs
+// kernel__update_vel of step n and kernel__update_stress of step n+1 over the
+// interior box, so the stresses cross HBM once for both updates and the new
+// velocities reach the stress stencil through LDS instead of HBM.  The old
+// fields are read from a.fin and the new ones written to a.f (two buffer sets
+// that swap every step), so a tile recomputes the velocities of the cells
+// around it from old values that no other tile overwrites.  A tile is FW x TY
+// stress columns (TY = 4 SR); its 64 x (TY + 4) velocity columns (two halo
+// columns and rows on each side) run two planes ahead of the stress update
+// and are kept in a ring of six planes in LDS.  Velocities outside the
+// interior box (side slabs, bottom band, outer halo) are read from a.f, where
+// the side-column and band launches that precede this one have put them.
+// ---------------------------------------------------------------------------
+#define FW (BX - 4)
+
+template <int NM, int SR>
+__global__ void __launch_bounds__(256) k_fused(StepArgs a)
+{
+    constexpr int TY = 4 * SR, VR = TY + 4, NS = 6;
+    __shared__ float vs[NS][3][VR][BX];
+    int bx, byy, bz;
+    plain_tile(a, bx, byy, bz);
+    const Grid &g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = g.ib + bx * FW, j0 = g.jb + byy * TY;
+    const int il = min(i0 + FW - 1, g.ie), jl = min(j0 + TY - 1, g.je);  // last stress column and row
+    const int k0 = a.zc[bz], k1 = a.zc[bz + 1] - 1;                      // interior planes k0..k1
+    const long long P = g.plane;
+    const int Y = g.pitch;
+    const float dt = a.dt;
+    const float *__restrict__ Sxx = a.fin.Sxx;
+    const float *__restrict__ Syy = a.fin.Syy;
+    const float *__restrict__ Szz = a.fin.Szz;
+    const float *__restrict__ Syz = a.fin.Syz;
+    const float *__restrict__ Sxz = a.fin.Sxz;
+    const float *__restrict__ Sxy = a.fin.Sxy;
+    const float *__restrict__ rho = a.m.rho;
+    const float *__restrict__ mu = a.m.mu;
+
+    // velocity points of this thread: column iv, rows j0 - 2 + ty + 4 r.
+    // vf: 0 not needed by the tile, 1 read from a.f, 2 computed, 3 computed
+    // and stored (the tile's own cells)
+    const int iv = i0 - 2 + tx;
+    const int kq = max(k0 - 2, g.kb);   // first computed plane: the z queues start there
+    int vf[SR + 1], vb[SR + 1][4];
+    float sxz_m2[SR + 1], sxz_m1[SR + 1], sxz_0[SR + 1], syz_m2[SR + 1], syz_m1[SR + 1], syz_0[SR + 1];
+    float szz_m1[SR + 1], szz_0[SR + 1], szz_p1[SR + 1], rho0[SR + 1];
+#pragma unroll
+    for (int r = 0; r <= SR; ++r) {
+        const int jv = j0 - 2 + ty + 4 * r;
+        const bool need = iv <= il + 2 && jv <= jl + 2;
+        const bool comp = need && iv >= g.ib && iv <= g.ie && jv >= g.jb && jv <= g.je;
+        const bool own = comp && iv >= i0 && iv <= il && jv >= j0 && jv <= jl;
+        vf[r] = !need ? 0 : !comp ? 1 : own ? 3 : 2;
+        vb[r][0] = vb[r][1] = vb[r][2] = vb[r][3] = 0;
+        sxz_m2[r] = sxz_m1[r] = sxz_0[r] = syz_m2[r] = syz_m1[r] = syz_0[r] = 0.0f;
+        szz_m1[r] = szz_0[r] = szz_p1[r] = rho0[r] = 0.0f;
+        if (comp) {
+            const long long c2 = g2idx(g, iv, jv);
+            vb[r][0] = a.m.kfs_top[c2]; vb[r][1] = a.m.kfs_bot[c2];
+            vb[r][2] = a.m.kob_top[c2]; vb[r][3] = a.m.kob_bot[c2];
+            const long long p = gidx(g, iv, jv, kq);
+            sxz_m2[r] = Sxz[p - 2 * P]; sxz_m1[r] = Sxz[p - P]; sxz_0[r] = Sxz[p];
+            syz_m2[r] = Syz[p - 2 * P]; syz_m1[r] = Syz[p - P]; syz_0[r] = Syz[p];
+            szz_m1[r] = Szz[p - P]; szz_0[r] = Szz[p]; szz_p1[r] = Szz[p + P];
+            rho0[r] = rho[p];
+        }
+    }
+
+    // stress cells of this thread: column is, rows j0 + ty + 4 s
+    const int is = i0 + tx;
+    bool sa[SR];
+    int sb[SR][4];
+    float mu00[SR], mu10[SR], mu01[SR], mu11[SR];
+#pragma unroll
+    for (int s = 0; s < SR; ++s) {
+        const int js = j0 + ty + 4 * s;
+        sa[s] = tx < FW && is <= il && js <= jl;
+        sb[s][0] = sb[s][1] = sb[s][2] = sb[s][3] = 0;
+        mu00[s] = mu10[s] = mu01[s] = mu11[s] = 0.0f;
+        if (sa[s]) {
+            const long long c2 = g2idx(g, is, js);
+            sb[s][0] = a.m.kfs_top[c2]; sb[s][1] = a.m.kfs_bot[c2];
+            sb[s][2] = a.m.kob_top[c2]; sb[s][3] = a.m.kob_bot[c2];
+            const long long p = gidx(g, is, js, k0);
+            mu00[s] = mu[p]; mu10[s] = mu[p + 1]; mu01[s] = mu[p + Y]; mu11[s] = mu[p + 1 + Y];
+        }
+    }
+
+    for (int kv = k0 - 2; kv <= k1 + 2; ++kv) {
+        // velocities of plane kv, kernel__update_vel (m_kernel.f90:75-140)
+        const int sl = (kv - k0 + 2) % NS;
+#pragma unroll
+        for (int r = 0; r <= SR; ++r) {
+            if (vf[r] == 0) continue;
+            const int row = ty + 4 * r;
+            const long long p = gidx(g, iv, j0 - 2 + row, kv);
+            float vx, vy, vz;
+            if (vf[r] >= 2 && kv >= g.kb && kv <= g.ke) {
+                const float sxz_p1 = Sxz[p + P], syz_p1 = Syz[p + P], szz_p2 = Szz[p + 2 * P];
+                const float rho_kp = rho[p + P];
+                const bool in = in_band(kv, vb[r][0], vb[r][1], vb[r][2], vb[r][3]);
+                const float re40x = in ? a.c.re40x[1] : a.c.re40x[0];
+                const float re41x = in ? a.c.re41x[1] : a.c.re41x[0];
+                const float re40y = in ? a.c.re40y[1] : a.c.re40y[0];
+                const float re41y = in ? a.c.re41y[1] : a.c.re41y[0];
+                const float re40z = in ? a.c.re40z[1] : a.c.re40z[0];
+                const float re41z = in ? a.c.re41z[1] : a.c.re41z[0];
+
+                const float sxx_im1 = Sxx[p - 1], sxx_0 = Sxx[p], sxx_ip1 = Sxx[p + 1], sxx_ip2 = Sxx[p + 2];
+                const float sxy_0 = Sxy[p];
+                const float sxy_jm2 = Sxy[p - 2 * Y], sxy_jm1 = Sxy[p - Y], sxy_jp1 = Sxy[p + Y];
+                const float sxy_im2 = Sxy[p - 2], sxy_im1 = Sxy[p - 1], sxy_ip1 = Sxy[p + 1];
+                const float syy_jm1 = Syy[p - Y], syy_0 = Syy[p], syy_jp1 = Syy[p + Y], syy_jp2 = Syy[p + 2 * Y];
+                const float syz_jm2 = Syz[p - 2 * Y], syz_jm1 = Syz[p - Y], syz_jp1 = Syz[p + Y];
+                const float sxz_im2 = Sxz[p - 2], sxz_im1 = Sxz[p - 1], sxz_ip1 = Sxz[p + 1];
+
+                const float d3Sx3 = (sxx_ip1 - sxx_0) * re40x - (sxx_ip2 - sxx_im1) * re41x
+                                  + (sxy_0 - sxy_jm1) * re40y - (sxy_jp1 - sxy_jm2) * re41y
+                                  + (sxz_0[r] - sxz_m1[r]) * re40z - (sxz_p1 - sxz_m2[r]) * re41z;
+                const float d3Sy3 = (sxy_0 - sxy_im1) * re40x - (sxy_ip1 - sxy_im2) * re41x
+                                  + (syy_jp1 - syy_0) * re40y - (syy_jp2 - syy_jm1) * re41y
+                                  + (syz_0[r] - syz_m1[r]) * re40z - (syz_p1 - syz_m2[r]) * re41z;
+                const float d3Sz3 = (sxz_0[r] - sxz_im1) * re40x - (sxz_ip1 - sxz_im2) * re41x
+                                  + (syz_0[r] - syz_jm1) * re40y - (syz_jp1 - syz_jm2) * re41y
+                                  + (szz_p1[r] - szz_0[r]) * re40z - (szz_p2 - szz_m1[r]) * re41z;
+
+                vx = ld1<true>(a.fin.Vx + p) + 2.0f / (rho0[r] + rho[p + 1]) * d3Sx3 * dt;
+                vy = ld1<true>(a.fin.Vy + p) + 2.0f / (rho0[r] + rho[p + Y]) * d3Sy3 * dt;
+                vz = ld1<true>(a.fin.Vz + p) + 2.0f / (rho0[r] + rho_kp) * d3Sz3 * dt;
+                if (vf[r] == 3 && kv >= k0 && kv <= k1) {
+                    st1<true>(a.f.Vx + p, vx);
+                    st1<true>(a.f.Vy + p, vy);
+                    st1<true>(a.f.Vz + p, vz);
+                }
+                sxz_m2[r] = sxz_m1[r]; sxz_m1[r] = sxz_0[r]; sxz_0[r] = sxz_p1;
+                syz_m2[r] = syz_m1[r]; syz_m1[r] = syz_0[r]; syz_0[r] = syz_p1;
+                szz_m1[r] = szz_0[r]; szz_0[r] = szz_p1[r]; szz_p1[r] = szz_p2;
+                rho0[r] = rho_kp;
+            } else {
+                vx = a.f.Vx[p];
+                vy = a.f.Vy[p];
+                vz = a.f.Vz[p];
+            }
+            vs[sl][0][row][tx] = vx;
+            vs[sl][1][row][tx] = vy;
+            vs[sl][2][row][tx] = vz;
+        }
+        __syncthreads();
+
+        // stresses of plane kv - 2 from the velocities of planes kv - 4 .. kv
+        const int k = kv - 2;
+        if (k < k0) continue;
+        const int s0 = (k - k0 + 2) % NS, sm1 = (k - k0 + 1) % NS, sm2 = (k - k0) % NS;
+        const int sp1 = (k - k0 + 3) % NS, sp2 = (k - k0 + 4) % NS;
+#pragma unroll
+        for (int s = 0; s < SR; ++s) {
+            if (!sa[s]) continue;
+            const int row = ty + 4 * s + 2, col = tx + 2;
+            const int js = j0 + ty + 4 * s;
+            const long long p = gidx(g, is, js, k);
+            const float mu00n = mu[p + P], mu10n = mu[p + P + 1], mu01n = mu[p + P + Y];
+            VStencil v;
+            v.vx_m1 = vs[sm1][0][row][col]; v.vx_0 = vs[s0][0][row][col];
+            v.vx_p1 = vs[sp1][0][row][col]; v.vx_p2 = vs[sp2][0][row][col];
+            v.vy_m1 = vs[sm1][1][row][col]; v.vy_0 = vs[s0][1][row][col];
+            v.vy_p1 = vs[sp1][1][row][col]; v.vy_p2 = vs[sp2][1][row][col];
+            v.vz_m2 = vs[sm2][2][row][col]; v.vz_m1 = vs[sm1][2][row][col];
+            v.vz_0 = vs[s0][2][row][col]; v.vz_p1 = vs[sp1][2][row][col];
+            v.vx_im2 = vs[s0][0][row][col - 2]; v.vx_im1 = vs[s0][0][row][col - 1];
+            v.vx_ip1 = vs[s0][0][row][col + 1];
+            v.vx_jm1 = vs[s0][0][row - 1][col]; v.vx_jp1 = vs[s0][0][row + 1][col];
+            v.vx_jp2 = vs[s0][0][row + 2][col];
+            v.vy_jm2 = vs[s0][1][row - 2][col]; v.vy_jm1 = vs[s0][1][row - 1][col];
+            v.vy_jp1 = vs[s0][1][row + 1][col];
+            v.vy_im1 = vs[s0][1][row][col - 1]; v.vy_ip1 = vs[s0][1][row][col + 1];
+            v.vy_ip2 = vs[s0][1][row][col + 2];
+            v.vz_im1 = vs[s0][2][row][col - 1]; v.vz_ip1 = vs[s0][2][row][col + 1];
+            v.vz_ip2 = vs[s0][2][row][col + 2];
+            v.vz_jm1 = vs[s0][2][row - 1][col]; v.vz_jp1 = vs[s0][2][row + 1][col];
+            v.vz_jp2 = vs[s0][2][row + 2][col];
+            const long long rr = ((long long)(k - g.kb) * g.nyk + (js - g.jb)) * g.rpitch + (is - 1);
+            stress_cell<NM, true, true>(a, p, rr, in_band(k, sb[s][0], sb[s][1], sb[s][2], sb[s][3]), v, mu00[s],
+                                        mu00n, mu10[s], mu10n, mu01[s], mu01n, mu11[s]);
+            mu00[s] = mu00n; mu10[s] = mu10n; mu01[s] = mu01n;
+            mu11[s] = mu[p + P + 1 + Y];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Side columns (kbeg_a = 1, m_global.f90:334-343): y slabs as full-row tiles,
 // x slabs of interior rows as tiles na columns wide with several rows per
 // wave (tiles from a host list).  Each lane marches its column through the
@@ -927,6 +1138,10 @@ hipError_t launch_stress_cols(swpc_ctx *c, int mask)
 {
     StepArgs a;
     step_args(c, a, mask);
+    if (mask == MASK_PML) {   // the bottom band alone: only the chunks that reach below the interior box
+        a.zc += c->zband[0];
+        a.nbz -= c->zband[0];
+    }
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
     dim3 block(a.bxw, a.byi), grid(a.nbx * a.nby * a.nbz);
     const bool nt = (c->variant & 1) != 0;
@@ -951,6 +1166,10 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
     StepArgs a;
     step_args(c, a, mask);
     a.zc = c->d_zchunks + c->nbz + 1;
+    if (mask == MASK_PML) {
+        a.zc += c->zband[1];
+        a.nbz -= c->zband[1];
+    }
     if (c->g.nxk <= 0 || c->g.nyk <= 0 || !((mask & MASK_INT) || a.has_pml)) return hipSuccess;
     const dim3 grid(a.nbx * a.nby * a.nbz), block(a.bxw, a.byi);
     const bool pf = (c->variant & 8) != 0;
@@ -961,6 +1180,34 @@ hipError_t launch_vel_cols(swpc_ctx *c, int mask)
         if (pf) hipLaunchKernelGGL((k_vel_int<false, true>), grid, block, 0, c->stream, a);
         else hipLaunchKernelGGL((k_vel_int<false, false>), grid, block, 0, c->stream, a);
     }
+    return hipGetLastError();
+}
+
+// the fused interior launch: c->fields holds the new fields, c->fields_in
+// the old ones; tiles of FW x 4 sr columns (sr = 2 with variant bit 8)
+hipError_t launch_fused(swpc_ctx *c)
+{
+    StepArgs a;
+    step_args(c, a, MASK_INT);
+    const Grid &g = c->g;
+    if (g.nxk <= 0 || g.nyk <= 0 || c->nfz <= 0 || !c->fields_in[0]) return hipErrorInvalidValue;
+    const int sr = (c->variant & 256) ? 2 : 1;
+    a.zc = c->d_fzc;
+    a.nbx = (g.nxk + FW - 1) / FW;
+    a.nby = (g.nyk + 4 * sr - 1) / (4 * sr);
+    a.nbz = c->nfz;
+    const dim3 grid(a.nbx * a.nby * a.nbz), block(BX, 4);
+#define SWPC_FUSED_CASE(NMV)                                                                          \
+    case NMV:                                                                                         \
+        if (sr == 2) hipLaunchKernelGGL((k_fused<NMV, 2>), grid, block, 0, c->stream, a);              \
+        else hipLaunchKernelGGL((k_fused<NMV, 1>), grid, block, 0, c->stream, a);                      \
+        break;
+    switch (g.nm) {
+        SWPC_FUSED_CASE(0) SWPC_FUSED_CASE(1) SWPC_FUSED_CASE(2) SWPC_FUSED_CASE(3) SWPC_FUSED_CASE(4)
+        SWPC_FUSED_CASE(5) SWPC_FUSED_CASE(6) SWPC_FUSED_CASE(7) SWPC_FUSED_CASE(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef SWPC_FUSED_CASE
     return hipGetLastError();
 }
 

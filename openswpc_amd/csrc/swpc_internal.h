@@ -192,12 +192,18 @@ struct swpc_ctx {
     int kchunk, by;
     int nbz;           // k chunks of the interior-column kernels
     int *d_zchunks;    // first plane of each chunk, stress phase then velocity phase (2 x (nbz+1))
+    int zband[2];      // per phase, the first chunk that holds planes of the bottom PML band
+    int *d_fzc;        // chunks of the fused launch over the interior planes (nfz + 1 entries)
+    int nfz;
+    float *fields2[SWPC_NFIELD];  // second field set of the fused steps (variant bit 7)
     int4 *d_sched[2];  // merged interior + side-column schedules of the two phases (variant bit 5)
     int nsched[2];
     int variant;       // bit 0: non-temporal single-use streams in the interior stress kernel,
                        // bit 1: side-column PML kernels on a second stream,
                        // bit 2: 32-wide interior tiles, bit 3: velocity loads one plane ahead,
-                       // bit 4: adjoint records copied, bit 5: interior and side columns in one launch
+                       // bit 4: adjoint records copied, bit 5: interior and side columns in one launch,
+                       // bit 7: the velocity update fused to the next stress update (k_fused, two field
+                       // sets), bit 8: fused tiles of 8 rows instead of 4
     long long bytes;
     int own_arena;     // the context reserved the device's striped arena itself (swpc_create)
     struct AdjWorkspace *adj_ws;  // adjoint state, gradient sums and checkpoints (adjoint.hip)
@@ -236,6 +242,7 @@ bool merged_launch_ready(const swpc_ctx *c);
 hipError_t launch_stress_all(swpc_ctx *c);
 hipError_t launch_vel_all(swpc_ctx *c);
 hipError_t launch_vel_cols(swpc_ctx *c, int mask);
+hipError_t launch_fused(swpc_ctx *c);
 hipError_t launch_kernel_stress(swpc_ctx *c);
 hipError_t launch_kernel_vel(swpc_ctx *c);
 hipError_t launch_pml_stress(swpc_ctx *c);

@@ -552,12 +552,14 @@ def stripe_free_bytes(device=0, region=0):
     return int(load_library().swpc_stripe_free_bytes(int(device), int(region)))
 
 
-def stripe_bytes_per_region(nx, ny, nz, nm, na=0, nregions=HBM_REGIONS, margin=1.05):
+def stripe_bytes_per_region(nx, ny, nz, nm, na=0, nregions=HBM_REGIONS, margin=1.05, field_sets=1):
     """Arena bytes per region that hold one context's per-cell arrays:
     9 fields and 5 medium arrays of (nx+6+32-aligned) x (ny+6) x (nz+6)
     floats, 6*nm memory variables per interior cell, 18 auxiliaries per PML
     cell; the largest region share, with 256 MiB granules and a 5 % margin
-    (add a granule or two of slack for a single context)."""
+    (add a granule or two of slack for a single context).  field_sets=2
+    counts the second set of 9 fields that the fused steps (variant bit 7)
+    hold."""
     pitch = -(-(32 + nx + 3) // 32) * 32
     n = pitch * (ny + 6) * (nz + 6) * 4
     rn = -(-nx // 32) * 32 * max(ny - 2 * na, 0) * max(nz - na, 0) * 4
@@ -567,7 +569,7 @@ def stripe_bytes_per_region(nx, ny, nz, nm, na=0, nregions=HBM_REGIONS, margin=1
     up = lambda b: -(-int(b) // gran) * gran if b >= gran else -(-int(b) // 65536) * 65536  # noqa: E731
     per = [0] * nregions
     # fields and medium: 14 arrays; regions as the library assigns them
-    for r in (0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 0, 1, 2, 2):
+    for r in (0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 0, 1, 2, 2) + (0, 1, 2) * 3 * (field_sets - 1):
         per[r % nregions] += up(n)
     npack = (6 * nm + 3) // 4
     if nm > 0:
