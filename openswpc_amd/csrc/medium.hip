@@ -213,6 +213,18 @@ __global__ void __launch_bounds__(256) k_medium_vrange(Grid g, const float *rho,
     }
 }
 
+// the two surface maps in the host layout, i fastest over -2..n+3, rows j:
+// one contiguous block the host copies back in a single transfer
+__global__ void __launch_bounds__(256) k_medium_maps_out(Grid g, const int *kfs, const int *kob, int *out)
+{
+    const int i = -2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = -2 + (int)blockIdx.y;
+    if (i > g.nx + 3 || j > g.ny + 3) return;
+    const long long w = g.nx + 6, q = (long long)(j + 2) * w + (i + 2);
+    out[q] = kfs[g2idx(g, i, j)];
+    out[(long long)(g.ny + 6) * w + q] = kob[g2idx(g, i, j)];
+}
+
 __global__ void k_medium_ext_init(unsigned int *ext)
 {
     if (threadIdx.x == 0) {
@@ -241,9 +253,11 @@ extern "C" int swpc_medium_layered(swpc_ctx *c, const float *prof, const float *
     if (hipSetDevice(c->device) != hipSuccess) return ctx_fail(c, SWPC_EHIP, "swpc_medium_layered: hipSetDevice");
     hipStream_t s = c->stream;
     const int nzr = g.nz + 6;
-    DevBuf dprof, dmaps, dext;
+    DevBuf dprof, dmaps, dext, dout;
+    const size_t nmap = (size_t)(g.nx + 6) * (g.ny + 6);
     hipError_t e = hipMalloc(&dprof.p, (size_t)5 * nzr * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&dmaps.p, (size_t)2 * g.plane * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dout.p, 2 * nmap * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&dext.p, 4 * sizeof(unsigned int));
     if (e != hipSuccess) return ctx_fail(c, SWPC_ENOMEM, "swpc_medium_layered: device memory for the set-up");
     e = hipMemcpyAsync(dprof.p, prof, (size_t)5 * nzr * sizeof(float), hipMemcpyHostToDevice, s);
@@ -287,10 +301,11 @@ extern "C" int swpc_medium_layered(swpc_ctx *c, const float *prof, const float *
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(c->kob, kob_tmp, (size_t)g.plane * sizeof(int), hipMemcpyDeviceToDevice, s);
     // the maps back in the host layout (i fastest over -2..n+3, rows j)
-    const size_t w = (size_t)(g.nx + 6) * sizeof(int), sp = (size_t)g.pitch * sizeof(int);
-    const long long o0 = g2idx(g, -2, -2);
-    if (e == hipSuccess) e = hipMemcpy2DAsync(kfs_out, w, kfs + o0, sp, w, g.ny + 6, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpy2DAsync(kob_out, w, kob_tmp + o0, sp, w, g.ny + 6, hipMemcpyDeviceToHost, s);
+    hipLaunchKernelGGL(k_medium_maps_out, cols, dim3(64), 0, s, g, kfs, kob_tmp, (int *)dout.p);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(kfs_out, dout.p, nmap * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(kob_out, (int *)dout.p + nmap, nmap * sizeof(int), hipMemcpyDeviceToHost, s);
     unsigned int ext[4] = {0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(ext, dext.p, 3 * sizeof(unsigned int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
