@@ -141,3 +141,40 @@ def test_fused_equals_phase_launches_on_a_larger_grid():
     for o in out[1:]:
         for n, a in out[0].items():
             assert ulp_diff(a, o[n]) == 0, n
+
+
+def test_fused_cerjan_keeps_the_phase_launches(orc):
+    """the fused launch needs the PML: a Cerjan run with bit 7 holds no second
+    field set and steps as before"""
+    m = O.build_model(small_config(abc="cerjan", nm=2))
+    run = O.OracleRun(m, 32)
+    run.run(15)
+    g = gpu_from_model(m, kchunk=8, by=4, variant=9)
+    base = g.device_bytes()
+    g.set_tuning(8, 4, FUSED)
+    assert g.device_bytes() == base
+    g.step(1, 15, use_graph=True, block=5)
+    g.synchronize()
+    _check(g, run, "cerjan, variant bit 7", kinds=("vel",))
+
+
+def test_second_field_set_comes_and_goes(orc):
+    """bit 7 allocates nine more field arrays, clearing it frees them, and
+    the state survives both changes"""
+    m = O.build_model(small_config(nm=3))
+    run = O.OracleRun(m, 32)
+    run.run(24)
+    g = gpu_from_model(m, kchunk=8, by=4, variant=9)
+    base = g.device_bytes()
+    g.step(1, 8)
+    g.set_tuning(0, 0, FUSED)
+    grown = g.device_bytes() - base
+    cfg = m.cfg
+    pitch = -(-(32 + cfg.nx + 3) // 32) * 32
+    assert grown == 9 * pitch * (cfg.ny + 6) * (cfg.nz + 6) * 4, grown
+    g.step(9, 8, use_graph=True, block=4)
+    g.set_tuning(0, 0, 9)
+    assert g.device_bytes() == base
+    g.step(17, 8)
+    g.synchronize()
+    _check(g, run, "bit 7 set and cleared mid-run")
