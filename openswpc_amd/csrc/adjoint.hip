@@ -205,9 +205,10 @@ __device__ static inline float4 prof(const float4 *p, int n) { return p[n - 1]; 
 // halves the occupancy; keeping whole planes on one XCD overflows its L2
 // with the z neighbours of nine density arrays.  Both measured slower.
 // Giving each XCD a contiguous eighth of a plane's tiles keeps halo rows in
-// one L2 but puts the absorber rows, the costliest, on two XCDs: 2.79
-// against 2.84 ms per gradient step at 256 x 256 x 192, 10.04 against 9.93
-// ms per source-step at 384 x 384 x 256, NM = 3.)
+// one L2: 2.79 against 2.84 ms per gradient step at 256 x 256 x 192, but
+// 10.04 against 9.93 ms per source-step at 384 x 384 x 256, NM = 3.  With
+// the absorber tile rows dealt out one tile per XCD and equal runs of the
+// rest, the same: 2.83 against 2.88, 10.08 against 9.89.)
 #define ADJ_CELL(a)                                                \
     const Grid &g = (a).g;                                         \
     const int nbx_ = (g.nx + 63) >> 6, nby_ = (g.ny + 3) >> 2;     \
