@@ -2,7 +2,7 @@
 
 The library evaluates every expression in the reference's FP32 order with
 IEEE division and no FMA contraction, so the target is bit-for-bit equality
-(0 ulp) with the oracle's FP32 variant.  The north-star tolerance of
+(0 ulp) with the oracle's FP32 variant.  The accuracy bar of
 relative L2 <= 1e-5 is asserted as well, so a failure of the stronger
 property still reports how far apart the results are.
 """
