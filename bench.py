@@ -127,7 +127,7 @@ def measure_batch(steps, warmup, block_req):
     # steps), each with the plain defaults rather than the batch's tuned shape
     b.close()
     for s, sp in zip(sims, specs):
-        s.set_tuning(16 if sp.nm else 32, 4, 1 | 8)
+        s.set_tuning(-1, 4, 1 | 8)
     for s in sims:
         s.step(it, block, use_graph=True, block=block)
     for s in sims:

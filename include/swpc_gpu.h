@@ -402,7 +402,8 @@ int swpc_stripe_release(int device);
 int64_t swpc_stripe_free_bytes(int device, int region);
 
 /* Launch-shape tuning knobs.  kchunk: k-planes marched per thread block
-   (0 = keep; default 16, 32 when nm = 0); by: rows per 64-wide tile, 1, 2
+   (0 = keep, -1 = the default: 16, 32 when nm = 0, halved on small grids
+   until the interior launch has two blocks per CU); by: rows per 64-wide tile, 1, 2
    or 4 (0 = keep; default 4); variant: kernel variant flags (-1 = keep;
    default 9): bit 0 = non-temporal loads and
    stores for the single-use streams of the stress update, bit 1 = side-column

@@ -226,6 +226,28 @@ extern "C" void *swpc_staging_buffer(swpc_ctx *ctx, int64_t *nfloats)
     return ctx ? (void *)ctx->staging : nullptr;
 }
 
+// The default k chunk of a context.  16 planes was fastest on 1024x1024x512
+// with NM = 3 among 8..128 (the stress and side-column launches favour short
+// chunks); an elastic update streams far fewer arrays, so each chunk's
+// priming loads weigh more and 32 planes is faster there (256x256x192,
+// NM = 0).  Small grids halve it until the interior launch has two blocks per
+// CU: 128^3 with a 20-cell PML took 0.233 ms/step in 352 blocks of 16 planes
+// and 0.161 in 704 blocks of 8.
+static int default_kchunk(const swpc_ctx *c)
+{
+    const Grid &g = c->g;
+    int kc = g.nm > 0 ? 16 : 32;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0) {
+        (void)hipGetLastError();
+        ncu = 256;
+    }
+    const long long tiles =
+        g.nxk > 0 && g.nyk > 0 ? (long long)((g.ie - 1) / 64 - (g.ib - 1) / 64 + 1) * ((g.nyk + 3) / 4) : 0;
+    while (tiles > 0 && kc > 4 && tiles * ((g.nz + kc - 1) / kc) < 2LL * ncu) kc /= 2;
+    return kc;
+}
+
 extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
 {
     if (!d || !out) return SWPC_EINVAL;
@@ -240,11 +262,6 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     memset(c, 0, sizeof(*c));
     c->desc = *d;
     c->device = d->device;
-    // k chunk: 16 planes was fastest on 1024x1024x512 with NM = 3 among 8..128
-    // (the stress and side-column launches favour short chunks); an elastic
-    // update streams far fewer arrays, so each chunk's priming loads weigh
-    // more and 32 planes is faster there (256x256x192, NM = 0)
-    c->kchunk = d->nm > 0 ? 16 : 32;
     c->by = 4;
     c->variant = 1 | 8;
     if (hipSetDevice(d->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
@@ -273,6 +290,7 @@ extern "C" int swpc_create(const swpc_desc *d, swpc_ctx **out)
     }
     g.nxk = g.ie - g.ib + 1;
     g.nyk = g.je - g.jb + 1;
+    c->kchunk = default_kchunk(c);
     g.nzk = g.ke - g.kb + 1;
     // memory variables keep full-width rows (i = 1 at offset 0) so that they
     // align with the field rows; only interior cells are ever touched
@@ -1281,13 +1299,14 @@ extern "C" int swpc_synchronize(swpc_ctx *c)
 extern "C" int swpc_set_tuning(swpc_ctx *c, int kchunk, int by, int variant)
 {
     if (!c) return SWPC_EINVAL;
-    if (kchunk < 0 || by < 0 || by == 3 || by > 4)
-        return fail(c, SWPC_EINVAL, "swpc_set_tuning: kchunk >= 0, by in {1, 2, 4} (launch bound 256)");
+    if (kchunk < -1 || by < 0 || by == 3 || by > 4)
+        return fail(c, SWPC_EINVAL, "swpc_set_tuning: kchunk >= -1, by in {1, 2, 4} (launch bound 256)");
     HIPCHK(c, hipSetDevice(c->device));
     // the tables rebuilt below may be read by work on any stream of the
     // device (the context's own, a batch's that steps this member)
     HIPCHK(c, hipDeviceSynchronize());
     if (kchunk > 0) c->kchunk = kchunk;
+    else if (kchunk == -1) c->kchunk = default_kchunk(c);
     if (by > 0) c->by = by;
     if (variant >= 0) c->variant = variant;
     reset_graph(c);
