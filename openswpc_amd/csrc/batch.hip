@@ -471,7 +471,7 @@ extern "C" int swpc_batch_tune(swpc_batch *b, int trial_steps, int *kchunk_out, 
     BCHK(b, hipSetDevice(b->device));
     int rc;
     if ((rc = batch_sync_members(b))) return rc;
-    const int kc_cand[] = {12, 16, 24, 32, 48, 64};
+    const int kc_cand[] = {8, 12, 16, 24, 32, 48, 64};
     const int by_cand[] = {2, 4};
     const int w_cand[] = {64, 32};
     const int pk_cand[] = {0, 1, 2};
